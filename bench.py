@@ -1,0 +1,164 @@
+"""Benchmark: converged eigenpairs/s (FP64 true residual <= 1e-8) of the
+three-fluxonia flat grid (three PhaseGrid(-6pi, 6pi, 512; order=2) modes,
+N = 512^3 = 134 217 728 real states, Lanczos lowest 10) on one MI355X, plus the
+fused apply's HBM bandwidth against peak.
+
+A step is one full Lanczos solve (operator construction, the solve, the FP64
+acceptance residuals).  Prints ONE JSON line (rank 0)."""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "kronfreecircuits.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def apply_bandwidth(H, prec, reps=20):
+    """Average fused-apply duration from HIP events on the library's stream."""
+    import torch
+    dt = torch.float64 if prec == "f64" else torch.float32
+    X = torch.randn(H.N, dtype=dt, device="cuda:0")
+    Y = torch.empty_like(X)
+    torch.cuda.synchronize()
+    s = torch.cuda.ExternalStream(H.ctx.stream)
+    for _ in range(3):
+        H.mul(Y, X, prec)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        H.mul(Y, X, prec)
+    e1.record(s)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    nbytes = 2 * H.N * (8 if prec == "f64" else 4)
+    del X, Y
+    return t, nbytes / t / 1e9
+
+
+def cpu_baseline(theta, N, n_matvec_gpu, m):
+    """The oracle's per-term chain apply (the reference's algorithm restated,
+    src/apply.jl:370-432) timed on this host; a bounded sample of applies at the
+    full size, converted to eigenpairs/s with the GPU solve's matvec count."""
+    import numpy as np
+
+    from oracle import kfc_oracle as O
+    Ho = O.three_fluxonia_operator(theta["E_C"], theta["E_L"], theta["E_J"], theta["phi_ext"], N=N, order=2)
+    x = np.random.default_rng(0).standard_normal(Ho.N)
+    t0 = time.perf_counter()
+    napp = 0
+    while True:
+        O.apply_chain(Ho, x)
+        napp += 1
+        if time.perf_counter() - t0 > 10 or napp >= 3:
+            break
+    t = (time.perf_counter() - t0) / napp
+    return {"value": m / (n_matvec_gpu * t), "unit": "eigenpairs/s", "cores": 1, "kind": "port",
+            "sample": f"{napp} numpy oracle applies (per-term two-buffer chain, 1 thread) at N={N}^3, "
+                      f"{t:.2f} s each, scaled by the GPU solve's {n_matvec_gpu} matvecs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--N", type=int, default=512)
+    ap.add_argument("--mode", default="mixed", choices=["f64", "mixed"])
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import kfc
+    from kfc import configs
+
+    theta = configs.three_fluxonia_theta(args.seed)
+    m = 10
+    log(f"[bench] three fluxonia N={args.N}^3 theta={ {k: [round(float(x), 4) for x in v] for k, v in theta.items()} }")
+
+    def step(mode):
+        H = configs.three_fluxonia_operator(theta, N=args.N)
+        E, V, st = kfc.solve_stats(kfc.Lanczos(tol=args.tol, mode=mode), H, m)
+        return H, E, st
+
+    # apply bandwidth (the dominant kernel)
+    H = configs.three_fluxonia_operator(theta, N=args.N)
+    assert H.fused
+    t64, gbs64 = apply_bandwidth(H, "f64")
+    t32, gbs32 = apply_bandwidth(H, "f32")
+    log(f"[bench] fused apply: f64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s | f32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
+    del H
+    torch.cuda.empty_cache()
+
+    for i in range(args.warmup):
+        _, E, st = step(args.mode)
+        log(f"[bench] warmup {i}: {st['seconds_total']:.2f} s, matvec {st['n_matvec']}, resid {st['max_residual']:.2e}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    counted, stats = 0, []
+    for i in range(args.steps):
+        _, E, st = step(args.mode)
+        counted += m if st["max_residual"] <= 1e-8 else 0
+        stats.append(st)
+        log(f"[bench] step {i}: {st['seconds_total']:.2f} s, matvec {st['n_matvec']} (f32 {st['n_matvec_f32']}), "
+            f"resid {st['max_residual']:.2e}, E0 {E[0]:.10f}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = counted / wall
+    st = stats[-1]
+    apply_calls = st["n_apply_calls"]
+    # roofline of the dominant kernel: live HIP-event time of the solve's applies
+    avg_apply = st["seconds_apply_gpu"] / max(1, apply_calls)
+    prec_bytes = 4 if args.mode == "mixed" else 8
+    bytes_per_launch = 2 * (args.N ** 3) * prec_bytes
+    achieved = bytes_per_launch / avg_apply / 1e9 if args.mode == "f64" else gbs32
+    roof = {"bound": "hbm", "achieved": round(gbs64 if args.mode == "f64" else gbs32, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round((gbs64 if args.mode == "f64" else gbs32) / HBM_PEAK_GBS, 4),
+            "traffic": None, "kernel": "k_stencil", "in_solve_avg_apply_us": round(avg_apply * 1e6, 1),
+            "in_solve_achieved_gbs": round(achieved, 1)}
+    extra = {}
+    if not args.skip_plain and args.mode != "f64":
+        _, E64, st64 = step("f64")
+        extra["plain_fp64"] = {"seconds": round(st64["seconds_total"], 3), "n_matvec": st64["n_matvec"],
+                               "max_residual": st64["max_residual"],
+                               "eigenpairs_per_s": round(m / st64["seconds_total"], 4),
+                               "max_abs_dE_vs_mixed": float(np.max(np.abs(E64 - E)))}
+        log(f"[bench] plain f64: {st64['seconds_total']:.2f} s matvec {st64['n_matvec']}")
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(theta, args.N, st["n_matvec"], m)
+    out = {"metric": "eigenpairs/s at resid<=1e-8 (three fluxonia 512^3, Lanczos lowest 10)",
+           "value": round(value, 5), "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.mode == "mixed" else "f64",
+           "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "three_fluxonia_flat", "dims": [args.N] * 3, "order": 2, "m": m,
+                      "solver": "Lanczos", "mode": args.mode, "tol": args.tol, "seed": args.seed},
+           "roofline": roof, "cpu_baseline": cpu,
+           "apply": {"f64_us": round(t64 * 1e6, 1), "f64_gbs": round(gbs64, 1), "f32_us": round(t32 * 1e6, 1),
+                     "f32_gbs": round(gbs32, 1)},
+           "solve": {"n_matvec": st["n_matvec"], "n_matvec_f32": st["n_matvec_f32"], "restarts": st["iterations"],
+                     "seconds_apply_gpu": round(st["seconds_apply_gpu"], 3), "max_residual": st["max_residual"]},
+           **extra}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+/* kfc.h — C ABI of libkfc.so, the MI355X (gfx950) Kronecker-free apply and
+ * eigensolver library.
+ *
+ * Every entry point takes plain pointers and sizes.  Host descriptors are
+ * copied to the device at creation; device buffers passed to kfc_apply are
+ * owned by the caller; the library never frees caller memory.  Status codes
+ * are returned as int (KFC_OK == 0); kfc_last_error(ctx) holds the message of
+ * the last failure on that context.
+ *
+ * Each declaration names the KronFreeCircuits.jl interface it replaces
+ * (file:line in the reference repository).
+ */
+#ifndef KFC_H
+#define KFC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the error classes the reference raises) ----------------------- */
+enum kfc_status {
+    KFC_OK = 0,
+    KFC_EARG = 1,        /* ArgumentError (src/lobpcg.jl:224-233, src/apply.jl:225-231) */
+    KFC_EDIM = 2,        /* DimensionMismatch (src/apply.jl:10-13, 410-411) */
+    KFC_ENOCONV = 3,     /* "did not converge" ErrorException (src/lobpcg.jl:292-294, src/solvers.jl:97-98) */
+    KFC_EBREAKDOWN = 4,  /* eigensolver breakdown (src/lobpcg.jl:195-198, src/chebyshev.jl:124-125) */
+    KFC_EDEVICE = 5,     /* HIP runtime failure */
+    KFC_ENOMEM = 6,      /* device allocation failure */
+    KFC_EUNSUPPORTED = 7 /* a combination this build does not serve */
+};
+
+/* ---- element types: the operator's type T (src/apply.jl:232-233) ---------------- */
+enum kfc_dtype { KFC_F64 = 0, KFC_C128 = 1 };
+
+/* ---- arithmetic precision of an application (FP64 or the FP32 bulk path) --------- */
+enum kfc_precision { KFC_PREC_F64 = 0, KFC_PREC_F32 = 1 };
+
+/* ---- solver precision mode ------------------------------------------------------- */
+enum kfc_mode {
+    KFC_MODE_F64 = 0,    /* every apply and reduction in FP64 (the plain path) */
+    KFC_MODE_MIXED = 1   /* FP32 applies for the bulk iteration, FP64 Rayleigh-Ritz,
+                            reorthogonalization and a final FP64 refinement */
+};
+
+/* ---- local operator kinds (src/localops.jl:18-73) -------------------------------- */
+enum kfc_kind { KFC_DIAG = 0, KFC_BANDED = 1, KFC_SHIFT = 2, KFC_DENSE = 3 };
+
+/* One factor (axis, op) of an AxisTerm (src/apply.jl:157-166).  Complex data is
+ * interleaved (re, im) doubles; real data is plain doubles. */
+typedef struct {
+    int32_t axis;            /* 0-based axis, strictly increasing within a term */
+    int32_t kind;            /* enum kfc_kind */
+    int64_t d;               /* dimension of the op == dims[axis] */
+    int32_t is_complex;      /* 1: the data arrays below are interleaved complex */
+    int32_t nbands;          /* BANDED: number of bands */
+    const int64_t* offsets;  /* BANDED: nbands offsets o_b, Y[i] += w_b[i] X[i+o_b] */
+    const double* weights;   /* BANDED: d x nbands, column-major (src/apply.jl:82-88) */
+    int64_t shift;           /* SHIFT: s, Y[i+s] = f X[i] */
+    double factor_re, factor_im; /* SHIFT: f */
+    const double* matrix;    /* DENSE: d x d column-major */
+    const double* diag;      /* DIAG: d entries */
+} kfc_factor_desc;
+
+/* AxisTerm(coeff, factors) (src/apply.jl:157-166). */
+typedef struct {
+    double coeff_re, coeff_im;
+    int32_t nfactors;
+    const kfc_factor_desc* factors;
+} kfc_term_desc;
+
+/* KronFreeOperator(dims, terms; constant, block_capacity, hermitian)
+ * (src/apply.jl:222-253).  Terms whose factors are all DIAG (or none) are fused
+ * with the constant into the diagonal, the rest are kept, as the reference does. */
+typedef struct {
+    int32_t ndims;
+    const int64_t* dims;     /* axis 0 fastest (column-major, src/apply.jl:18-19) */
+    int32_t dtype;           /* enum kfc_dtype: KFC_F64 on a real node, KFC_C128 otherwise */
+    double constant_re, constant_im;
+    int32_t nterms;
+    const kfc_term_desc* terms;
+    int64_t block_capacity;  /* solver block width reserved for scratch */
+    int32_t hermitian;
+} kfc_op_desc;
+
+/* Work counters returned by the solvers; they feed node_timings
+ * (src/materialize.jl:674-684, 801-809). */
+typedef struct {
+    int64_t iterations;      /* LOBPCG/Chebyshev iterations, Lanczos restarts */
+    int64_t n_matvec;        /* H columns applied (all precisions) */
+    int64_t n_matvec_f32;    /* of which in FP32 */
+    int64_t n_apply_calls;   /* block apply launches */
+    double seconds_total;
+    double seconds_apply;     /* host wall time around the applies */
+    double seconds_apply_gpu; /* HIP-event time of the applies on the context stream */
+    double max_residual;     /* worst fresh FP64 true residual of the returned pairs */
+    int32_t n_converged;
+    int32_t warm;            /* 1 when an X0 was used */
+    int32_t degree_last;     /* Chebyshev: degree of the last filter */
+    int32_t retried;         /* LOBPCG: 1 when the random-block retry ran */
+} kfc_stats;
+
+typedef struct kfc_ctx kfc_ctx;
+typedef struct kfc_op kfc_op;
+
+/* ---- context (device + stream) ---------------------------------------------------- */
+int kfc_ctx_create(kfc_ctx** ctx, int device);
+int kfc_ctx_destroy(kfc_ctx* ctx);
+const char* kfc_last_error(const kfc_ctx* ctx);
+/* The HIP stream every call of this context runs on (a hipStream_t). */
+void* kfc_ctx_stream(kfc_ctx* ctx);
+
+/* ---- operator ------------------------------------------------------------------- */
+/* Replaces KronFreeOperator(dims, terms; constant, block_capacity, on)
+ * (src/apply.jl:222-253) with on = a device array type, and _on_storage
+ * (src/apply.jl:258-267): host data copied to the device once. */
+int kfc_op_create(kfc_ctx* ctx, const kfc_op_desc* desc, kfc_op** op);
+int kfc_op_destroy(kfc_op* op);
+/* prod(dims) */
+int64_t kfc_op_size(const kfc_op* op);
+/* 1 when every application of op runs as one fused stencil pass */
+int kfc_op_fused(const kfc_op* op);
+
+/* Replaces LinearAlgebra.mul!(y, H, x) (src/apply.jl:416-432):
+ * Y = H X for ncols columns with leading dimension ld (>= prod(dims)).
+ * X, Y are device pointers of FP64 (or interleaved complex128) when precision is
+ * KFC_PREC_F64, FP32 (complex64) when KFC_PREC_F32.  Y must not alias X
+ * (ArgumentError, src/apply.jl:412).  Enqueued on the context stream; returns
+ * after the launch (kfc_sync waits). */
+int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision);
+int kfc_sync(kfc_ctx* ctx);
+
+/* Replaces diagonal(H) / _full_diagonal(H) (src/apply.jl:455-478): the real
+ * diagonal of H written to host out[prod(dims)]. */
+int kfc_op_diagonal(kfc_op* op, double* out);
+/* Replaces gershgorin_bound(H) (src/apply.jl:586-593). */
+int kfc_op_gershgorin(kfc_op* op, double* out);
+/* Replaces product_seed(H, k) (src/apply.jl:655-685): k orthonormal product
+ * states into host X[prod(dims) x k] (complex interleaved for KFC_C128). */
+int kfc_op_product_seed(kfc_op* op, int64_t k, double* X);
+
+/* ---- solvers (the Solver plugin contract, src/solvers.jl:3-31) ------------------- */
+/* Each writes E[m] ascending and V[prod(dims) x m] (host, column-major,
+ * complex interleaved for KFC_C128) and fills stats (may be NULL). */
+
+/* Replaces lanczos_solve(::Lanczos, H, m) (src/solvers.jl:85-106):
+ * thick-restart Lanczos, krylovdim = max(2m+10, krylovdim), residual tol. */
+int kfc_lanczos(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxiter, uint64_t seed,
+                int32_t mode, double* E, double* V, kfc_stats* stats);
+
+/* Replaces lobpcg_solve(::LOBPCG, H, m; X0) (src/lobpcg.jl:217-301).
+ * precond: 0 = :ritz, 1 = :jacobi, 2 = :none; seed_kind: 0 = :product, 1 = :random.
+ * X0: NULL or host prod(dims) x m (the warm start, src/materialize.jl:656-666). */
+enum kfc_precond { KFC_PRECOND_RITZ = 0, KFC_PRECOND_JACOBI = 1, KFC_PRECOND_NONE = 2 };
+enum kfc_seed { KFC_SEED_PRODUCT = 0, KFC_SEED_RANDOM = 1 };
+int kfc_lobpcg(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond,
+               double eta, int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode,
+               double* E, double* V, kfc_stats* stats);
+
+/* Replaces solve(::ChebyshevFilter, H, m) (src/chebyshev.jl:310-406).
+ * ortho: 0 = :svqb, 1 = :cholqr; bound: a finite number, or NaN for :gershgorin. */
+int kfc_chebyshev(kfc_op* op, int64_t m, int32_t degree, int64_t extra, int64_t maxiter,
+                  int32_t rescale_every, int32_t ortho, double bound, int32_t seed_kind, double tol,
+                  uint64_t seed, int32_t mode, double* E, double* V, kfc_stats* stats);
+
+/* Replaces solve(::Dense, H, m) (src/solvers.jl:44-50): Matrix(H) by applying H
+ * to identity blocks on the device, then a Hermitian eigendecomposition.
+ * m == 0 returns all prod(dims) pairs. */
+int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats);
+
+/* Fresh FP64 true residuals ||H v_j - E_j v_j|| of host pairs (E, V[N x m]):
+ * _worst_true_residual per column (src/solvers.jl:65-69). */
+int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid);
+
+/* ---- batched sweeps (no reference counterpart; the closest is bench/common.jl:135-163).
+ * A batch of independent two-mode instances of the coupled-transmon form on
+ * periodic grids, solved in persistent launches.  n_inst instances; per instance i:
+ * grid sizes N1[i], N2[i] and parameters P[i*8 .. i*8+7] =
+ * (E_J1, E_C1, n_g1, E_J2, E_C2, n_g2, g, order).  Each leaf keeps k_leaf states,
+ * the top keeps k_top: E_out[i*k_top ..], resid_out[i*k_top ..] (FP64 true
+ * residuals of the top pairs). */
+int kfc_batch_transmons(kfc_ctx* ctx, int64_t n_inst, const int64_t* N1, const int64_t* N2, const double* P,
+                        int32_t k_leaf, int32_t k_top, double tol, int32_t mode, double* E_out,
+                        double* resid_out, kfc_stats* stats);
+
+/* ---- learned warm start (no reference counterpart; feeds X0 of kfc_lobpcg).
+ * A 6 -> 256 -> 256 -> 1024 GELU MLP with BF16 weights and FP32 accumulation. */
+typedef struct kfc_mlp kfc_mlp;
+int kfc_mlp_create(kfc_ctx* ctx, const int32_t* layer_dims, int32_t nlayers, uint64_t seed, kfc_mlp** mlp);
+int kfc_mlp_destroy(kfc_mlp* mlp);
+/* Host FP32 parameters in the order W1, b1, W2, b2, ... (W row-major out x in). */
+int kfc_mlp_set_params(kfc_mlp* mlp, const float* params);
+int kfc_mlp_get_params(kfc_mlp* mlp, float* params);
+int64_t kfc_mlp_num_params(const kfc_mlp* mlp);
+/* Host X[batch x in] -> host Y[batch x out]. */
+int kfc_mlp_forward(kfc_mlp* mlp, const float* X, int64_t batch, float* Y);
+/* Gradients of sum(dY * Y) into host grads (same layout as params). */
+int kfc_mlp_backward(kfc_mlp* mlp, const float* X, const float* dY, int64_t batch, float* grads);
+/* nsteps of Adam on the mean-squared error against host targets T[n x out];
+ * returns the last loss. */
+int kfc_mlp_train(kfc_mlp* mlp, const float* X, const float* T, int64_t n, int64_t batch, int64_t nsteps,
+                  float lr, uint64_t seed, float* last_loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KFC_H */

@@ -1,0 +1,684 @@
+// Kronecker-free operator application on gfx950.
+//
+// Two paths, chosen per operator at creation:
+//  * the fused stencil path: every kept term is a product of banded, shift or
+//    diagonal factors with offsets in [-1, 1] on at most three axes.  The whole
+//    H x (fused diagonal, every term, every band) runs as ONE pass over x: a
+//    64 x 4 plane tile (plus a one-point halo) is staged in LDS, each thread
+//    keeps a 3 x 3 x 3 register queue along the slowest axis and marches along
+//    it, so x is read once and y written once (2 N s bytes per column).
+//  * the chain path: any operator.  y = diag .* x, then each kept term's
+//    factors chained through two scratch buffers with the coefficient folded
+//    into the last factor's launch (which accumulates into y).
+//    This follows the reference's term chain (src/apply.jl:370-432) with the
+//    accumulate pass fused away.
+#include "kfc_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+// ------------------------------------------------------------------------------------------
+// Host: the fused diagonal (src/apply.jl:232-246, _add_term_diagonal! 438-447)
+
+static std::vector<cd> factor_diagonal(const Factor& f) {
+    // diagonal(op) (src/localops.jl:102-108)
+    std::vector<cd> d(f.d, cd(0));
+    if (f.kind == KFC_DIAG) return f.diag;
+    if (f.kind == KFC_DENSE) {
+        for (int64_t i = 0; i < f.d; ++i) d[i] = f.matrix[i + i * f.d];
+    } else if (f.kind == KFC_BANDED) {
+        for (size_t b = 0; b < f.offsets.size(); ++b)
+            if (f.offsets[b] == 0) {
+                for (int64_t i = 0; i < f.d; ++i) d[i] = f.weights[i + b * f.d];
+                break;
+            }
+    } else if (f.kind == KFC_SHIFT && f.shift == 0) {
+        for (auto& x : d) x = f.factor;
+    }
+    return d;
+}
+
+int op_host_diag(kfc_op* op) {
+    if (!op->hdiag.empty() || op->N == 0) return KFC_OK;
+    const int K = op->ndims;
+    std::vector<int64_t> stride(K, 1);
+    for (int a = 1; a < K; ++a) stride[a] = stride[a - 1] * op->dims[a - 1];
+    op->hdiag.assign(op->N, op->constant);
+    for (const Term& t : op->diagterms) {
+        if (t.factors.empty()) continue;   // the constant is already in
+        std::vector<std::vector<cd>> dv;
+        for (const Factor& f : t.factors) dv.push_back(factor_diagonal(f));
+        for (int64_t idx = 0; idx < op->N; ++idx) {
+            cd v = t.coeff;
+            for (size_t k = 0; k < t.factors.size(); ++k) {
+                int a = t.factors[k].axis;
+                v *= dv[k][(idx / stride[a]) % op->dims[a]];
+            }
+            op->hdiag[idx] += v;
+        }
+    }
+    return KFC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Host: the fused stencil plan
+
+namespace {
+
+struct Entry {
+    cd c;
+    int off[3];
+    std::vector<cd> a[3];   // per kernel axis; empty = ones
+};
+
+// A band along one operator axis: offset and the weight vector indexed by the
+// output row.  Rows whose source i+o leaves 0..d-1 read zero padding in the
+// fused kernel, so their weights are set to a valid row's value (this keeps
+// constant bands constant).
+struct Band {
+    int64_t off;
+    std::vector<cd> w;
+};
+
+bool factor_bands(const Factor& f, std::vector<Band>& out) {
+    const int64_t d = f.d;
+    auto push = [&](int64_t o, std::vector<cd> w) {
+        int64_t lo = std::max<int64_t>(0, -o), hi = std::min<int64_t>(d, d - o);
+        if (lo >= hi) return;   // no valid row: the band contributes nothing
+        for (int64_t i = 0; i < lo; ++i) w[i] = w[lo];
+        for (int64_t i = hi; i < d; ++i) w[i] = w[hi - 1];
+        out.push_back({o, std::move(w)});
+    };
+    if (f.kind == KFC_DIAG) {
+        push(0, f.diag);
+    } else if (f.kind == KFC_BANDED) {
+        for (size_t b = 0; b < f.offsets.size(); ++b) {
+            std::vector<cd> w(f.weights.begin() + b * d, f.weights.begin() + (b + 1) * d);
+            push(f.offsets[b], std::move(w));
+        }
+    } else if (f.kind == KFC_SHIFT) {
+        // Y[t] = f X[t - s]  (src/apply.jl:111-118): offset -s, constant weight
+        push(-f.shift, std::vector<cd>(d, f.factor));
+    } else {
+        return false;
+    }
+    for (const Band& b : out)
+        if (b.off < -1 || b.off > 1) return false;
+    return true;
+}
+
+bool is_const(const std::vector<cd>& v, cd& c) {
+    if (v.empty()) { c = 1; return true; }
+    for (const cd& x : v)
+        if (x != v[0]) return false;
+    c = v[0];
+    return true;
+}
+
+void normalize(Entry& e) {
+    for (int k = 0; k < 3; ++k) {
+        cd c;
+        if (!e.a[k].empty() && is_const(e.a[k], c)) {
+            e.c *= c;
+            e.a[k].clear();
+        }
+    }
+}
+
+bool try_merge(Entry& x, const Entry& y) {
+    int diff = -1, ndiff = 0;
+    for (int k = 0; k < 3; ++k)
+        if (x.a[k] != y.a[k]) { diff = k; ++ndiff; }
+    if (ndiff == 0) { x.c += y.c; return true; }
+    if (ndiff > 1) return false;
+    // x.c * ... * a + y.c * ... * b  ==  ... * (x.c a + y.c b)
+    const int64_t n = std::max(x.a[diff].size(), y.a[diff].size());
+    std::vector<cd> m(n);
+    for (int64_t i = 0; i < n; ++i) {
+        cd xa = x.a[diff].empty() ? cd(1) : x.a[diff][i];
+        cd ya = y.a[diff].empty() ? cd(1) : y.a[diff][i];
+        m[i] = x.c * xa + y.c * ya;
+    }
+    x.c = 1;
+    x.a[diff] = std::move(m);
+    normalize(x);
+    return true;
+}
+
+}  // namespace
+
+int op_build_plan(kfc_op* op) {
+    StencilPlan& P = op->plan;
+    P = StencilPlan();
+    const int K = op->ndims;
+    if (K < 1 || K > 3 || op->N == 0) return KFC_OK;
+    // kernel axes: lanes along operator axis 0, march along the last operator axis
+    int kaxis_of[3] = {0, 1, 2};
+    if (K == 3) { P.kaxis[0] = 0; P.kaxis[1] = 1; P.kaxis[2] = 2; }
+    if (K == 2) { P.kaxis[0] = 0; P.kaxis[1] = -1; P.kaxis[2] = 1; kaxis_of[1] = 2; }
+    if (K == 1) { P.kaxis[0] = 0; P.kaxis[1] = -1; P.kaxis[2] = -1; }
+    for (int k = 0; k < 3; ++k) P.kd[k] = P.kaxis[k] < 0 ? 1 : op->dims[P.kaxis[k]];
+
+    std::vector<Entry> entries;
+    for (const Term& t : op->terms) {
+        std::vector<std::vector<Band>> fb;
+        for (const Factor& f : t.factors) {
+            std::vector<Band> bands;
+            if (!factor_bands(f, bands)) return KFC_OK;   // not eligible: chain path
+            fb.push_back(std::move(bands));
+        }
+        std::vector<size_t> idx(fb.size(), 0);
+        bool any_empty = false;
+        for (auto& b : fb) any_empty |= b.empty();
+        if (any_empty) continue;
+        while (true) {
+            Entry e;
+            e.c = t.coeff;
+            e.off[0] = e.off[1] = e.off[2] = 0;
+            for (size_t k = 0; k < fb.size(); ++k) {
+                int ka = kaxis_of[t.factors[k].axis];
+                const Band& b = fb[k][idx[k]];
+                e.off[ka] = int(b.off);
+                e.a[ka] = b.w;
+            }
+            normalize(e);
+            entries.push_back(std::move(e));
+            size_t k = 0;
+            while (k < fb.size() && ++idx[k] == fb[k].size()) idx[k++] = 0;
+            if (k == fb.size()) break;
+        }
+    }
+    std::vector<Entry> slots[27];
+    for (Entry& e : entries) {
+        int p = (e.off[0] + 1) + 3 * (e.off[1] + 1) + 9 * (e.off[2] + 1);
+        bool merged = false;
+        for (Entry& x : slots[p])
+            if (try_merge(x, e)) { merged = true; break; }
+        if (!merged) slots[p].push_back(std::move(e));
+    }
+    for (int p = 0; p < 27; ++p) {
+        if (slots[p].size() > 1) return KFC_OK;   // not rank-1 per position
+        if (slots[p].size() == 1) {
+            Entry& e = slots[p][0];
+            P.pos[p].used = e.c != cd(0);
+            P.pos[p].c = e.c;
+            P.pos[p].u = e.a[0];
+            P.pos[p].v = e.a[1];
+            P.pos[p].w = e.a[2];
+        }
+    }
+    // the fused diagonal as per-axis sums plus a few products
+    P.D0.assign(P.kd[0], op->constant);
+    for (const Term& t : op->diagterms) {
+        if (t.factors.empty()) continue;
+        if (t.factors.size() == 1) {
+            const Factor& f = t.factors[0];
+            int ka = kaxis_of[f.axis];
+            std::vector<cd>& D = ka == 0 ? P.D0 : ka == 1 ? P.D1 : P.D2;
+            if (D.empty()) D.assign(P.kd[ka], cd(0));
+            for (int64_t i = 0; i < f.d; ++i) D[i] += t.coeff * f.diag[i];
+        } else {
+            StencilPlan::DProd dp;
+            dp.c = t.coeff;
+            for (const Factor& f : t.factors) {
+                int ka = kaxis_of[f.axis];
+                (ka == 0 ? dp.a0 : ka == 1 ? dp.a1 : dp.a2) = f.diag;
+            }
+            P.dprod.push_back(std::move(dp));
+        }
+    }
+    if (P.dprod.size() > 4) {
+        P.dprod.clear();
+        P.need_full_diag = true;
+        P.D0.clear();
+        P.D1.clear();
+        P.D2.clear();
+    }
+    P.ok = true;
+    return KFC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Device data (one precision) upload
+
+template <class S>
+static const S* upload(kfc_op* op, DeviceData& dd, const std::vector<cd>& v, int* rc) {
+    if (v.empty()) return nullptr;
+    std::vector<S> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = s_from<S>(v[i]);
+    auto b = std::make_unique<DevBuf>();
+    hipError_t e = b->ensure(h.size() * sizeof(S));
+    if (e == hipSuccess) e = hipMemcpy(b->p, h.data(), h.size() * sizeof(S), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *rc = kfc_fail(op->ctx, KFC_ENOMEM, std::string("device upload: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    const S* p = b->as<S>();
+    dd.blobs.push_back(std::move(b));
+    return p;
+}
+
+static const int64_t* upload_i64(kfc_op* op, DeviceData& dd, const std::vector<int64_t>& v, int* rc) {
+    if (v.empty()) return nullptr;
+    auto b = std::make_unique<DevBuf>();
+    hipError_t e = b->ensure(v.size() * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemcpy(b->p, v.data(), v.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *rc = kfc_fail(op->ctx, KFC_ENOMEM, std::string("device upload: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    const int64_t* p = b->as<int64_t>();
+    dd.blobs.push_back(std::move(b));
+    return p;
+}
+
+template <class S>
+static int build_device(kfc_op* op, DeviceData& dd) {
+    int rc = KFC_OK;
+    dd.elem = sizeof(S);
+    const StencilPlan& P = op->plan;
+    if (P.ok) {
+        for (int p = 0; p < 27; ++p) {
+            if (!P.pos[p].used) continue;
+            dd.pu[p] = upload<S>(op, dd, P.pos[p].u, &rc);
+            dd.pv[p] = upload<S>(op, dd, P.pos[p].v, &rc);
+            dd.pw[p] = upload<S>(op, dd, P.pos[p].w, &rc);
+        }
+        dd.D0 = upload<S>(op, dd, P.D0, &rc);
+        dd.D1 = upload<S>(op, dd, P.D1, &rc);
+        dd.D2 = upload<S>(op, dd, P.D2, &rc);
+        for (size_t k = 0; k < P.dprod.size(); ++k) {
+            dd.dp0[k] = upload<S>(op, dd, P.dprod[k].a0, &rc);
+            dd.dp1[k] = upload<S>(op, dd, P.dprod[k].a1, &rc);
+            dd.dp2[k] = upload<S>(op, dd, P.dprod[k].a2, &rc);
+        }
+    }
+    if (!P.ok || P.need_full_diag) {
+        op_host_diag(op);
+        std::vector<S> h(op->N);
+        for (int64_t i = 0; i < op->N; ++i) h[i] = s_from<S>(op->hdiag[i]);
+        hipError_t e = dd.diag.ensure(h.size() * sizeof(S));
+        if (e == hipSuccess) e = hipMemcpy(dd.diag.p, h.data(), h.size() * sizeof(S), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return kfc_fail(op->ctx, KFC_ENOMEM, std::string("diagonal upload: ") + hipGetErrorString(e));
+    }
+    if (!P.ok) {
+        dd.fdev.assign(op->terms.size(), {});
+        for (size_t t = 0; t < op->terms.size(); ++t) {
+            for (const Factor& f : op->terms[t].factors) {
+                DeviceData::FDev fd;
+                if (f.kind == KFC_DENSE) fd.data = upload<S>(op, dd, f.matrix, &rc);
+                else if (f.kind == KFC_DIAG) fd.data = upload<S>(op, dd, f.diag, &rc);
+                else if (f.kind == KFC_BANDED) {
+                    fd.data = upload<S>(op, dd, f.weights, &rc);
+                    fd.offs = upload_i64(op, dd, f.offsets, &rc);
+                }
+                dd.fdev[t].push_back(fd);
+            }
+        }
+    }
+    if (rc != KFC_OK) return rc;
+    dd.built = true;
+    return KFC_OK;
+}
+
+int op_ensure_device(kfc_op* op, int prec) {
+    DeviceData& dd = op->dd[prec];
+    if (dd.built) return KFC_OK;
+    if (prec == KFC_PREC_F64) return op->cplx ? build_device<Cx<double>>(op, dd) : build_device<double>(op, dd);
+    return op->cplx ? build_device<Cx<float>>(op, dd) : build_device<float>(op, dd);
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused stencil kernel
+
+template <class S>
+struct StArgs {
+    const S* X;
+    S* Y;
+    const S* A;
+    int64_t ld;
+    int d0, d1, d2;
+    int64_t s1, s2;
+    uint32_t used;
+    int L, nb0, nb1, nbz, ncols, nlog;
+    S c[27];
+    const S* u[27];
+    const S* v[27];
+    const S* w[27];
+    const S* D0;
+    const S* D1;
+    const S* D2;
+    const S* Dfull;
+    int ndp;
+    S dpc[4];
+    const S* dp0[4];
+    const S* dp1[4];
+    const S* dp2[4];
+    S alpha, beta, gamma;
+};
+
+constexpr int ST_TX = 64, ST_TY = 4, ST_RW = ST_TX + 2, ST_RH = ST_TY + 2, ST_TILE = ST_RW * ST_RH;
+
+template <class S>
+__global__ __launch_bounds__(256) void k_stencil(const StArgs<S> a) {
+    __shared__ S tile[2][ST_TILE];
+    const int tid = threadIdx.x, lane = tid & 63, wy = tid >> 6;
+    // XCD-aware order: the dispatcher places block b on XCD b % 8; give each XCD a
+    // contiguous range of logical tiles so halo rows are shared inside one L2.
+    const int nblk = gridDim.x;
+    const int per = nblk >> 3;
+    const int lg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lg >= a.nlog) return;   // padding blocks (the whole block leaves before any barrier)
+    int rem = lg;
+    const int b0 = rem % a.nb0; rem /= a.nb0;
+    const int b1 = rem % a.nb1; rem /= a.nb1;
+    const int bz = rem % a.nbz; rem /= a.nbz;
+    const int col = rem;
+    const int i0 = b0 * ST_TX + lane, i1 = b1 * ST_TY + wy;
+    const bool live = i0 < a.d0 && i1 < a.d1;
+    const int c0 = min(i0, a.d0 - 1), c1 = min(i1, a.d1 - 1);
+    const S one = s_one<S>(), zero = s_zero(one);
+
+    S pre[27];
+#pragma unroll
+    for (int p = 0; p < 27; ++p) {
+        pre[p] = zero;
+        if ((a.used >> p) & 1u) {
+            S x = a.c[p];
+            if (a.u[p]) x = s_mul(x, a.u[p][c0]);
+            if (a.v[p]) x = s_mul(x, a.v[p][c1]);
+            pre[p] = x;
+        }
+    }
+    S d01 = zero;
+    if (a.D0) d01 = s_add(d01, a.D0[c0]);
+    if (a.D1) d01 = s_add(d01, a.D1[c1]);
+    S dpre[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        dpre[k] = zero;
+        if (k < a.ndp) {
+            S x = a.dpc[k];
+            if (a.dp0[k]) x = s_mul(x, a.dp0[k][c0]);
+            if (a.dp1[k]) x = s_mul(x, a.dp1[k][c1]);
+            dpre[k] = x;
+        }
+    }
+
+    const S* __restrict__ Xc = a.X + (int64_t)col * a.ld;
+    S* __restrict__ Yc = a.Y + (int64_t)col * a.ld;
+    const S* __restrict__ Ac = a.A ? a.A + (int64_t)col * a.ld : nullptr;
+    const int z0 = bz * a.L, z1 = min(z0 + a.L, a.d2);
+
+    // tile element e <-> (row r, column cc): global (b0*64 - 1 + cc, b1*4 - 1 + r)
+    const int e1 = tid + 256;
+    const int r0 = tid / ST_RW, cc0 = tid % ST_RW;
+    const int r1 = e1 / ST_RW, cc1 = e1 % ST_RW;
+    const int g00 = b0 * ST_TX - 1 + cc0, g01 = b1 * ST_TY - 1 + r0;
+    const int g10 = b0 * ST_TX - 1 + cc1, g11 = b1 * ST_TY - 1 + r1;
+    const bool in0 = g00 >= 0 && g00 < a.d0 && g01 >= 0 && g01 < a.d1;
+    const bool in1 = e1 < ST_TILE && g10 >= 0 && g10 < a.d0 && g11 >= 0 && g11 < a.d1;
+    const int64_t off0 = in0 ? (int64_t)g00 + (int64_t)g01 * a.s1 : 0;
+    const int64_t off1 = in1 ? (int64_t)g10 + (int64_t)g11 * a.s1 : 0;
+
+    auto load = [&](int z, S& v0, S& v1) {
+        const bool zin = z >= 0 && z < a.d2;
+        const int64_t zo = (int64_t)z * a.s2;
+        v0 = (zin && in0) ? Xc[off0 + zo] : zero;
+        v1 = (zin && in1) ? Xc[off1 + zo] : zero;
+    };
+    auto put = [&](int slot, const S& v0, const S& v1) {
+        tile[slot][tid] = v0;
+        if (e1 < ST_TILE) tile[slot][e1] = v1;
+    };
+
+    S q[3][3][3];
+    S v0, v1;
+    load(z0 - 1, v0, v1);
+    put(0, v0, v1);
+    load(z0, v0, v1);
+    put(1, v0, v1);
+    __syncthreads();
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            q[0][dy][dx] = tile[0][(wy + dy) * ST_RW + lane + dx];
+            q[1][dy][dx] = tile[1][(wy + dy) * ST_RW + lane + dx];
+        }
+    __syncthreads();
+    load(z0 + 1, v0, v1);
+
+    const int64_t base = (int64_t)c0 + (int64_t)c1 * a.s1;
+    for (int z = z0, k = 0; z < z1; ++z, ++k) {
+        const int slot = k & 1;
+        put(slot, v0, v1);
+        __syncthreads();
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) q[2][dy][dx] = tile[slot][(wy + dy) * ST_RW + lane + dx];
+        if (z + 1 < z1) load(z + 2, v0, v1);
+        if (live) {
+            S dv = d01;
+            if (a.D2) dv = s_add(dv, a.D2[z]);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                if (kk < a.ndp) dv = s_fma(dv, dpre[kk], a.dp2[kk] ? a.dp2[kk][z] : one);
+            const int64_t idx = base + (int64_t)z * a.s2;
+            if (a.Dfull) dv = s_add(dv, a.Dfull[idx]);
+            S acc = s_mul(dv, q[1][1][1]);
+#pragma unroll
+            for (int p = 0; p < 27; ++p) {
+                if ((a.used >> p) & 1u) {
+                    const int dx = p % 3, dy = (p / 3) % 3, dz = p / 9;
+                    const S wz = a.w[p] ? s_mul(pre[p], a.w[p][z]) : pre[p];
+                    acc = s_fma(acc, wz, q[dz][dy][dx]);
+                }
+            }
+            S out = s_mul(a.alpha, acc);
+            out = s_fma(out, a.beta, q[1][1][1]);
+            if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
+            Yc[idx] = out;
+        }
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                q[0][dy][dx] = q[1][dy][dx];
+                q[1][dy][dx] = q[2][dy][dx];
+            }
+    }
+}
+
+template <class S>
+static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld,
+                          cd alpha, cd beta, cd gamma, hipStream_t st) {
+    const DeviceData& dd = op->dd[std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1];
+    const StencilPlan& P = op->plan;
+    StArgs<S> a;
+    std::memset(&a, 0, sizeof(a));
+    a.X = static_cast<const S*>(X);
+    a.Y = static_cast<S*>(Y);
+    a.A = static_cast<const S*>(A);
+    a.ld = ld;
+    a.d0 = int(P.kd[0]);
+    a.d1 = int(P.kd[1]);
+    a.d2 = int(P.kd[2]);
+    a.s1 = P.kd[0];
+    a.s2 = P.kd[0] * P.kd[1];
+    a.used = 0;
+    for (int p = 0; p < 27; ++p) {
+        if (!P.pos[p].used) continue;
+        a.used |= 1u << p;
+        a.c[p] = s_from<S>(P.pos[p].c);
+        a.u[p] = static_cast<const S*>(dd.pu[p]);
+        a.v[p] = static_cast<const S*>(dd.pv[p]);
+        a.w[p] = static_cast<const S*>(dd.pw[p]);
+    }
+    a.D0 = static_cast<const S*>(dd.D0);
+    a.D1 = static_cast<const S*>(dd.D1);
+    a.D2 = static_cast<const S*>(dd.D2);
+    a.Dfull = P.need_full_diag ? dd.diag.as<S>() : nullptr;
+    a.ndp = int(P.dprod.size());
+    for (int k = 0; k < a.ndp; ++k) {
+        a.dpc[k] = s_from<S>(P.dprod[k].c);
+        a.dp0[k] = static_cast<const S*>(dd.dp0[k]);
+        a.dp1[k] = static_cast<const S*>(dd.dp1[k]);
+        a.dp2[k] = static_cast<const S*>(dd.dp2[k]);
+    }
+    a.alpha = s_from<S>(alpha);
+    a.beta = s_from<S>(beta);
+    a.gamma = s_from<S>(gamma);
+    a.nb0 = int((a.d0 + ST_TX - 1) / ST_TX);
+    a.nb1 = int((a.d1 + ST_TY - 1) / ST_TY);
+    a.ncols = int(ncols);
+    int L = 64;
+    while (L > 8 && (int64_t)a.nb0 * a.nb1 * ((a.d2 + L - 1) / L) * ncols < 8192) L >>= 1;
+    L = std::min(L, a.d2);
+    a.L = std::max(L, 1);
+    a.nbz = (a.d2 + a.L - 1) / a.L;
+    int64_t nlog = (int64_t)a.nb0 * a.nb1 * a.nbz * ncols;
+    if (nlog >= (1ll << 31) - 8) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "apply: grid too large");
+    a.nlog = int(nlog);
+    const int64_t nblk = (nlog + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    KFC_HIP(op->ctx, hipGetLastError());
+    return KFC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Chain path kernels
+
+template <class S>
+__global__ void k_diag_init(const S* __restrict__ D, const S* __restrict__ X, S* __restrict__ Y, int64_t N,
+                            int64_t ld, int64_t total) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = g / N, t = g - col * N;
+        Y[t + col * ld] = s_mul(D[t], X[t + col * ld]);
+    }
+}
+
+// One factor along an axis (stride, d): dst = op(src), or dst += coeff * op(src).
+template <class S>
+__global__ void k_factor(const S* __restrict__ src, int64_t lds, S* __restrict__ dst, int64_t ldd, int64_t N,
+                         int64_t total, int64_t stride, int64_t d, int kind, int nb,
+                         const int64_t* __restrict__ offs, const S* __restrict__ data, int64_t shift, S factor,
+                         int accumulate, S coeff) {
+    const S zero = s_zero(coeff);
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = g / N, t = g - col * N;
+        const int64_t i = (t / stride) % d;
+        const int64_t b = t - i * stride;
+        const S* s = src + col * lds;
+        S acc = zero;
+        if (kind == KFC_DIAG) {
+            acc = s_mul(data[i], s[t]);
+        } else if (kind == KFC_BANDED) {
+            for (int k = 0; k < nb; ++k) {
+                const int64_t j = i + offs[k];
+                if (j >= 0 && j < d) acc = s_fma(acc, data[i + k * d], s[b + j * stride]);
+            }
+        } else if (kind == KFC_SHIFT) {
+            const int64_t j = i - shift;
+            if (j >= 0 && j < d) acc = s_mul(factor, s[b + j * stride]);
+        } else {
+            for (int64_t j = 0; j < d; ++j) acc = s_fma(acc, data[i + j * d], s[b + j * stride]);
+        }
+        S* o = dst + col * ldd;
+        o[t] = accumulate ? s_fma(o[t], coeff, acc) : acc;
+    }
+}
+
+template <class S>
+__global__ void k_axpby3(S* __restrict__ C, const S* __restrict__ B, const S* __restrict__ A, int64_t N, int64_t ld,
+                         int64_t total, S a, S b, S g) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = x / N, t = x - col * N, i = t + col * ld;
+        S v = s_mul(a, C[i]);
+        v = s_fma(v, b, B[i]);
+        if (A) v = s_fma(v, g, A[i]);
+        C[i] = v;
+    }
+}
+
+static unsigned grid_for(int64_t total) {
+    int64_t g = (total + 255) / 256;
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>(g, 65536)));
+}
+
+template <class S>
+static int launch_chain(kfc_op* op, const void* Xv, void* Yv, int64_t ncols, int64_t ld, hipStream_t st) {
+    const int prec = std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1;
+    DeviceData& dd = op->dd[prec];
+    const int64_t N = op->N, total = N * ncols;
+    const S* X = static_cast<const S*>(Xv);
+    S* Y = static_cast<S*>(Yv);
+    hipLaunchKernelGGL(k_diag_init<S>, dim3(grid_for(total)), dim3(256), 0, st, dd.diag.as<S>(), X, Y, N, ld, total);
+    size_t need = size_t(total) * sizeof(S);
+    bool multi = false;
+    for (const Term& t : op->terms) multi |= t.factors.size() > 1;
+    if (multi) {
+        KFC_HIP(op->ctx, dd.buf1.ensure(need));
+        KFC_HIP(op->ctx, dd.buf2.ensure(need));
+    }
+    std::vector<int64_t> stride(op->ndims, 1);
+    for (int a = 1; a < op->ndims; ++a) stride[a] = stride[a - 1] * op->dims[a - 1];
+    for (size_t ti = 0; ti < op->terms.size(); ++ti) {
+        const Term& t = op->terms[ti];
+        const S* src = X;
+        int64_t lds = ld;
+        for (size_t k = 0; k < t.factors.size(); ++k) {
+            const Factor& f = t.factors[k];
+            const DeviceData::FDev& fd = dd.fdev[ti][k];
+            const bool last = k + 1 == t.factors.size();
+            S* dst = last ? Y : (k % 2 == 0 ? dd.buf1.as<S>() : dd.buf2.as<S>());
+            const int64_t ldd = last ? ld : N;
+            hipLaunchKernelGGL(k_factor<S>, dim3(grid_for(total)), dim3(256), 0, st, src, lds, dst, ldd, N, total,
+                               stride[f.axis], f.d, f.kind, int(f.offsets.size()), fd.offs,
+                               static_cast<const S*>(fd.data), f.shift, s_from<S>(f.factor), last ? 1 : 0,
+                               s_from<S>(t.coeff));
+            src = dst;
+            lds = ldd;
+        }
+    }
+    KFC_HIP(op->ctx, hipGetLastError());
+    return KFC_OK;
+}
+
+template <class S>
+static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
+                   cd gamma, hipStream_t st) {
+    if (op->plan.ok) return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+    int rc = launch_chain<S>(op, X, Y, ncols, ld, st);
+    if (rc != KFC_OK) return rc;
+    if (alpha != cd(1) || beta != cd(0) || A) {
+        const int64_t total = op->N * ncols;
+        hipLaunchKernelGGL(k_axpby3<S>, dim3(grid_for(total)), dim3(256), 0, st, static_cast<S*>(Y),
+                           static_cast<const S*>(X), static_cast<const S*>(A), op->N, ld, total, s_from<S>(alpha),
+                           s_from<S>(beta), s_from<S>(gamma));
+        KFC_HIP(op->ctx, hipGetLastError());
+    }
+    return KFC_OK;
+}
+
+static int dispatch(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, int prec, cd alpha,
+                    cd beta, cd gamma, hipStream_t st) {
+    if (ncols == 0 || op->N == 0) return KFC_OK;
+    int rc = op_ensure_device(op, prec);
+    if (rc != KFC_OK) return rc;
+    if (prec == KFC_PREC_F64)
+        return op->cplx ? apply_t<Cx<double>>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st)
+                        : apply_t<double>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+    return op->cplx ? apply_t<Cx<float>>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st)
+                    : apply_t<float>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+}
+
+int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st) {
+    return dispatch(op, X, nullptr, Y, ncols, ld, prec, cd(1), cd(0), cd(0), st);
+}
+
+int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec, double a,
+                  double b, double g, hipStream_t st) {
+    return dispatch(op, B, A, C, ncols, ld, prec, cd(a), cd(b), cd(g), st);
+}

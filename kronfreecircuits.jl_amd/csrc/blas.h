@@ -1,0 +1,61 @@
+// Block linear algebra on tall device blocks (N rows, a few dozen columns):
+// Gram matrices, block combinations, column reductions, the Lanczos fused
+// orthogonalization passes and the seeded device RNG.  Reductions accumulate
+// in FP64 whatever the storage precision.
+#pragma once
+#include "kfc_internal.h"
+
+template <class S> struct AccOf { using type = double; };
+template <class R> struct AccOf<Cx<R>> { using type = Cx<double>; };
+
+// Scratch owned by a solver call: device partials and staging.
+struct Work {
+    kfc_ctx* ctx = nullptr;
+    DevBuf part;      // reduction partials
+    DevBuf small;     // small matrices / vectors uploaded to the device
+    DevBuf small2;
+    std::vector<unsigned char> host;
+};
+
+// G (p x q, column-major, host) = A^H B over N rows.
+template <class S>
+int blas_gram(Work& w, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q, int64_t N, std::vector<cd>& G);
+// C[:, 0:q] = A[:, 0:p] M (+ beta C), M p x q column-major host; C must not alias A.
+template <class S>
+int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M, int q, S* C, int64_t ldc, int64_t N,
+              double beta);
+// Column 2-norms of A[:, 0:p] (host).
+template <class S>
+int blas_colnorms(Work& w, const S* A, int64_t lda, int p, int64_t N, std::vector<double>& nrm);
+// A[:, j] *= s[j]
+template <class S>
+int blas_scale_cols(Work& w, S* A, int64_t lda, int p, int64_t N, const std::vector<double>& s);
+// R[:, j] = AX[:, j] - theta[j] X[:, j]; returns column norms of R.
+template <class S>
+int blas_residual(Work& w, S* R, int64_t ldr, const S* AX, int64_t ldax, const S* X, int64_t ldx, int p, int64_t N,
+                  const std::vector<double>& theta, std::vector<double>& nrm);
+// R[:, j] /= hypot(D - theta_j, eta)  (RitzJacobi ldiv!, src/lobpcg.jl:114-119), or
+// R[:, j] *= inv (FixedJacobi, src/lobpcg.jl:31-34) when eta < 0 (theta ignored, inv precomputed in D).
+template <class S>
+int blas_precond(Work& w, S* R, int64_t ldr, int p, int64_t N, const double* D, const std::vector<double>& theta,
+                 double eta);
+// theta[j] = sum D |X|^2 / sum |X|^2  (_diagonal_expectation!, src/lobpcg.jl:73-96)
+template <class S>
+int blas_diag_expect(Work& w, const S* X, int64_t ldx, int p, int64_t N, const double* D, std::vector<double>& theta);
+// Y = alpha X + beta Y on p columns
+template <class S>
+int blas_axpby(Work& w, const S* X, int64_t ldx, S* Y, int64_t ldy, int p, int64_t N, double alpha, double beta);
+// copy with precision conversion (columns)
+template <class S, class T>
+int blas_convert(Work& w, const S* X, int64_t ldx, T* Y, int64_t ldy, int p, int64_t N);
+// N x p standard normal (complex: both parts), deterministic in (seed, column offset)
+template <class S>
+int blas_randn(Work& w, S* X, int64_t ldx, int p, int64_t N, uint64_t seed);
+
+// Lanczos pass 1: h[i] = s_i * V[:, i]^H w for i < k (one pass over V), plus |w|^2.
+template <class S>
+int lz_multidot(Work& w, const S* V, int64_t ldv, int k, const double* scale, const S* wv, int64_t N,
+                std::vector<cd>& h, double* wnorm2);
+// Lanczos pass 2: w -= sum_i V[:, i] (s_i h_i), returns |w|^2 after.
+template <class S>
+int lz_update(Work& w, const S* V, int64_t ldv, int k, const std::vector<cd>& hs, S* wv, int64_t N, double* wnorm2);

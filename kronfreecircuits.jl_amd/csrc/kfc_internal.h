@@ -1,0 +1,204 @@
+// Internal declarations of libkfc: device scalar types, the operator
+// structure, and the launch helpers shared by the translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/kfc.h"
+
+// ------------------------------------------------------------------------------------------
+// Scalar types.  Complex values are interleaved (re, im) pairs, as the C ABI passes them.
+
+template <class R>
+struct Cx {
+    R re, im;
+};
+
+template <class S> struct RealOf { using type = S; };
+template <class R> struct RealOf<Cx<R>> { using type = R; };
+template <class S> struct IsCx { static constexpr bool value = false; };
+template <class R> struct IsCx<Cx<R>> { static constexpr bool value = true; };
+
+__host__ __device__ inline float s_zero(float) { return 0.f; }
+__host__ __device__ inline double s_zero(double) { return 0.0; }
+template <class R> __host__ __device__ inline Cx<R> s_zero(Cx<R>) { return Cx<R>{R(0), R(0)}; }
+
+__host__ __device__ inline float s_mul(float a, float b) { return a * b; }
+__host__ __device__ inline double s_mul(double a, double b) { return a * b; }
+template <class R> __host__ __device__ inline Cx<R> s_mul(Cx<R> a, Cx<R> b) {
+    return Cx<R>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline float s_add(float a, float b) { return a + b; }
+__host__ __device__ inline double s_add(double a, double b) { return a + b; }
+template <class R> __host__ __device__ inline Cx<R> s_add(Cx<R> a, Cx<R> b) { return Cx<R>{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline float s_sub(float a, float b) { return a - b; }
+__host__ __device__ inline double s_sub(double a, double b) { return a - b; }
+template <class R> __host__ __device__ inline Cx<R> s_sub(Cx<R> a, Cx<R> b) { return Cx<R>{a.re - b.re, a.im - b.im}; }
+// a + b*c
+__host__ __device__ inline float s_fma(float a, float b, float c) { return fmaf(b, c, a); }
+__host__ __device__ inline double s_fma(double a, double b, double c) { return fma(b, c, a); }
+__host__ __device__ inline float r_fma(float a, float b, float c) { return fmaf(a, b, c); }
+__host__ __device__ inline double r_fma(double a, double b, double c) { return fma(a, b, c); }
+template <class R> __host__ __device__ inline Cx<R> s_fma(Cx<R> a, Cx<R> b, Cx<R> c) {
+    return Cx<R>{r_fma(b.re, c.re, r_fma(-b.im, c.im, a.re)), r_fma(b.re, c.im, r_fma(b.im, c.re, a.im))};
+}
+__host__ __device__ inline float s_conj(float a) { return a; }
+__host__ __device__ inline double s_conj(double a) { return a; }
+template <class R> __host__ __device__ inline Cx<R> s_conj(Cx<R> a) { return Cx<R>{a.re, -a.im}; }
+__host__ __device__ inline float s_abs2(float a) { return a * a; }
+__host__ __device__ inline double s_abs2(double a) { return a * a; }
+template <class R> __host__ __device__ inline R s_abs2(Cx<R> a) { return a.re * a.re + a.im * a.im; }
+// real scalar times S
+__host__ __device__ inline float s_scale(float r, float a) { return r * a; }
+__host__ __device__ inline double s_scale(double r, double a) { return r * a; }
+template <class R> __host__ __device__ inline Cx<R> s_scale(R r, Cx<R> a) { return Cx<R>{r * a.re, r * a.im}; }
+// conversions from a host complex<double>
+template <class S> inline S s_from(std::complex<double> z);
+template <> inline float s_from<float>(std::complex<double> z) { return float(z.real()); }
+template <> inline double s_from<double>(std::complex<double> z) { return z.real(); }
+template <> inline Cx<float> s_from<Cx<float>>(std::complex<double> z) { return Cx<float>{float(z.real()), float(z.imag())}; }
+template <> inline Cx<double> s_from<Cx<double>>(std::complex<double> z) { return Cx<double>{z.real(), z.imag()}; }
+template <class S> __host__ __device__ inline S s_one();
+template <> __host__ __device__ inline float s_one<float>() { return 1.f; }
+template <> __host__ __device__ inline double s_one<double>() { return 1.0; }
+template <> __host__ __device__ inline Cx<float> s_one<Cx<float>>() { return Cx<float>{1.f, 0.f}; }
+template <> __host__ __device__ inline Cx<double> s_one<Cx<double>>() { return Cx<double>{1.0, 0.0}; }
+
+using cd = std::complex<double>;
+
+// ------------------------------------------------------------------------------------------
+// Context
+
+struct kfc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 256;
+};
+
+int kfc_fail(kfc_ctx* ctx, int code, const std::string& msg);
+
+#define KFC_HIP(ctx, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return kfc_fail((ctx), KFC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// A device allocation owned by the library.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ------------------------------------------------------------------------------------------
+// Operator (host copy of the descriptor plus device data per precision)
+
+struct Factor {
+    int axis = 0;
+    int kind = KFC_DIAG;
+    int64_t d = 0;
+    std::vector<int64_t> offsets;
+    std::vector<cd> weights;   // d x nbands, column-major
+    int64_t shift = 0;
+    cd factor = 0;
+    std::vector<cd> matrix;    // d x d column-major
+    std::vector<cd> diag;      // d
+};
+
+struct Term {
+    cd coeff = 0;
+    std::vector<Factor> factors;
+};
+
+// One position of the fused stencil: x at (i0+dx, i1+dy, i2+dz) times
+// c * u[i0] * v[i1] * w[i2] (vectors empty = ones).
+struct StencilPos {
+    bool used = false;
+    cd c = 0;
+    std::vector<cd> u, v, w;
+};
+
+struct StencilPlan {
+    bool ok = false;
+    int kaxis[3] = {0, -1, -1};   // kernel axis -> operator axis (-1: unit axis)
+    int64_t kd[3] = {1, 1, 1};
+    StencilPos pos[27];
+    std::vector<cd> D0, D1, D2;   // separable diagonal per kernel axis (empty = zero)
+    struct DProd { cd c; std::vector<cd> a0, a1, a2; };
+    std::vector<DProd> dprod;     // multi-axis diagonal products (<= 4)
+    bool need_full_diag = false;
+};
+
+// Device data of one precision/scalar kind, built on first use.
+struct DeviceData {
+    bool built = false;
+    size_t elem = 0;                // bytes per scalar
+    DevBuf diag;                    // N (only when the chain path or a non-separable diagonal needs it)
+    std::vector<std::unique_ptr<DevBuf>> blobs;  // factor and plan tables
+    // kept terms: per factor, device pointers
+    struct FDev { const void* data = nullptr; const int64_t* offs = nullptr; };
+    std::vector<std::vector<FDev>> fdev;
+    // stencil plan tables
+    const void* pu[27] = {}; const void* pv[27] = {}; const void* pw[27] = {};
+    const void* D0 = nullptr; const void* D1 = nullptr; const void* D2 = nullptr;
+    const void* dp0[4] = {}; const void* dp1[4] = {}; const void* dp2[4] = {};
+    DevBuf buf1, buf2;              // chain scratch
+};
+
+struct kfc_op {
+    kfc_ctx* ctx = nullptr;
+    int ndims = 0;
+    std::vector<int64_t> dims;
+    int64_t N = 0;
+    bool cplx = false;
+    cd constant = 0;
+    std::vector<Term> diagterms;    // fused into the diagonal (the constant as a zero-factor term)
+    std::vector<Term> terms;        // kept
+    int64_t capacity = 1;
+    bool hermitian = true;
+    std::vector<cd> hdiag;          // host fused diagonal (built on demand)
+    StencilPlan plan;
+    DeviceData dd[2];               // [KFC_PREC_F64], [KFC_PREC_F32]
+    double gershgorin = 0;
+    bool have_gershgorin = false;
+};
+
+// Builders and launches (apply.hip)
+int op_build_plan(kfc_op* op);
+int op_host_diag(kfc_op* op);
+int op_ensure_device(kfc_op* op, int prec);
+int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st);
+// Chebyshev step fused into the apply: C = a*(H B) + b*B + g*A  (A may be null)
+int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec,
+                  double a, double b, double g, hipStream_t st);
+
+// Host linear algebra (linalg.cpp): Hermitian eigendecomposition, ascending.
+// A is n x n column-major (overwritten); w[n]; Z[n x n] eigenvectors.
+void herm_eig(int n, std::vector<cd>& A, std::vector<double>& w, std::vector<cd>& Z);
+void sym_eig(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);

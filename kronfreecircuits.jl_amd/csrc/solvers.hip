@@ -1,0 +1,918 @@
+// Eigensolver loops on the device: Dense, thick-restart Lanczos and LOBPCG.
+// Block data stay on the device; only small projected matrices cross to the
+// host, where they are diagonalized in FP64 (herm_eig).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <random>
+
+#include "blas.h"
+
+int op_full_diagonal(kfc_op* op, std::vector<double>& D);
+int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+inline double secs(clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); }
+
+template <class S> struct Lower { using type = S; };
+template <> struct Lower<double> { using type = float; };
+template <> struct Lower<Cx<double>> { using type = Cx<float>; };
+
+template <class S> constexpr int prec_of() { return std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1; }
+
+// Host <-> device block transfer of N x p (host cd array, column-major)
+template <class S>
+int upload_block(kfc_ctx* ctx, const std::vector<cd>& H, int64_t N, int p, S* D, int64_t ld) {
+    std::vector<S> h(size_t(N) * p);
+    for (size_t i = 0; i < h.size(); ++i) h[i] = s_from<S>(H[i]);
+    KFC_HIP(ctx, hipMemcpy2DAsync(D, ld * sizeof(S), h.data(), N * sizeof(S), N * sizeof(S), p, hipMemcpyHostToDevice,
+                                  ctx->stream));
+    KFC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KFC_OK;
+}
+
+template <class S>
+int download_block(kfc_ctx* ctx, const S* D, int64_t ld, int64_t N, int p, std::vector<cd>& H) {
+    std::vector<S> h(size_t(N) * p);
+    KFC_HIP(ctx, hipMemcpy2DAsync(h.data(), N * sizeof(S), D, ld * sizeof(S), N * sizeof(S), p, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    KFC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    H.resize(h.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+        if constexpr (IsCx<S>::value) H[i] = cd(h[i].re, h[i].im);
+        else H[i] = cd(h[i], 0.0);
+    }
+    return KFC_OK;
+}
+
+void write_host_V(const kfc_op* op, const std::vector<cd>& V, double* out) {
+    for (size_t i = 0; i < V.size(); ++i) {
+        if (op->cplx) {
+            out[2 * i] = V[i].real();
+            out[2 * i + 1] = V[i].imag();
+        } else {
+            out[i] = V[i].real();
+        }
+    }
+}
+
+void read_host_V(const kfc_op* op, const double* in, size_t n, std::vector<cd>& V) {
+    V.resize(n);
+    for (size_t i = 0; i < n; ++i) V[i] = op->cplx ? cd(in[2 * i], in[2 * i + 1]) : cd(in[i], 0.0);
+}
+
+// Hermitian eig of a small host matrix (n x n column-major, symmetrized).
+void small_eig(int n, const std::vector<cd>& G, std::vector<double>& w, std::vector<cd>& Z) {
+    std::vector<cd> A(size_t(n) * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) A[i + size_t(j) * n] = 0.5 * (G[i + size_t(j) * n] + std::conj(G[j + size_t(i) * n]));
+    herm_eig(n, A, w, Z);
+}
+
+struct Timer {
+    double apply = 0, apply_gpu = 0;
+    int64_t cols = 0, cols32 = 0, calls = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Timer() {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+    }
+    ~Timer() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    Timer(const Timer&) = delete;
+    Timer& operator=(const Timer&) = delete;
+};
+
+template <class S>
+int apply_timed(kfc_op* op, const S* X, S* Y, int64_t ncols, int64_t ld, Timer& t, double alpha = 1.0) {
+    if (ncols == 0) return KFC_OK;
+    auto t0 = clk::now();
+    KFC_HIP(op->ctx, hipEventRecord(t.e0, op->ctx->stream));
+    int rc = alpha == 1.0 ? op_apply(op, X, Y, ncols, ld, prec_of<S>(), op->ctx->stream)
+                          : op_apply_cheb(op, X, nullptr, Y, ncols, ld, prec_of<S>(), alpha, 0.0, 0.0, op->ctx->stream);
+    if (rc != KFC_OK) return rc;
+    KFC_HIP(op->ctx, hipEventRecord(t.e1, op->ctx->stream));
+    KFC_HIP(op->ctx, hipEventSynchronize(t.e1));
+    float ms = 0;
+    KFC_HIP(op->ctx, hipEventElapsedTime(&ms, t.e0, t.e1));
+    t.apply_gpu += 1e-3 * ms;
+    t.apply += secs(t0);
+    t.cols += ncols;
+    if (prec_of<S>() == 1) t.cols32 += ncols;
+    t.calls += 1;
+    return KFC_OK;
+}
+
+// Fresh FP64 true residuals of host pairs.
+template <class S>
+int true_residuals(kfc_op* op, Work& w, const std::vector<double>& E, const std::vector<cd>& Vh, int m,
+                   std::vector<double>& res, Timer& t) {
+    const int64_t N = op->N;
+    DevBuf bx, by;
+    KFC_HIP(op->ctx, bx.ensure(sizeof(S) * size_t(N) * m));
+    KFC_HIP(op->ctx, by.ensure(sizeof(S) * size_t(N) * m));
+    int rc = upload_block<S>(op->ctx, Vh, N, m, bx.as<S>(), N);
+    if (rc == KFC_OK) rc = apply_timed<S>(op, bx.as<S>(), by.as<S>(), m, N, t);
+    if (rc == KFC_OK) rc = blas_residual<S>(w, by.as<S>(), N, by.as<S>(), N, bx.as<S>(), N, m, N, E, res);
+    return rc;
+}
+
+// ------------------------------------------------------------------------------------------
+// Dense: Matrix(H) by identity blocks on the device (src/apply.jl:506-531), host eig.
+
+template <class S>
+int dense_t(kfc_op* op, int64_t m, double* Eout, double* Vout, kfc_stats* st) {
+    auto t0 = clk::now();
+    const int64_t N = op->N;
+    if (N > 8192) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "Dense: prod(dims) above 8192 is not served; use an iterative solver");
+    Timer tm;
+    const int B = int(std::min<int64_t>(N, 256));
+    DevBuf be, by;
+    KFC_HIP(op->ctx, be.ensure(sizeof(S) * size_t(N) * B));
+    KFC_HIP(op->ctx, by.ensure(sizeof(S) * size_t(N) * B));
+    std::vector<cd> M(size_t(N) * N), blk;
+    for (int64_t c0 = 0; c0 < N; c0 += B) {
+        const int w = int(std::min<int64_t>(B, N - c0));
+        std::vector<cd> E(size_t(N) * w, cd(0));
+        for (int j = 0; j < w; ++j) E[(c0 + j) + size_t(j) * N] = 1;
+        int rc = upload_block<S>(op->ctx, E, N, w, be.as<S>(), N);
+        if (rc == KFC_OK) rc = apply_timed<S>(op, be.as<S>(), by.as<S>(), w, N, tm);
+        if (rc == KFC_OK) rc = download_block<S>(op->ctx, by.as<S>(), N, N, w, blk);
+        if (rc != KFC_OK) return rc;
+        std::copy(blk.begin(), blk.end(), M.begin() + size_t(c0) * N);
+    }
+    std::vector<double> w;
+    std::vector<cd> Z;
+    small_eig(int(N), M, w, Z);
+    const int64_t k = (m > 0 && m < N) ? m : N;
+    for (int64_t j = 0; j < k; ++j) Eout[j] = w[j];
+    Z.resize(size_t(N) * k);
+    write_host_V(op, Z, Vout);
+    if (st) {
+        *st = kfc_stats{};
+        st->n_matvec = tm.cols;
+        st->n_apply_calls = tm.calls;
+        st->seconds_apply = tm.apply;
+        st->seconds_apply_gpu = tm.apply_gpu;
+        st->seconds_total = secs(t0);
+        st->n_converged = int(k);
+    }
+    return KFC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Thick-restart Lanczos (Wu & Simon's restart with full reorthogonalization).
+// The basis columns are stored unnormalized with a host scale s_i, so that the
+// normalization never costs a pass over the vector: v_i = s_i * V[:, i].
+
+template <class S>
+struct LanczosOut {
+    std::vector<double> theta;   // Ritz values of the last projection
+    std::vector<double> resid;   // |beta y_k|
+    int converged = 0;
+    int restarts = 0;
+    bool invariant = false;
+};
+
+template <class S>
+int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter, const S* start, int64_t start_ld,
+                 uint64_t seed, Timer& tm, LanczosOut<S>& out, DevBuf& result) {
+    const int64_t N = op->N;
+    DevBuf bufA, bufB;
+    KFC_HIP(op->ctx, bufA.ensure(sizeof(S) * size_t(N) * (kd + 1)));
+    KFC_HIP(op->ctx, bufB.ensure(sizeof(S) * size_t(N) * (kd + 1)));
+    S* V = bufA.as<S>();
+    S* V2 = bufB.as<S>();
+    std::vector<double> scale(kd + 1, 1.0);
+    int rc;
+    if (start) rc = blas_convert<S, S>(w, start, start_ld, V, N, 1, N);
+    else rc = blas_randn<S>(w, V, N, 1, N, seed);
+    if (rc != KFC_OK) return rc;
+    std::vector<double> nrm;
+    rc = blas_colnorms<S>(w, V, N, 1, N, nrm);
+    if (rc != KFC_OK) return rc;
+    if (!(nrm[0] > 0)) return kfc_fail(op->ctx, KFC_EBREAKDOWN, "Lanczos: zero start vector");
+    scale[0] = 1.0 / nrm[0];
+    std::vector<cd> T(size_t(kd) * kd, cd(0));
+    int k = 0;
+    double beta_last = 0;
+    std::vector<double> theta;
+    std::vector<cd> Y;
+    for (int64_t rs = 0;; ++rs) {
+        int jend = kd;
+        for (int j = k; j < kd; ++j) {
+            S* wv = V + size_t(j + 1) * N;
+            rc = apply_timed<S>(op, V + size_t(j) * N, wv, 1, N, tm, scale[j]);
+            if (rc != KFC_OK) return rc;
+            std::vector<cd> h, h2;
+            double wn0 = 0, wn1 = 0;
+            rc = lz_multidot<S>(w, V, N, j + 1, scale.data(), wv, N, h, &wn0);
+            if (rc != KFC_OK) return rc;
+            std::vector<cd> hs(j + 1);
+            for (int i = 0; i <= j; ++i) hs[i] = scale[i] * h[i];
+            rc = lz_update<S>(w, V, N, j + 1, hs, wv, N, &wn1);
+            if (rc != KFC_OK) return rc;
+            // a second classical Gram-Schmidt pass when the first removed most of w
+            if (wn1 < 1e-4 * wn0) {
+                rc = lz_multidot<S>(w, V, N, j + 1, scale.data(), wv, N, h2, nullptr);
+                if (rc != KFC_OK) return rc;
+                for (int i = 0; i <= j; ++i) hs[i] = scale[i] * h2[i], h[i] += h2[i];
+                rc = lz_update<S>(w, V, N, j + 1, hs, wv, N, &wn1);
+                if (rc != KFC_OK) return rc;
+            }
+            for (int i = 0; i <= j; ++i) {
+                T[i + size_t(j) * kd] = h[i];
+                T[j + size_t(i) * kd] = std::conj(h[i]);
+            }
+            T[j + size_t(j) * kd] = cd(h[j].real(), 0);
+            const double beta = std::sqrt(std::max(wn1, 0.0));
+            beta_last = beta;
+            if (j + 1 < kd) {
+                T[(j + 1) + size_t(j) * kd] = 0;   // filled by the next step's projection
+                T[j + size_t(j + 1) * kd] = 0;
+            }
+            double hn = 0;
+            for (auto& x : h) hn += std::norm(x);
+            if (beta <= 1e-13 * std::sqrt(hn + wn0)) {   // an invariant subspace
+                jend = j + 1;
+                out.invariant = true;
+                scale[j + 1] = 0.0;
+                break;
+            }
+            scale[j + 1] = 1.0 / beta;
+        }
+        const int n = jend;
+        std::vector<cd> Tn(size_t(n) * n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) Tn[i + size_t(j) * n] = T[i + size_t(j) * kd];
+        small_eig(n, Tn, theta, Y);
+        std::vector<double> resid(n);
+        for (int i = 0; i < n; ++i) resid[i] = out.invariant ? 0.0 : beta_last * std::abs(Y[(n - 1) + size_t(i) * n]);
+        int conv = 0;
+        while (conv < n && resid[conv] <= tol) ++conv;
+        out.theta = theta;
+        out.resid = resid;
+        out.converged = conv;
+        out.restarts = int(rs);
+        if (conv >= m || rs >= maxiter || out.invariant || n < kd) {
+            // Ritz vectors of the lowest min(m, n) values
+            const int q = std::min(m, n);
+            std::vector<cd> M(size_t(n) * q);
+            for (int c = 0; c < q; ++c)
+                for (int i = 0; i < n; ++i) M[i + size_t(c) * n] = scale[i] * Y[i + size_t(c) * n];
+            KFC_HIP(op->ctx, result.ensure(sizeof(S) * size_t(N) * std::max(q, 1)));
+            rc = blas_gemm<S>(w, V, N, n, M, q, result.as<S>(), N, N, 0.0);
+            return rc;
+        }
+        // restart: keep the lowest `keep` Ritz vectors and the residual direction
+        int keep = (3 * kd + 2 * conv) / 5;
+        keep = std::max(keep, std::min(m + 1, kd - 2));
+        keep = std::min(keep, kd - 2);
+        std::vector<cd> M(size_t(kd) * keep);
+        for (int c = 0; c < keep; ++c)
+            for (int i = 0; i < kd; ++i) M[i + size_t(c) * kd] = scale[i] * Y[i + size_t(c) * kd];
+        rc = blas_gemm<S>(w, V, N, kd, M, keep, V2, N, N, 0.0);
+        if (rc != KFC_OK) return rc;
+        rc = blas_axpby<S>(w, V + size_t(kd) * N, N, V2 + size_t(keep) * N, N, 1, N, scale[kd], 0.0);
+        if (rc != KFC_OK) return rc;
+        std::swap(V, V2);
+        std::fill(scale.begin(), scale.end(), 1.0);
+        std::fill(T.begin(), T.end(), cd(0));
+        for (int i = 0; i < keep; ++i) T[i + size_t(i) * kd] = theta[i];
+        k = keep;
+    }
+}
+
+template <class S>
+int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxiter, uint64_t seed, int mode,
+              double* Eout, double* Vout, kfc_stats* st) {
+    auto t0 = clk::now();
+    const int64_t N = op->N;
+    if (m < 1) return kfc_fail(op->ctx, KFC_EARG, "Lanczos requires m >= 1, got " + std::to_string(m));
+    if (m > N) return kfc_fail(op->ctx, KFC_EARG, "Lanczos: m = " + std::to_string(m) + " exceeds the operator dimension " + std::to_string(N));
+    int kd = int(std::max<int64_t>(2 * m + 10, krylovdim));   // src/solvers.jl:92
+    kd = int(std::min<int64_t>(kd, N));
+    if (kd > 63) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "Lanczos: krylovdim above 63 is not served");
+    Work w;
+    w.ctx = op->ctx;
+    Timer tm;
+    DevBuf res64;
+    LanczosOut<S> out;
+    int rc;
+    if (mode == KFC_MODE_MIXED) {
+        using L = typename Lower<S>::type;
+        double hb = 0;
+        kfc_op_gershgorin(op, &hb);
+        double hn = 0;
+        // the FP32 floor: a few units of roundoff of the largest eigenvalue scale
+        const double tol32 = std::max(tol, 64.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
+        LanczosOut<L> o32;
+        DevBuf res32;
+        rc = lanczos_core<L>(op, w, int(m), kd, tol32, maxiter, nullptr, 0, seed, tm, o32, res32);
+        if (rc != KFC_OK) return rc;
+        (void)hn;
+        // FP64 continuation from the FP32 Ritz vectors combined into one start vector
+        const int q = int(std::min<int64_t>(m, N));
+        DevBuf s64;
+        KFC_HIP(op->ctx, s64.ensure(sizeof(S) * size_t(N) * q));
+        rc = blas_convert<L, S>(w, res32.as<L>(), N, s64.as<S>(), N, q, N);
+        if (rc != KFC_OK) return rc;
+        std::vector<cd> comb(q);
+        std::mt19937_64 g(seed ^ 0x5eedull);
+        std::normal_distribution<double> nd;
+        for (int i = 0; i < q; ++i) comb[i] = 1.0 + 0.25 * nd(g);
+        DevBuf st64;
+        KFC_HIP(op->ctx, st64.ensure(sizeof(S) * size_t(N)));
+        rc = blas_gemm<S>(w, s64.as<S>(), N, q, comb, 1, st64.as<S>(), N, N, 0.0);
+        if (rc != KFC_OK) return rc;
+        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, st64.as<S>(), N, seed, tm, out, res64);
+        out.restarts += o32.restarts;
+    } else {
+        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, nullptr, 0, seed, tm, out, res64);
+    }
+    if (rc != KFC_OK) return rc;
+    const int n = int(out.theta.size());
+    if (n < m || out.converged < m)
+        return kfc_fail(op->ctx, KFC_ENOCONV,
+                        "Lanczos: only " + std::to_string(std::min(out.converged, int(m))) + " of " + std::to_string(m) +
+                            " eigenpairs converged (normres = " +
+                            std::to_string(out.resid.empty() ? 0.0 : out.resid[std::min<int64_t>(m, n) - 1]) + ")");
+    std::vector<cd> Vh;
+    rc = download_block<S>(op->ctx, res64.as<S>(), N, N, int(m), Vh);
+    if (rc != KFC_OK) return rc;
+    std::vector<double> E(out.theta.begin(), out.theta.begin() + m);
+    for (int64_t j = 0; j < m; ++j) Eout[j] = E[j];
+    write_host_V(op, Vh, Vout);
+    if (st) {
+        std::vector<double> res;
+        rc = true_residuals<S>(op, w, E, Vh, int(m), res, tm);
+        if (rc != KFC_OK) return rc;
+        *st = kfc_stats{};
+        st->max_residual = *std::max_element(res.begin(), res.end());
+        st->iterations = out.restarts;
+        st->n_matvec = tm.cols - m;   // the acceptance check is not part of the solve
+        st->n_matvec_f32 = tm.cols32;
+        st->n_apply_calls = tm.calls;
+        st->seconds_apply = tm.apply;
+        st->seconds_apply_gpu = tm.apply_gpu;
+        st->n_converged = int(m);
+        st->seconds_total = secs(t0);
+    }
+    return KFC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// LOBPCG with soft locking (Knyazev; basis [X, P, R] kept orthonormal as in
+// Hetmaniuk & Lehoucq), wrapped with the reference's seed, guard,
+// preconditioner, retry and acceptance rules (src/lobpcg.jl:217-301).
+
+struct LobpcgCfg {
+    int m, w;
+    double tol;
+    int64_t maxiter;
+    int precond;
+    double eta;
+};
+
+struct LobpcgRes {
+    std::vector<double> theta, resid;
+    std::vector<cd> X;   // host N x w
+    int64_t iterations = 0;
+    bool breakdown = false;
+    std::string why;
+};
+
+// Orthonormalize the columns [c0, c0+nc) of S against [0, c0) (twice) and among
+// themselves (eigendecomposition of the Gram, small directions dropped); the
+// surviving columns are compacted to [c0, c0+k).  AS follows when given.
+template <class S>
+int ortho_block(Work& w, S* Sb, int64_t N, int c0, int nc, S* tmp, int& kout, double drop_rel) {
+    int rc;
+    for (int pass = 0; pass < 2 && c0 > 0; ++pass) {
+        std::vector<cd> G;
+        rc = blas_gram<S>(w, Sb, N, c0, Sb + size_t(c0) * N, N, nc, N, G);
+        if (rc != KFC_OK) return rc;
+        for (auto& x : G) x = -x;
+        rc = blas_gemm<S>(w, Sb, N, c0, G, nc, Sb + size_t(c0) * N, N, N, 1.0);
+        if (rc != KFC_OK) return rc;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<cd> G;
+        rc = blas_gram<S>(w, Sb + size_t(c0) * N, N, nc, Sb + size_t(c0) * N, N, nc, N, G);
+        if (rc != KFC_OK) return rc;
+        std::vector<double> d(nc);
+        for (int i = 0; i < nc; ++i) d[i] = 1.0 / std::sqrt(std::max(G[i + size_t(i) * nc].real(), 1e-300));
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nc; ++i) G[i + size_t(j) * nc] *= d[i] * d[j];
+        std::vector<double> lam;
+        std::vector<cd> Z;
+        small_eig(nc, G, lam, Z);
+        const double top = lam.empty() ? 0 : lam.back();
+        std::vector<int> keep;
+        for (int i = nc - 1; i >= 0; --i)
+            if (top > 0 && lam[i] > drop_rel * top) keep.push_back(i);
+        const int k = int(keep.size());
+        std::vector<cd> M(size_t(nc) * k);
+        for (int c = 0; c < k; ++c)
+            for (int i = 0; i < nc; ++i) M[i + size_t(c) * nc] = d[i] * Z[i + size_t(keep[c]) * nc] / std::sqrt(lam[keep[c]]);
+        rc = blas_gemm<S>(w, Sb + size_t(c0) * N, N, nc, M, k, tmp, N, N, 0.0);
+        if (rc != KFC_OK) return rc;
+        rc = blas_axpby<S>(w, tmp, N, Sb + size_t(c0) * N, N, k, N, 1.0, 0.0);
+        if (rc != KFC_OK) return rc;
+        nc = k;
+        if (k == 0) break;
+    }
+    kout = nc;
+    return KFC_OK;
+}
+
+// Generalized small eigenproblem GA c = theta GB c with GB Hermitian positive definite
+// (Cholesky reduction); returns false when GB is not positive definite.
+bool gen_eig(int n, const std::vector<cd>& GA, const std::vector<cd>& GB, std::vector<double>& th, std::vector<cd>& C) {
+    std::vector<cd> L(size_t(n) * n, cd(0));
+    for (int j = 0; j < n; ++j) {
+        double s = GB[j + size_t(j) * n].real();
+        for (int k = 0; k < j; ++k) s -= std::norm(L[j + size_t(k) * n]);
+        if (!(s > 0)) return false;
+        const double ljj = std::sqrt(s);
+        L[j + size_t(j) * n] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            cd t = 0.5 * (GB[i + size_t(j) * n] + std::conj(GB[j + size_t(i) * n]));
+            for (int k = 0; k < j; ++k) t -= L[i + size_t(k) * n] * std::conj(L[j + size_t(k) * n]);
+            L[i + size_t(j) * n] = t / ljj;
+        }
+    }
+    // Linv by forward substitution
+    std::vector<cd> Li(size_t(n) * n, cd(0));
+    for (int c = 0; c < n; ++c) {
+        for (int i = 0; i < n; ++i) {
+            cd t = (i == c) ? cd(1) : cd(0);
+            for (int k = 0; k < i; ++k) t -= L[i + size_t(k) * n] * Li[k + size_t(c) * n];
+            Li[i + size_t(c) * n] = t / L[i + size_t(i) * n];
+        }
+    }
+    // G = Li GA Li^H
+    std::vector<cd> T1(size_t(n) * n, cd(0)), G(size_t(n) * n, cd(0));
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) {
+            const cd a = GA[k + size_t(j) * n];
+            for (int i = 0; i < n; ++i) T1[i + size_t(j) * n] += Li[i + size_t(k) * n] * a;
+        }
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) {
+            const cd b = std::conj(Li[j + size_t(k) * n]);
+            for (int i = 0; i < n; ++i) G[i + size_t(j) * n] += T1[i + size_t(k) * n] * b;
+        }
+    std::vector<cd> Z;
+    small_eig(n, G, th, Z);
+    // C = Li^H Z
+    C.assign(size_t(n) * n, cd(0));
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) {
+            const cd z = Z[k + size_t(j) * n];
+            for (int i = 0; i < n; ++i) C[i + size_t(j) * n] += std::conj(Li[k + size_t(i) * n]) * z;
+        }
+    return true;
+}
+
+template <class S>
+int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>& X0h, const double* Ddev,
+                Timer& tm, LobpcgRes& out, double switch_tol) {
+    const int64_t N = op->N;
+    const int W = cfg.w;
+    DevBuf sA, sB, aA, aB, tmpb;
+    const size_t blk = sizeof(S) * size_t(N) * 3 * W;
+    KFC_HIP(op->ctx, sA.ensure(blk));
+    KFC_HIP(op->ctx, sB.ensure(blk));
+    KFC_HIP(op->ctx, aA.ensure(blk));
+    KFC_HIP(op->ctx, aB.ensure(blk));
+    KFC_HIP(op->ctx, tmpb.ensure(sizeof(S) * size_t(N) * W));
+    S* Sb = sA.as<S>();
+    S* S2 = sB.as<S>();
+    S* AS = aA.as<S>();
+    S* AS2 = aB.as<S>();
+    S* tmp = tmpb.as<S>();
+    auto col = [&](S* b, int c) { return b + size_t(c) * N; };
+    int rc = upload_block<S>(op->ctx, X0h, N, W, Sb, N);
+    if (rc != KFC_OK) return rc;
+    int kx = 0;
+    rc = ortho_block<S>(w, Sb, N, 0, W, tmp, kx, 1e-14);
+    if (rc != KFC_OK) return rc;
+    if (kx < W) {
+        out.breakdown = true;
+        out.why = "the initial block is rank deficient";
+        return KFC_OK;
+    }
+    rc = apply_timed<S>(op, Sb, AS, W, N, tm);
+    if (rc != KFC_OK) return rc;
+    // Rayleigh-Ritz of X
+    std::vector<cd> G;
+    rc = blas_gram<S>(w, Sb, N, W, AS, N, W, N, G);
+    if (rc != KFC_OK) return rc;
+    std::vector<double> theta;
+    std::vector<cd> Z;
+    small_eig(W, G, theta, Z);
+    rc = blas_gemm<S>(w, Sb, N, W, Z, W, S2, N, N, 0.0);
+    if (rc == KFC_OK) rc = blas_gemm<S>(w, AS, N, W, Z, W, AS2, N, N, 0.0);
+    if (rc != KFC_OK) return rc;
+    std::swap(Sb, S2);
+    std::swap(AS, AS2);
+    int np = 0, nlock = 0;
+    std::vector<double> res(W, INFINITY);
+    std::vector<double> prec_theta;
+    bool first_prep = true;
+    int64_t it = 0;
+    for (;; ++it) {
+        const int na = W - nlock;
+        // residuals of the active columns, into the R slot
+        const int cR = W + np;
+        std::vector<double> th_act(theta.begin() + nlock, theta.end()), rn;
+        rc = blas_residual<S>(w, col(Sb, cR), N, col(AS, nlock), N, col(Sb, nlock), N, na, N, th_act, rn);
+        if (rc != KFC_OK) return rc;
+        for (int j = 0; j < na; ++j) res[nlock + j] = rn[j];
+        for (double r : rn)
+            if (!std::isfinite(r)) {
+                out.breakdown = true;
+                out.why = "non-finite residual";
+                return KFC_OK;
+            }
+        int newly = 0;
+        while (nlock + newly < W && res[nlock + newly] <= cfg.tol) ++newly;
+        out.iterations = it;
+        bool done = nlock + newly >= W;
+        double worst = 0;
+        for (int j = nlock; j < W; ++j) worst = std::max(worst, res[j]);
+        if (done || it >= cfg.maxiter || (switch_tol > 0 && worst <= switch_tol)) {
+            nlock += newly;
+            break;
+        }
+        // drop the newly locked residual columns (they are a prefix of the active ones)
+        if (newly > 0) {
+            rc = blas_axpby<S>(w, col(Sb, cR + newly), N, tmp, N, na - newly, N, 1.0, 0.0);
+            if (rc == KFC_OK) rc = blas_axpby<S>(w, tmp, N, col(Sb, cR), N, na - newly, N, 1.0, 0.0);
+            if (rc != KFC_OK) return rc;
+        }
+        nlock += newly;
+        const int nact = W - nlock;
+        // preconditioner (precondprep! + ldiv!, src/lobpcg.jl:98-119)
+        if (cfg.precond == KFC_PRECOND_RITZ) {
+            if (first_prep) {
+                rc = blas_diag_expect<S>(w, col(Sb, nlock), N, nact, N, Ddev, prec_theta);
+                if (rc != KFC_OK) return rc;
+                first_prep = false;
+            } else {
+                prec_theta.assign(theta.begin() + nlock, theta.end());
+            }
+            rc = blas_precond<S>(w, col(Sb, cR), N, nact, N, Ddev, prec_theta, cfg.eta);
+        } else if (cfg.precond == KFC_PRECOND_JACOBI) {
+            rc = blas_precond<S>(w, col(Sb, cR), N, nact, N, Ddev + N, {}, -1.0);
+        }
+        if (rc != KFC_OK) return rc;
+        int nr = 0;
+        rc = ortho_block<S>(w, Sb, N, cR, nact, tmp, nr, 1e-12);
+        if (rc != KFC_OK) return rc;
+        if (nr == 0) {
+            out.breakdown = true;
+            out.why = "the preconditioned residual block vanished";
+            return KFC_OK;
+        }
+        rc = apply_timed<S>(op, col(Sb, cR), col(AS, cR), nr, N, tm);
+        if (rc != KFC_OK) return rc;
+        const int ns = W + np + nr;
+        std::vector<cd> GA, GB;
+        rc = blas_gram<S>(w, Sb, N, ns, AS, N, ns, N, GA);
+        if (rc == KFC_OK) rc = blas_gram<S>(w, Sb, N, ns, Sb, N, ns, N, GB);
+        if (rc != KFC_OK) return rc;
+        std::vector<double> th_all;
+        std::vector<cd> C;
+        int nsu = ns;
+        if (!gen_eig(ns, GA, GB, th_all, C)) {
+            // drop P and retry with [X, R]
+            if (np == 0) {
+                out.breakdown = true;
+                out.why = "the Rayleigh-Ritz Gram matrix is not positive definite";
+                return KFC_OK;
+            }
+            rc = blas_axpby<S>(w, col(Sb, cR), N, tmp, N, nr, N, 1.0, 0.0);
+            if (rc == KFC_OK) rc = blas_axpby<S>(w, tmp, N, col(Sb, W), N, nr, N, 1.0, 0.0);
+            if (rc == KFC_OK) rc = blas_axpby<S>(w, col(AS, cR), N, tmp, N, nr, N, 1.0, 0.0);
+            if (rc == KFC_OK) rc = blas_axpby<S>(w, tmp, N, col(AS, W), N, nr, N, 1.0, 0.0);
+            if (rc != KFC_OK) return rc;
+            np = 0;
+            nsu = W + nr;
+            std::vector<cd> ga2(size_t(nsu) * nsu), gb2(size_t(nsu) * nsu);
+            rc = blas_gram<S>(w, Sb, N, nsu, AS, N, nsu, N, ga2);
+            if (rc == KFC_OK) rc = blas_gram<S>(w, Sb, N, nsu, Sb, N, nsu, N, gb2);
+            if (rc != KFC_OK) return rc;
+            if (!gen_eig(nsu, ga2, gb2, th_all, C)) {
+                out.breakdown = true;
+                out.why = "the Rayleigh-Ritz Gram matrix is not positive definite";
+                return KFC_OK;
+            }
+        }
+        // new X: lowest W Ritz vectors; new P: the active ones' components outside X,
+        // orthonormalized in coefficient space against the new X coefficients
+        std::vector<cd> M(size_t(nsu) * (2 * W), cd(0));
+        for (int c = 0; c < W; ++c)
+            for (int i = 0; i < nsu; ++i) M[i + size_t(c) * nsu] = C[i + size_t(c) * nsu];
+        // GB-orthonormal coefficient vectors for P: zero the X rows of the active columns,
+        // then project out the new X coefficients and orthonormalize (in the GB inner product)
+        std::vector<std::vector<cd>> pcols;
+        // With S nearly orthonormal, use the Euclidean inner product of coefficients weighted by GB.
+        std::vector<cd> GBu(size_t(nsu) * nsu);
+        if (nsu == ns) GBu = GB;
+        else {
+            std::vector<cd> g2;
+            rc = blas_gram<S>(w, Sb, N, nsu, Sb, N, nsu, N, g2);
+            if (rc != KFC_OK) return rc;
+            GBu = g2;
+        }
+        auto ip = [&](const cd* a, const cd* b) {
+            cd s = 0;
+            for (int j = 0; j < nsu; ++j) {
+                cd t = 0;
+                for (int i = 0; i < nsu; ++i) t += std::conj(a[i]) * GBu[i + size_t(j) * nsu];
+                s += t * b[j];
+            }
+            return s;
+        };
+        int npn = 0;
+        for (int c = nlock; c < W; ++c) {
+            std::vector<cd> v(nsu, cd(0));
+            for (int i = W; i < nsu; ++i) v[i] = C[i + size_t(c) * nsu];
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int q = 0; q < W; ++q) {
+                    const cd* x = &C[size_t(q) * nsu];
+                    const cd s = ip(x, v.data());
+                    for (int i = 0; i < nsu; ++i) v[i] -= s * x[i];
+                }
+                for (auto& pc : pcols) {
+                    const cd s = ip(pc.data(), v.data());
+                    for (int i = 0; i < nsu; ++i) v[i] -= s * pc[i];
+                }
+            }
+            const double nv = std::sqrt(std::max(ip(v.data(), v.data()).real(), 0.0));
+            if (nv < 1e-10) continue;
+            for (auto& x : v) x /= nv;
+            pcols.push_back(v);
+            ++npn;
+        }
+        for (int c = 0; c < npn; ++c)
+            for (int i = 0; i < nsu; ++i) M[i + size_t(W + c) * nsu] = pcols[c][i];
+        M.resize(size_t(nsu) * (W + npn));
+        rc = blas_gemm<S>(w, Sb, N, nsu, M, W + npn, S2, N, N, 0.0);
+        if (rc == KFC_OK) rc = blas_gemm<S>(w, AS, N, nsu, M, W + npn, AS2, N, N, 0.0);
+        if (rc != KFC_OK) return rc;
+        std::swap(Sb, S2);
+        std::swap(AS, AS2);
+        np = npn;
+        theta.assign(th_all.begin(), th_all.begin() + W);
+    }
+    out.theta = theta;
+    out.resid = res;
+    // final residuals of every column (fresh from AX)
+    std::vector<double> rn;
+    rc = blas_residual<S>(w, tmp, N, AS, N, Sb, N, W, N, theta, rn);
+    if (rc != KFC_OK) return rc;
+    out.resid = rn;
+    return download_block<S>(op->ctx, Sb, N, N, W, out.X);
+}
+
+// MersenneTwister-free seeded perturbation of a host block: each column plus a
+// random vector of norm sqrt(eps) times the column's norm (_perturbed_seed, src/lobpcg.jl:167-177).
+void perturb(std::vector<cd>& X, int64_t N, int p, bool cplx, uint64_t seed) {
+    std::mt19937_64 g(seed);
+    std::normal_distribution<double> nd;
+    const double delta = std::sqrt(2.220446049250313e-16);
+    for (int j = 0; j < p; ++j) {
+        cd* c = &X[size_t(j) * N];
+        std::vector<cd> r(N);
+        double nc = 0, nr = 0;
+        for (int64_t i = 0; i < N; ++i) {
+            r[i] = cplx ? cd(nd(g), nd(g)) / std::sqrt(2.0) : cd(nd(g), 0);
+            nc += std::norm(c[i]);
+            nr += std::norm(r[i]);
+        }
+        const double f = delta * std::sqrt(nc) / std::sqrt(nr);
+        for (int64_t i = 0; i < N; ++i) c[i] += f * r[i];
+    }
+}
+
+void random_block(std::vector<cd>& X, int64_t N, int p, bool cplx, uint64_t seed) {
+    std::mt19937_64 g(seed);
+    std::normal_distribution<double> nd;
+    X.resize(size_t(N) * p);
+    for (auto& x : X) x = cplx ? cd(nd(g), nd(g)) : cd(nd(g), 0);
+}
+
+template <class S>
+int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int precond, double eta, int seed_kind,
+             const double* X0, uint64_t seed, int mode, double* Eout, double* Vout, kfc_stats* st) {
+    auto t0 = clk::now();
+    const int64_t N = op->N;
+    if (m < 1) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG requires m >= 1, got " + std::to_string(m));
+    if (N < 3 * m + 5)
+        return kfc_fail(op->ctx, KFC_EARG, "LOBPCG needs N >= 3m + 5, got N = " + std::to_string(N) + " for m = " +
+                                               std::to_string(m) + "; the block eigensolver is unreliable on so small a problem, use Dense()");
+    if (!(tol > 0) || !std::isfinite(tol)) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: tol must be positive and finite");
+    if (maxiter < 1) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: maxiter must be at least 1");
+    if (precond == KFC_PRECOND_RITZ && !(eta > 0 && std::isfinite(eta)))
+        return kfc_fail(op->ctx, KFC_EARG, "eta must be positive and finite");
+    if (guard < 0) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: guard must be nonnegative");
+    std::vector<cd> X0h;
+    if (X0) {
+        read_host_V(op, X0, size_t(N) * m, X0h);
+        for (const cd& x : X0h)
+            if (!std::isfinite(x.real()) || !std::isfinite(x.imag()))
+                return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: X0 has non-finite entries");
+        for (int64_t j = 0; j < m; ++j) {
+            bool z = true;
+            for (int64_t i = 0; i < N; ++i) z &= X0h[i + j * N] == cd(0);
+            if (z) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: column " + std::to_string(j + 1) + " of X0 is zero");
+        }
+    }
+    const int g = int(std::max<int64_t>(0, std::min<int64_t>(guard, (N - 5) / 3 - m)));   // src/lobpcg.jl:243
+    const int W = int(m) + g;
+    std::vector<cd> P;
+    if (seed_kind == KFC_SEED_PRODUCT) {
+        int rc = op_product_seed(op, W, P);
+        if (rc != KFC_OK) return rc;
+    }
+    // preconditioner diagonal on the device: D (and 1/(D - sigma) for :jacobi)
+    std::vector<double> D;
+    DevBuf dD;
+    if (precond != KFC_PRECOND_NONE) {
+        op_full_diagonal(op, D);
+        if (precond == KFC_PRECOND_JACOBI) {
+            double dmin = INFINITY;
+            for (double x : D) {
+                if (!std::isfinite(x)) return kfc_fail(op->ctx, KFC_EARG, "FixedJacobi: the diagonal has non-finite entries");
+                dmin = std::min(dmin, x);
+            }
+            const double sigma = dmin - 1;
+            if (!(sigma < dmin))
+                return kfc_fail(op->ctx, KFC_EARG, "FixedJacobi: the shift minimum(D) - 1 is not representable below minimum(D)");
+            const size_t n0 = D.size();
+            D.resize(2 * n0);
+            for (size_t i = 0; i < n0; ++i) D[n0 + i] = 1.0 / (D[i] - sigma);
+        }
+        KFC_HIP(op->ctx, dD.ensure(sizeof(double) * D.size()));
+        KFC_HIP(op->ctx, hipMemcpy(dD.p, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
+    }
+    // initial block (_seed_columns, src/lobpcg.jl:182-190)
+    auto seed_block = [&](bool random_only, uint64_t s) {
+        std::vector<cd> X(size_t(N) * W);
+        if (random_only) {
+            random_block(X, N, W, op->cplx, s);
+            return X;
+        }
+        if (X0) {
+            std::vector<cd> F = X0h;
+            perturb(F, N, int(m), op->cplx, 0x4b46);
+            std::copy(F.begin(), F.end(), X.begin());
+            if (!P.empty() && g > 0) {
+                std::vector<cd> Gd(P.begin() + size_t(m) * N, P.end());
+                perturb(Gd, N, g, op->cplx, 0x4b46);
+                std::copy(Gd.begin(), Gd.end(), X.begin() + size_t(m) * N);
+            } else if (g > 0) {
+                std::vector<cd> R;
+                random_block(R, N, g, op->cplx, s);
+                std::copy(R.begin(), R.end(), X.begin() + size_t(m) * N);
+            }
+        } else if (!P.empty()) {
+            X = P;
+            perturb(X, N, W, op->cplx, 0x4b46);
+        } else {
+            random_block(X, N, W, op->cplx, s);
+        }
+        return X;
+    };
+    Work w;
+    w.ctx = op->ctx;
+    Timer tm;
+    LobpcgCfg cfg{int(m), W, tol, maxiter, precond, eta};
+    const double* Dp = dD.as<double>();
+    LobpcgRes res;
+    bool retried = false;
+    auto run = [&](const std::vector<cd>& Xs) -> int {
+        res = LobpcgRes();
+        if (mode == KFC_MODE_MIXED) {
+            using L = typename Lower<S>::type;
+            double hb = 0;
+            kfc_op_gershgorin(op, &hb);
+            const double sw = std::max(tol, 256.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
+            LobpcgRes r32;
+            int rc = lobpcg_core<L>(op, w, cfg, Xs, Dp, tm, r32, sw);
+            if (rc != KFC_OK) return rc;
+            if (r32.breakdown) {
+                res = r32;
+                return KFC_OK;
+            }
+            LobpcgCfg c2 = cfg;
+            c2.maxiter = std::max<int64_t>(1, cfg.maxiter - r32.iterations);
+            rc = lobpcg_core<S>(op, w, c2, r32.X, Dp, tm, res, 0.0);
+            res.iterations += r32.iterations;
+            return rc;
+        }
+        return lobpcg_core<S>(op, w, cfg, Xs, Dp, tm, res, 0.0);
+    };
+    int rc = run(seed_block(seed_kind == KFC_SEED_RANDOM && !X0, seed));
+    if (rc != KFC_OK) return rc;
+    if (res.breakdown) {
+        // one retry from a random block (src/lobpcg.jl:272-289)
+        if (!X0 && seed_kind != KFC_SEED_PRODUCT)
+            return kfc_fail(op->ctx, KFC_EBREAKDOWN, "LOBPCG: the eigensolver failed (" + res.why +
+                                                         "). An exactly zero residual block can break its orthogonalization; use Dense() for such operators");
+        retried = true;
+        rc = run(seed_block(true, seed + 1));
+        if (rc != KFC_OK) return rc;
+        if (res.breakdown)
+            return kfc_fail(op->ctx, KFC_EBREAKDOWN, "LOBPCG: the eigensolver failed (" + res.why +
+                                                         "). An exactly zero residual block can break its orthogonalization; use Dense() for such operators");
+    }
+    int bad = 0;
+    double worst = 0;
+    for (int64_t j = 0; j < m; ++j) {
+        if (!(res.resid[j] <= tol)) ++bad;
+        worst = std::max(worst, res.resid[j]);
+    }
+    if (bad > 0)
+        return kfc_fail(op->ctx, KFC_ENOCONV, "LOBPCG: " + std::to_string(bad) + " of " + std::to_string(m) +
+                                                  " eigenpairs did not converge within " + std::to_string(maxiter) +
+                                                  " iterations (only " + std::to_string(m - bad) +
+                                                  " converged; maximum residual " + std::to_string(worst) +
+                                                  " > tol = " + std::to_string(tol) + ")");
+    for (int64_t j = 0; j < m; ++j) Eout[j] = res.theta[j];
+    std::vector<cd> Vh(res.X.begin(), res.X.begin() + size_t(N) * m);
+    write_host_V(op, Vh, Vout);
+    if (st) {
+        std::vector<double> E(res.theta.begin(), res.theta.begin() + m), tr;
+        const int64_t cols_solve = tm.cols;
+        rc = true_residuals<S>(op, w, E, Vh, int(m), tr, tm);
+        if (rc != KFC_OK) return rc;
+        *st = kfc_stats{};
+        st->iterations = res.iterations;
+        st->n_matvec = cols_solve;
+        st->n_matvec_f32 = tm.cols32;
+        st->n_apply_calls = tm.calls;
+        st->seconds_apply = tm.apply;
+        st->seconds_apply_gpu = tm.apply_gpu;
+        st->max_residual = *std::max_element(tr.begin(), tr.end());
+        st->n_converged = int(m);
+        st->warm = X0 ? 1 : 0;
+        st->retried = retried ? 1 : 0;
+        st->seconds_total = secs(t0);
+    }
+    return KFC_OK;
+}
+
+template <class S>
+int residuals_t(kfc_op* op, int64_t m, const double* E, const double* V, double* out) {
+    Work w;
+    w.ctx = op->ctx;
+    Timer tm;
+    std::vector<cd> Vh;
+    read_host_V(op, V, size_t(op->N) * m, Vh);
+    std::vector<double> Ev(E, E + m), r;
+    int rc = true_residuals<S>(op, w, Ev, Vh, int(m), r, tm);
+    if (rc != KFC_OK) return rc;
+    std::copy(r.begin(), r.end(), out);
+    return KFC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats) {
+    if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_dense: null argument");
+    if (m < 0) return kfc_fail(op->ctx, KFC_EARG, "Dense requires m >= 0, got " + std::to_string(m));
+    return op->cplx ? dense_t<Cx<double>>(op, m, E, V, stats) : dense_t<double>(op, m, E, V, stats);
+}
+
+int kfc_lanczos(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxiter, uint64_t seed, int32_t mode,
+                double* E, double* V, kfc_stats* stats) {
+    if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_lanczos: null argument");
+    return op->cplx ? lanczos_t<Cx<double>>(op, m, krylovdim, tol, maxiter, seed, mode, E, V, stats)
+                    : lanczos_t<double>(op, m, krylovdim, tol, maxiter, seed, mode, E, V, stats);
+}
+
+int kfc_lobpcg(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond, double eta,
+               int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode, double* E, double* V,
+               kfc_stats* stats) {
+    if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_lobpcg: null argument");
+    return op->cplx ? lobpcg_t<Cx<double>>(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V, stats)
+                    : lobpcg_t<double>(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V, stats);
+}
+
+int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid) {
+    if (!op || !E || !V || !resid) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_residuals: null argument");
+    return op->cplx ? residuals_t<Cx<double>>(op, m, E, V, resid) : residuals_t<double>(op, m, E, V, resid);
+}
+
+}  // extern "C"

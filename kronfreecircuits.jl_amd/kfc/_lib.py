@@ -1,0 +1,178 @@
+"""ctypes binding of libkfc.so (include/kfc.h).
+
+The library is loaded from this directory; a missing library raises at import
+of any entry point that needs it (there is no CPU fallback)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkfc.so")
+
+KFC_OK, KFC_EARG, KFC_EDIM, KFC_ENOCONV, KFC_EBREAKDOWN, KFC_EDEVICE, KFC_ENOMEM, KFC_EUNSUPPORTED = range(8)
+KFC_F64, KFC_C128 = 0, 1
+KFC_PREC_F64, KFC_PREC_F32 = 0, 1
+KFC_MODE_F64, KFC_MODE_MIXED = 0, 1
+KFC_DIAG, KFC_BANDED, KFC_SHIFT, KFC_DENSE = 0, 1, 2, 3
+KFC_PRECOND = {"ritz": 0, "jacobi": 1, "none": 2}
+KFC_SEED = {"product": 0, "random": 1}
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError."""
+
+
+class DimensionMismatch(ValueError):
+    """The reference's DimensionMismatch."""
+
+
+class ConvergenceError(RuntimeError):
+    """An eigensolver's "did not converge" error."""
+
+
+class BreakdownError(RuntimeError):
+    """An eigensolver breakdown (orthogonalization failure)."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure."""
+
+
+class UnsupportedError(RuntimeError):
+    """A combination this build does not serve."""
+
+
+_ERR = {KFC_EARG: ArgumentError, KFC_EDIM: DimensionMismatch, KFC_ENOCONV: ConvergenceError,
+        KFC_EBREAKDOWN: BreakdownError, KFC_EDEVICE: DeviceError, KFC_ENOMEM: DeviceError,
+        KFC_EUNSUPPORTED: UnsupportedError}
+
+
+class FactorDesc(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("kind", C.c_int32), ("d", C.c_int64), ("is_complex", C.c_int32),
+                ("nbands", C.c_int32), ("offsets", C.POINTER(C.c_int64)), ("weights", C.POINTER(C.c_double)),
+                ("shift", C.c_int64), ("factor_re", C.c_double), ("factor_im", C.c_double),
+                ("matrix", C.POINTER(C.c_double)), ("diag", C.POINTER(C.c_double))]
+
+
+class TermDesc(C.Structure):
+    _fields_ = [("coeff_re", C.c_double), ("coeff_im", C.c_double), ("nfactors", C.c_int32),
+                ("factors", C.POINTER(FactorDesc))]
+
+
+class OpDesc(C.Structure):
+    _fields_ = [("ndims", C.c_int32), ("dims", C.POINTER(C.c_int64)), ("dtype", C.c_int32),
+                ("constant_re", C.c_double), ("constant_im", C.c_double), ("nterms", C.c_int32),
+                ("terms", C.POINTER(TermDesc)), ("block_capacity", C.c_int64), ("hermitian", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("n_matvec", C.c_int64), ("n_matvec_f32", C.c_int64),
+                ("n_apply_calls", C.c_int64), ("seconds_total", C.c_double), ("seconds_apply", C.c_double), ("seconds_apply_gpu", C.c_double),
+                ("max_residual", C.c_double), ("n_converged", C.c_int32), ("warm", C.c_int32),
+                ("degree_last", C.c_int32), ("retried", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol include/kfc.h declares, with its signature (restype, argtypes)
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+SIGNATURES = {
+    "kfc_ctx_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "kfc_ctx_destroy": (C.c_int, [_P]),
+    "kfc_last_error": (C.c_char_p, [_P]),
+    "kfc_ctx_stream": (_P, [_P]),
+    "kfc_op_create": (C.c_int, [_P, C.POINTER(OpDesc), C.POINTER(_P)]),
+    "kfc_op_destroy": (C.c_int, [_P]),
+    "kfc_op_size": (C.c_int64, [_P]),
+    "kfc_op_fused": (C.c_int, [_P]),
+    "kfc_apply": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_int32]),
+    "kfc_sync": (C.c_int, [_P]),
+    "kfc_op_diagonal": (C.c_int, [_P, _D]),
+    "kfc_op_gershgorin": (C.c_int, [_P, _D]),
+    "kfc_op_product_seed": (C.c_int, [_P, C.c_int64, _D]),
+    "kfc_lanczos": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_int32, _D, _D,
+                              C.POINTER(Stats)]),
+    "kfc_lobpcg": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int32, C.c_double, C.c_int32, _D,
+                             C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
+    "kfc_chebyshev": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_double,
+                                C.c_int32, C.c_double, C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
+    "kfc_dense": (C.c_int, [_P, C.c_int64, _D, _D, C.POINTER(Stats)]),
+    "kfc_residuals": (C.c_int, [_P, C.c_int64, _D, _D, _D]),
+    "kfc_batch_transmons": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _D, C.c_int32,
+                                      C.c_int32, C.c_double, C.c_int32, _D, _D, C.POINTER(Stats)]),
+    "kfc_mlp_create": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.c_uint64, C.POINTER(_P)]),
+    "kfc_mlp_destroy": (C.c_int, [_P]),
+    "kfc_mlp_set_params": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "kfc_mlp_get_params": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "kfc_mlp_num_params": (C.c_int64, [_P]),
+    "kfc_mlp_forward": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)]),
+    "kfc_mlp_backward": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)]),
+    "kfc_mlp_train": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int64, C.c_int64,
+                                C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libkfc.so with typed entry points (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libkfc.so is not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def missing_symbols():
+    """Symbols declared in include/kfc.h that the loaded library does not export."""
+    L = lib()
+    return [n for n in SIGNATURES if not hasattr(L, n)]
+
+
+def check(rc, ctx=None):
+    if rc != KFC_OK:
+        msg = lib().kfc_last_error(ctx).decode(errors="replace")
+        raise _ERR.get(rc, RuntimeError)(msg)
+
+
+class Context:
+    """A device context (HIP device + stream); one per device per process."""
+
+    def __init__(self, device=0):
+        h = _P()
+        check(lib().kfc_ctx_create(C.byref(h), int(device)), None)
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        check(lib().kfc_sync(self.handle), self.handle)
+
+    @property
+    def stream(self):
+        return lib().kfc_ctx_stream(self.handle)
+
+    def __del__(self):
+        try:
+            if self.handle and _lib is not None:
+                _lib.kfc_ctx_destroy(self.handle)
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(device=0):
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]

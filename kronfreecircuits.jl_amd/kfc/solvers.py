@@ -1,0 +1,187 @@
+"""The Solver plugin surface (src/solvers.jl:3-31) and its keyword structs
+(src/tree.jl:6-351), served by libkfc on the device.
+
+``solve(s, H, m; X0)`` returns ``(E, V)`` on the host with ``E`` ascending and
+``V`` of shape ``(N, m)``; ``solve_stats`` also returns the work counters.
+Every solver accepts ``mode="f64"`` (the plain FP64 path) or ``mode="mixed"``
+(FP32 applies for the bulk iteration, FP64 Rayleigh-Ritz, reorthogonalization
+and a final FP64 refinement), a solver field added beside the reference's
+unchanged keywords."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .operator import pack_block, unpack_block
+
+_MODES = {"f64": L.KFC_MODE_F64, "mixed": L.KFC_MODE_MIXED}
+
+
+def _mode(m):
+    if m not in _MODES:
+        raise L.ArgumentError(f"mode must be 'f64' or 'mixed', got {m!r}")
+    return _MODES[m]
+
+
+class Solver:
+    pass
+
+
+class Dense(Solver):
+    """Full dense diagonalization (src/tree.jl:4)."""
+
+    def __repr__(self):
+        return "Dense()"
+
+
+class Lanczos(Solver):
+    """``Lanczos(; tol=1e-10, krylovdim=0, maxiter=1000, verbosity=0, on)`` (src/tree.jl:6-23)."""
+
+    def __init__(self, tol=1e-10, krylovdim=0, maxiter=1000, verbosity=0, on=None, mode="f64", seed=0x4c5a):
+        self.tol, self.krylovdim, self.maxiter, self.verbosity, self.on = tol, krylovdim, maxiter, verbosity, on
+        self.mode, self.seed = mode, seed
+        _mode(mode)
+
+
+class LOBPCG(Solver):
+    """``LOBPCG(; tol=1e-8, maxiter=1000, preconditioner=:ritz, eta=0.01, guard=4,
+    seed=:product, on, callback)`` (src/tree.jl:66-87)."""
+
+    def __init__(self, tol=1e-8, maxiter=1000, preconditioner="ritz", eta=0.01, guard=4, seed="product", on=None,
+                 callback=None, mode="f64", rng_seed=0x4b46):
+        if not (tol > 0 and math.isfinite(tol)):
+            raise L.ArgumentError(f"LOBPCG: tol must be positive and finite, got {tol}")
+        if maxiter < 1:
+            raise L.ArgumentError(f"LOBPCG: maxiter must be at least 1, got {maxiter}")
+        if preconditioner not in ("ritz", "jacobi", "none"):
+            raise L.ArgumentError(f"LOBPCG: preconditioner must be :ritz, :jacobi or :none, got :{preconditioner}")
+        if not (eta > 0 and math.isfinite(eta)):
+            raise L.ArgumentError(f"LOBPCG: eta must be positive and finite, got {eta}")
+        if guard < 0:
+            raise L.ArgumentError(f"LOBPCG: guard must be nonnegative, got {guard}")
+        if seed not in ("product", "random"):
+            raise L.ArgumentError(f"LOBPCG: seed must be :product or :random, got {seed!r}")
+        self.tol, self.maxiter, self.preconditioner, self.eta = tol, maxiter, preconditioner, eta
+        self.guard, self.seed, self.on, self.callback, self.mode, self.rng_seed = guard, seed, on, callback, mode, rng_seed
+        _mode(mode)
+
+
+class ChebyshevFilter(Solver):
+    """``ChebyshevFilter(; tol=1e-8, degree=100, extra=40, maxiter=50, rescale_every=20,
+    ortho=:svqb, bound=:gershgorin, seed=:product, on, callback)`` (src/tree.jl:293-321)."""
+
+    def __init__(self, tol=1e-8, degree=100, extra=40, maxiter=50, rescale_every=20, ortho="svqb", bound="gershgorin",
+                 seed="product", on=None, callback=None, mode="f64", rng_seed=0x4346):
+        if not (tol > 0 and math.isfinite(tol)):
+            raise L.ArgumentError(f"ChebyshevFilter: tol must be positive and finite, got {tol}")
+        if not 1 <= degree <= 200:
+            raise L.ArgumentError(f"ChebyshevFilter: degree must lie in 1:200, got {degree}")
+        if extra < 2:
+            raise L.ArgumentError("ChebyshevFilter: extra must be at least 2 so at least one column serves as a "
+                                  f"convergence guard; the top column sets the filter cut and cannot be one, got {extra}")
+        if maxiter < 1:
+            raise L.ArgumentError(f"ChebyshevFilter: maxiter must be at least 1, got {maxiter}")
+        if rescale_every < 1:
+            raise L.ArgumentError(f"ChebyshevFilter: rescale_every must be at least 1, got {rescale_every}")
+        if ortho not in ("svqb", "cholqr"):
+            raise L.ArgumentError(f"ChebyshevFilter: ortho must be :svqb or :cholqr, got {ortho!r}")
+        if not (bound == "gershgorin" or (isinstance(bound, (int, float)) and math.isfinite(bound))):
+            raise L.ArgumentError(f"ChebyshevFilter: bound must be :gershgorin or a finite number, got {bound!r}")
+        if seed not in ("product", "random"):
+            raise L.ArgumentError(f"ChebyshevFilter: seed must be :product or :random, got {seed!r}")
+        self.tol, self.degree, self.extra, self.maxiter = tol, degree, extra, maxiter
+        self.rescale_every, self.ortho, self.bound, self.seed = rescale_every, ortho, bound, seed
+        self.on, self.callback, self.mode, self.rng_seed = on, callback, mode, rng_seed
+        _mode(mode)
+
+
+class Auto(Solver):
+    """``Auto(; dense_cutoff=1000, on)`` (src/tree.jl:343-351): Dense up to the
+    cutoff (or when N < 3m+5), ChebyshevFilter at a device leaf, LOBPCG at a branch."""
+
+    def __init__(self, dense_cutoff=1000, on=None, mode="f64"):
+        if dense_cutoff < 0:
+            raise L.ArgumentError(f"Auto: dense_cutoff must be nonnegative, got {dense_cutoff}")
+        self.dense_cutoff, self.on, self.mode = dense_cutoff, on, mode
+
+
+def resolve(s, info):
+    """``resolve(::Auto, info)`` (src/materialize.jl:550-559); other solvers resolve to themselves."""
+    if not isinstance(s, Auto):
+        return s
+    n = int(np.prod(info["dims"]))
+    m = info["m"]
+    if n <= s.dense_cutoff or (0 < m and n < 3 * m + 5):
+        return Dense()
+    if info["isleaf"]:
+        return ChebyshevFilter(mode=s.mode)
+    return LOBPCG(preconditioner="ritz", seed="product", mode=s.mode)
+
+
+def _out(H, m):
+    E = np.empty(m)
+    V = np.empty(H.N * m * (2 if H.dtype == np.complex128 else 1))
+    return E, V
+
+
+def solve_stats(s, H, m, X0=None):
+    """``solve(s, H, m; X0)`` plus the solver's work counters."""
+    lib = L.lib()
+    st = L.Stats()
+    if isinstance(s, Dense):
+        k = m if 0 < m < H.N else H.N
+        E, V = _out(H, k)
+        L.check(lib.kfc_dense(H.handle, int(m), E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st)),
+                H.ctx.handle)
+        return E, unpack_block(V, H.N, k, H.dtype), st.as_dict()
+    E, V = _out(H, m)
+    if isinstance(s, Lanczos):
+        rc = lib.kfc_lanczos(H.handle, int(m), int(s.krylovdim), float(s.tol), int(s.maxiter), int(s.seed),
+                             _mode(s.mode), E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
+    elif isinstance(s, LOBPCG):
+        x0 = None
+        if X0 is not None:
+            X0 = np.asarray(X0)
+            if X0.shape != (H.N, m):
+                raise L.DimensionMismatch(f"LOBPCG: X0 has size {X0.shape}, expected ({H.N}, {m})")
+            if H.dtype == np.float64 and np.iscomplexobj(X0):
+                if np.any(np.imag(X0) != 0):
+                    raise L.ArgumentError("LOBPCG: the warm-start block X0 is complex but the operator is real; "
+                                          "pass a real block or complexify the operator")
+                X0 = np.real(X0)
+            x0 = pack_block(X0, H.dtype)
+        rc = lib.kfc_lobpcg(H.handle, int(m), int(s.guard), float(s.tol), int(s.maxiter),
+                            L.KFC_PRECOND[s.preconditioner], float(s.eta), L.KFC_SEED[s.seed],
+                            x0.ctypes.data_as(L._D) if x0 is not None else None, int(s.rng_seed), _mode(s.mode),
+                            E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
+    elif isinstance(s, ChebyshevFilter):
+        nb = min(m + s.extra, H.N)
+        rc = lib.kfc_chebyshev(H.handle, int(m), int(s.degree), int(s.extra), int(s.maxiter), int(s.rescale_every),
+                               0 if s.ortho == "svqb" else 1, math.nan if s.bound == "gershgorin" else float(s.bound),
+                               L.KFC_SEED[s.seed], float(s.tol), int(s.rng_seed), _mode(s.mode),
+                               E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
+        del nb
+    else:
+        raise L.ArgumentError(f"no device method for solver {type(s).__name__}")
+    L.check(rc, H.ctx.handle)
+    return E, unpack_block(V, H.N, m, H.dtype), st.as_dict()
+
+
+def solve(s, H, m, X0=None):
+    """``solve(s, H, m; X0) -> (E, V)`` (src/solvers.jl:3-31)."""
+    E, V, _ = solve_stats(s, H, m, X0)
+    return E, V
+
+
+def residuals(H, E, V):
+    """Fresh FP64 true residuals ||H v_j - E_j v_j|| (src/solvers.jl:65-69)."""
+    m = len(E)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    Vb = pack_block(V, H.dtype)
+    out = np.empty(m)
+    L.check(L.lib().kfc_residuals(H.handle, m, E.ctypes.data_as(L._D), Vb.ctypes.data_as(L._D),
+                                  out.ctypes.data_as(L._D)), H.ctx.handle)
+    return out

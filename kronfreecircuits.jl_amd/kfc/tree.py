@@ -1,0 +1,273 @@
+"""The caller side of the hot path: circuit terms, the real node operator and a
+tree materialization that hands each node's KronFreeOperator to a Solver.
+
+Follows src/terms.jl:119-168 (circuit_terms, constant_term),
+src/character.jl:276-315 (_real_expansion), src/materialize.jl:583-647
+(_node_operator, _real_node_terms), src/tree.jl:500-529 (route) and
+src/materialize.jl:427-459 (ports V' O V), with spec coefficients given as
+numbers (a spec evaluated at theta)."""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .bases import Basis, canonical_port, character, localop
+from .localops import DenseOp, DiagonalOp, dense, imagpart, realpart
+from .operator import AxisTerm, KronFreeOperator
+from .solvers import Auto, Dense, LOBPCG, resolve, solve_stats
+
+
+@dataclass
+class NumericSpec:
+    """The canonical form 4(n-n_g)^T E_C (n-n_g) + 1/2 phi^T E_L phi + flux_linear . phi
+    - sum E cos(omega . phi + offset) + constant (src/spec.jl:33-52) at one theta."""
+    modes: list
+    E_C: np.ndarray
+    E_L: np.ndarray
+    flux_linear: list
+    n_g: list
+    cosines: list
+    constant: float = 0.0
+
+
+def circuit_terms(spec):
+    """``circuit_terms(spec)`` (src/terms.jl:119-157): [(coeff, [(mode, port)])]."""
+    K = len(spec.modes)
+    E_C, E_L, names = np.asarray(spec.E_C, float), np.asarray(spec.E_L, float), spec.modes
+    out = []
+
+    def add(c, fs):
+        if c != 0:
+            out.append((c, [(names[m], p) for m, p in fs]))
+    for i in range(K):
+        add(4 * E_C[i, i], [(i, ("n2", 0.0))])
+    for i in range(K):
+        for j in range(i + 1, K):
+            add(4 * (E_C[i, j] + E_C[j, i]), [(i, ("n", 0.0)), (j, ("n", 0.0))])
+    for i in range(K):
+        add(-4 * sum((E_C[i, j] + E_C[j, i]) * spec.n_g[j] for j in range(K)), [(i, ("n", 0.0))])
+    for i in range(K):
+        add(0.5 * E_L[i, i], [(i, ("phi2", 0.0))])
+    for i in range(K):
+        for j in range(i + 1, K):
+            add(0.5 * (E_L[i, j] + E_L[j, i]), [(i, ("phi", 0.0)), (j, ("phi", 0.0))])
+    for i in range(K):
+        add(spec.flux_linear[i], [(i, ("phi", 0.0))])
+    for energy, omega, offset in spec.cosines:
+        for s in (1, -1):
+            add(-0.5 * energy * np.exp(1j * s * offset),
+                [(j, ("expiphi", float(s * omega[j]))) for j in range(K) if omega[j] != 0])
+    return out
+
+
+def constant_term(spec):
+    """``constant_term`` (src/terms.jl:164-168): constant + 4 n_g^T E_C n_g."""
+    ng = np.asarray(spec.n_g, float)
+    return spec.constant + 4 * float(ng @ np.asarray(spec.E_C, float) @ ng)
+
+
+def _re_ipow(c, m):
+    c = complex(c)
+    return [c.real, -c.imag, -c.real, c.imag][m % 4]
+
+
+class Node:
+    """``Node(children; solver=Auto(), keep=lowest(k), compute, real)`` (src/tree.jl:419-456):
+    children are ``(mode, basis)`` pairs or Nodes."""
+
+    def __init__(self, children, keep, solver=None, compute=None, real=True, ports=()):
+        self.children = list(children)
+        self.keep = int(keep)
+        self.solver = solver or Auto()
+        self.compute = int(compute) if compute is not None else int(keep)
+        self.real = real
+        self.ports = list(ports)
+
+    def modes(self):
+        out = []
+        for c in self.children:
+            out += [c[0]] if isinstance(c, tuple) else c.modes()
+        return out
+
+    def child_of(self, mode):
+        for a, c in enumerate(self.children):
+            if (isinstance(c, tuple) and c[0] == mode) or (isinstance(c, Node) and mode in c.modes()):
+                return a
+        raise KeyError(mode)
+
+    def leaf_basis(self, mode):
+        c = self.children[self.child_of(mode)]
+        return c[1] if isinstance(c, tuple) else c.leaf_basis(mode)
+
+
+@dataclass
+class MaterializedNode:
+    node: Node
+    dims: list
+    energies: np.ndarray
+    vectors: np.ndarray
+    children: list
+    operator: KronFreeOperator = None
+    solver: object = None
+    stats: dict = field(default_factory=dict)
+    seconds: float = 0.0
+
+
+def _axis_apply(op, V, dims, a):
+    """op applied along axis a of the column-major tensors V[:, j] (host)."""
+    T = V.reshape(list(dims) + [V.shape[1]], order="F")
+    T = np.moveaxis(np.tensordot(dense(op), np.moveaxis(T, a, 0), axes=(1, 0)), 0, a)
+    return T.reshape(V.shape, order="F")
+
+
+def port_matrix(mn, mode, port):
+    """``_port_matrix`` (src/materialize.jl:443-459): V' O V with O the mode's port
+    on its axis (a basis child's localop or a node child's port)."""
+    a = mn.node.child_of(mode)
+    ch = mn.node.children[a]
+    op = localop(ch[1], port) if isinstance(ch, tuple) else DenseOp(port_matrix(mn.children[a], mode, port))
+    V = mn.vectors
+    return V.conj().T @ _axis_apply(op, V, mn.dims, a)
+
+
+def joint_port(mn, factors):
+    """``_joint_port`` (src/materialize.jl:490-518): V' (x_a O_a) V."""
+    V = mn.vectors
+    W = V
+    for mode, port in factors:
+        a = mn.node.child_of(mode)
+        ch = mn.node.children[a]
+        op = localop(ch[1], port) if isinstance(ch, tuple) else DenseOp(port_matrix(mn.children[a], mode, port))
+        W = _axis_apply(op, W, mn.dims, a)
+    return V.conj().T @ W
+
+
+def node_axisterms(node, kids, terms, real):
+    """The AxisTerms of a node (``_node_operator`` / ``_real_node_terms``,
+    src/materialize.jl:583-647): on a real node every routed term's
+    Re(c (x) O) expansion with the real terms on equal (axis, canonical
+    sub-key, component) keys summed; each node child's kept energies as a
+    DiagonalOp on its axis."""
+    out = []
+    index, sums, firsts = {}, [], []
+    for c, fs in terms:
+        groups = {}
+        for mode, port in fs:
+            groups.setdefault(node.child_of(mode), []).append((mode, port))
+        ops = []
+        for a in sorted(groups):
+            g = groups[a]
+            ch = node.children[a]
+            if len(g) == 1:
+                mode, port = g[0]
+                op = localop(ch[1], port) if isinstance(ch, tuple) else DenseOp(port_matrix(kids[a], mode, port))
+                kind = character(node.leaf_basis(mode), port)
+                cport, sgn = canonical_port(port, kind)
+                ops.append((a, op, kind, sgn, (("single", mode, cport),)))
+            else:
+                op = DenseOp(joint_port(kids[a], g))
+                kinds = [character(node.leaf_basis(mm), pp) for mm, pp in g]
+                kind = "paired" if "paired" in kinds else ("imaginary" if kinds.count("imaginary") % 2 else "real")
+                ops.append((a, op, kind, 1, (("joint",) + tuple(sorted((mm, pp) for mm, pp in g)),)))
+        if not real:
+            out.append(AxisTerm(complex(c), [(a, op) for a, op, _, _, _ in ops]))
+            continue
+        comps = []
+        for a, op, kind, sgn, key in ops:
+            R = None if kind == "imaginary" else realpart(op)
+            I = None if kind == "real" else imagpart(op)
+            ch = (False,) if kind == "real" else ((True,) if kind == "imaginary" else (False, True))
+            comps.append((a, sgn, R, I, ch, key))
+        for S in itertools.product(*[x[4] for x in comps]):
+            coeff = _re_ipow(c, sum(S))
+            if coeff == 0:
+                continue
+            sign = 1
+            for x, inS in zip(comps, S):
+                if inS:
+                    sign *= x[1]
+            key = tuple((x[0], x[5], inS) for x, inS in zip(comps, S))
+            facs = [(x[0], x[3] if inS else x[2]) for x, inS in zip(comps, S)]
+            if key in index:
+                sums[index[key]] += coeff * sign
+            else:
+                index[key] = len(sums)
+                sums.append(coeff * sign)
+                firsts.append((sign, facs))
+    for s, (sign, facs) in zip(sums, firsts):
+        if s != 0:
+            out.append(AxisTerm(s * sign, facs))
+    for a, ch in enumerate(node.children):
+        if isinstance(ch, Node):
+            out.append(AxisTerm(1.0, [(a, DiagonalOp(np.asarray(kids[a].energies, float)))]))
+    return out
+
+
+def route(root, terms):
+    """``route`` (src/tree.jl:500-529): each term to the lowest node containing its modes."""
+    routed = {}
+
+    def lowest(nd, modes):
+        for ch in nd.children:
+            if isinstance(ch, Node) and all(m in ch.modes() for m in modes):
+                return lowest(ch, modes)
+        return nd
+    for t in terms:
+        routed.setdefault(id(lowest(root, [f[0] for f in t[1]])), []).append(t)
+    return routed
+
+
+class System:
+    """A materialized tree: ``energies(sys)``, ``port(sys, mode, port)``."""
+
+    def __init__(self, root):
+        self.root = root
+
+    def energies(self):
+        return self.root.energies
+
+    def port(self, mode, port):
+        return port_matrix(self.root, mode, port)
+
+
+def materialize(root, spec, previous=None, X0=None):
+    """``materialize(root, spec, theta; previous)`` (src/materialize.jl:177-192, 713-731):
+    children first, then each node's operator and solve on the device.
+    ``X0`` (a dict node -> block) supplies LOBPCG warm starts directly."""
+    import time
+    routed = route(root, circuit_terms(spec))
+    const = constant_term(spec)
+    prev_map = {}
+    if previous is not None:
+        def walk(a, b):
+            prev_map[id(a)] = b
+            for ca, cb in zip(a.children, b.children):
+                if isinstance(ca, Node):
+                    walk(ca, cb)
+        walk(root, previous.root)
+
+    def visit(nd, is_root):
+        kids = [None if isinstance(ch, tuple) else visit(ch, False) for ch in nd.children]
+        dims = [ch[1].dim if isinstance(ch, tuple) else len(k.energies) for ch, k in zip(nd.children, kids)]
+        t0 = time.perf_counter()
+        m = min(nd.compute, int(np.prod(dims)))
+        s = resolve(nd.solver, {"isleaf": all(isinstance(c, tuple) for c in nd.children), "dims": dims, "m": m})
+        ts = node_axisterms(nd, kids, routed.get(id(nd), []), nd.real)
+        cap = m + s.guard if isinstance(s, LOBPCG) else 1
+        H = KronFreeOperator(dims, ts, constant=const if is_root else 0.0, block_capacity=max(1, cap))
+        x0 = None
+        if isinstance(s, LOBPCG):
+            if X0 is not None and id(nd) in X0:
+                x0 = X0[id(nd)]
+            elif id(nd) in prev_map and list(prev_map[id(nd)].dims) == dims:
+                pv = prev_map[id(nd)].vectors           # _initial_block (src/materialize.jl:656-666)
+                k = min(pv.shape[1], m)
+                x0 = np.concatenate([pv[:, :k], np.random.default_rng(0).standard_normal((pv.shape[0], m - k))], 1)
+        E, V, st = solve_stats(s, H, m if not isinstance(s, Dense) else m, x0)
+        keep = min(nd.keep, len(E))
+        return MaterializedNode(nd, dims, np.asarray(E)[:keep], np.asarray(V)[:, :keep], kids, H, s, st,
+                                time.perf_counter() - t0)
+    return System(visit(root, True))

@@ -1,0 +1,122 @@
+"""Parity of the HIP apply (kfc_apply through the C ABI) with the oracle's
+per-term chain (src/apply.jl:370-432) on the same operators and inputs."""
+import numpy as np
+import pytest
+
+from oracle import kfc_oracle as O
+from tests.helpers import random_ops, to_kfc, to_oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL64 = 1e-13      # FP64 apply: relative to max |y|
+TOL32 = 2e-6       # FP32 apply: relative to max |y|
+
+
+def _run(H, X, precision="f64"):
+    dt = {("f64", False): torch.float64, ("f64", True): torch.complex128,
+          ("f32", False): torch.float32, ("f32", True): torch.complex64}[(precision, np.iscomplexobj(X) or H.dtype == np.complex128)]
+    Xd = torch.tensor(np.ascontiguousarray(X.T), dtype=dt, device="cuda:0")
+    Yd = torch.empty_like(Xd)
+    H.mul(Yd, Xd, precision)
+    H.sync()
+    return Yd.cpu().numpy().T
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("dims", [[3, 4, 2], [5, 6, 4], [7], [9, 5], [4, 3, 2, 3]])
+def test_chain_path_random(dims, cplx):
+    rng = np.random.default_rng(hash((tuple(dims), cplx)) % 2**32)
+    terms = random_ops(rng, dims, 6, cplx)
+    H = to_kfc(dims, terms, constant=0.37)
+    Ho = to_oracle(dims, terms, constant=0.37)
+    X = rng.standard_normal((H.N, 5)) + (1j * rng.standard_normal((H.N, 5)) if cplx else 0)
+    assert _rel(_run(H, X), O.apply_chain(Ho, X)) < TOL64
+    assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("dims", [[70, 9, 11], [64, 4, 3], [130, 7], [200], [33, 17, 5]])
+def test_fused_path_random(dims, cplx):
+    """Banded/shift/diagonal terms with offsets in [-1, 1] run as one fused pass."""
+    rng = np.random.default_rng(hash((tuple(dims), cplx, 1)) % 2**32)
+    terms = random_ops(rng, dims, 7, cplx, kinds=("banded", "diag", "shift"), maxoff=1)
+    # varying weights on at most one axis per position: give every band constant weights
+    terms2 = []
+    for c, fs in terms:
+        fs2 = []
+        for a, kind, data in fs:
+            if kind == "banded":
+                data = (data[0], [np.full(dims[a], w[0]) for w in data[1]])
+            fs2.append((a, kind, data))
+        terms2.append((c, fs2))
+    H = to_kfc(dims, terms2, constant=-0.5)
+    assert H.fused
+    Ho = to_oracle(dims, terms2, constant=-0.5)
+    X = rng.standard_normal((H.N, 3)) + (1j * rng.standard_normal((H.N, 3)) if cplx else 0)
+    assert _rel(_run(H, X), O.apply_chain(Ho, X)) < TOL64
+    assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
+
+
+@pytest.mark.parametrize("N", [16, 33, 64, 100])
+def test_three_fluxonia_fused_parity(N):
+    from kfc import configs
+    th = configs.three_fluxonia_theta(11)
+    H = configs.three_fluxonia_operator(th, N=N)
+    assert H.fused
+    Ho = O.three_fluxonia_operator(th["E_C"], th["E_L"], th["E_J"], th["phi_ext"], N=N, order=2)
+    rng = np.random.default_rng(N)
+    X = rng.standard_normal((H.N, 2))
+    Yo = O.apply_chain(Ho, X)
+    assert _rel(_run(H, X), Yo) < TOL64
+    assert _rel(_run(H, X, "f32"), Yo) < TOL32 * 10
+
+
+def test_leading_dimension_and_views():
+    from kfc import configs
+    th = configs.three_fluxonia_theta(3)
+    H = configs.three_fluxonia_operator(th, N=24)
+    Ho = O.three_fluxonia_operator(th["E_C"], th["E_L"], th["E_J"], th["phi_ext"], N=24, order=2)
+    rng = np.random.default_rng(1)
+    big = torch.tensor(rng.standard_normal((4, H.N + 37)), device="cuda:0")
+    X = big[:, 5:5 + H.N]
+    Y = torch.zeros_like(big)[:, 3:3 + H.N]
+    H.mul(Y, X)
+    H.sync()
+    Yo = O.apply_chain(Ho, X.cpu().numpy().T).T
+    assert _rel(Y.cpu().numpy(), Yo) < TOL64
+
+
+def test_alias_rejected():
+    import kfc
+    from kfc import configs
+    H = configs.three_fluxonia_operator(configs.three_fluxonia_theta(3), N=12)
+    X = torch.zeros((2, H.N), dtype=torch.float64, device="cuda:0")
+    with pytest.raises(kfc.ArgumentError):
+        H.mul(X, X)
+
+
+def test_matrix_equals_kron_reference():
+    rng = np.random.default_rng(5)
+    dims = [4, 3, 5]
+    terms = random_ops(rng, dims, 5, True)
+    H = to_kfc(dims, terms, 1.5)
+    Ho = to_oracle(dims, terms, 1.5)
+    M = _run(H, np.eye(H.N, dtype=complex))
+    assert np.allclose(M, O.kron_reference(dims, Ho.terms + Ho.diagterms, 0.0), atol=1e-12)
+
+
+def test_derived_quantities():
+    from kfc import configs
+    th = configs.three_fluxonia_theta(3)
+    H = configs.three_fluxonia_operator(th, N=10)
+    Ho = O.three_fluxonia_operator(th["E_C"], th["E_L"], th["E_J"], th["phi_ext"], N=10, order=2)
+    assert abs(H.gershgorin_bound() - O.gershgorin_bound(Ho)) < 1e-9 * abs(O.gershgorin_bound(Ho))
+    assert np.allclose(H.diagonal(), O.full_diagonal(Ho), atol=1e-10)
+    P, Po = H.product_seed(6), O.product_seed(Ho, 6)
+    # same product states up to each column's sign
+    assert np.allclose(np.abs(np.sum(P * Po, axis=0)), 1.0, atol=1e-10)

@@ -1,0 +1,113 @@
+"""Device solvers against the oracle's Dense (numpy eigh) and the reference's
+golden energies (test/fixtures/scqubits_reference.json)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import kfc_oracle as O
+from tests.helpers import oracle_from_kfc_terms
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _ref(golden_dir, name):
+    with open(os.path.join(golden_dir, "scqubits_reference.json")) as f:
+        e = json.load(f)[name]
+    return np.array(e["energies"]), e["params"]
+
+
+def test_dense_two_transmons_vs_scqubits(golden_dir):
+    """test/test_materialize.jl:142-160 through the device Dense solver."""
+    import kfc
+    from kfc import configs
+    E, p = _ref(golden_dir, "two_transmons")
+    sys_ = kfc.materialize(configs.two_transmons_tree(), configs.two_transmons_spec(**configs.TWO_TRANSMONS_THETA))
+    En = sys_.energies()
+    assert np.max(np.abs(En[:3] - E[:3])) < 1e-8
+    assert np.max(np.abs(En - E)) < 5e-4
+
+
+def test_transmon_leaf_vs_scqubits(golden_dir):
+    import kfc
+    E, p = _ref(golden_dir, "transmon")
+    spec = kfc.NumericSpec(["varphi_1"], [[p["E_C"]]], [[0.0]], [0.0], [p["n_g"]], [(p["E_J"], [1], 0.0)])
+    s = kfc.materialize(kfc.Node([("varphi_1", kfc.ChargeBasis(30))], keep=5), spec)
+    assert np.max(np.abs(s.energies() - E)) < 1e-10
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+def test_lanczos_small_fluxonia_vs_dense(mode):
+    import kfc
+    from kfc import configs
+    th = configs.three_fluxonia_theta(2)
+    H = configs.three_fluxonia_operator(th, N=14)
+    Ho = oracle_from_kfc_terms(H)
+    w = np.linalg.eigvalsh(O.to_matrix(Ho))
+    E, V, st = kfc.solve_stats(kfc.Lanczos(tol=1e-10, mode=mode), H, 10)
+    assert np.max(np.abs(E - w[:10])) < 1e-9 * max(1.0, np.max(np.abs(w[:10])))
+    r = O.true_residuals(Ho, E, V)
+    assert np.max(r) < 1e-8
+    assert np.allclose(V.T @ V, np.eye(10), atol=1e-10)
+    assert st["max_residual"] < 1e-8
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+@pytest.mark.parametrize("precond", ["ritz", "jacobi", "none"])
+def test_lobpcg_zero_pi_tree_small(mode, precond):
+    """A small zero-pi tree: leaves Dense, parent LOBPCG vs the parent's dense spectrum."""
+    import kfc
+    from kfc import configs
+    th = configs.zero_pi_theta(1)
+    spec, E_Cz = configs.zero_pi_spec(th["E_CS"], th["n_g"])
+    root = configs.zero_pi_tree(E_Cz, ncut=12, Nphi=64, Nzeta=10, keep=10, m=6,
+                                solver=kfc.LOBPCG(tol=1e-8, preconditioner=precond, mode=mode))
+    s = kfc.materialize(root, spec)
+    H = s.root.operator
+    Ho = oracle_from_kfc_terms(H)
+    w = np.linalg.eigvalsh(O.to_matrix(Ho))
+    assert np.max(np.abs(s.energies() - w[:6])) < 1e-9 * max(1.0, np.max(np.abs(w[:6])))
+    r = O.true_residuals(Ho, s.energies(), s.root.vectors)
+    assert np.max(r) < 1e-8
+
+
+def test_lobpcg_warm_start_and_errors():
+    import kfc
+    from kfc import configs
+    th = configs.three_fluxonia_theta(5)
+    H = configs.three_fluxonia_operator(th, N=10)
+    E, V = kfc.solve(kfc.LOBPCG(tol=1e-8), H, 4)
+    E2, V2, st = kfc.solve_stats(kfc.LOBPCG(tol=1e-8), H, 4, X0=V)
+    assert st["warm"] == 1 and np.allclose(E, E2, atol=1e-8)
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve(kfc.LOBPCG(), H, 400)                    # N < 3m + 5
+    bad = V.copy()
+    bad[:, 1] = 0
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve(kfc.LOBPCG(), H, 4, X0=bad)              # zero column
+    bad[0, 0] = np.nan
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve(kfc.LOBPCG(), H, 4, X0=bad)              # non-finite
+    with pytest.raises(kfc.ConvergenceError):
+        kfc.solve(kfc.LOBPCG(maxiter=1, tol=1e-12), H, 4)  # nonconvergence
+
+
+def test_dense_complex_periodic_vs_closed_form():
+    """Free rotor on a PeriodicGrid (complex node) through the device Dense solver."""
+    import math
+    import kfc
+    N, EC, ng, order = 32, 1.1, 0.27, 8
+    b = kfc.PeriodicGrid(N, order)
+    H = kfc.KronFreeOperator([N], [kfc.AxisTerm(4 * EC, [(0, kfc.localop(b, ("n2", 0)))]),
+                                   kfc.AxisTerm(-8 * EC * ng, [(0, kfc.localop(b, ("n", 0)))])],
+                             constant=4 * EC * ng ** 2)
+    E, V = kfc.solve(kfc.Dense(), H, 0)
+    h = order // 2
+    w1, w2 = kfc.fd_weights(order, 1), kfc.fd_weights(order, 2)
+    D = 2 * math.pi / N
+    ks = np.arange(-N // 2, N // 2)
+    s1 = np.array([sum(w1[j + h] * math.sin(j * k * D) for j in range(-h, h + 1)) / D for k in ks])
+    s2 = np.array([-sum(w2[j + h] * math.cos(j * k * D) for j in range(-h, h + 1)) / D ** 2 for k in ks])
+    assert np.allclose(E, np.sort(4 * EC * (s2 - 2 * ng * s1 + ng ** 2)), atol=1e-9)
