@@ -3,10 +3,6 @@
 #include "kfc_internal.h"
 
 extern "C" {
-int kfc_chebyshev(kfc_op* op, int64_t, int32_t, int64_t, int64_t, int32_t, int32_t, double, int32_t, double, uint64_t,
-                  int32_t, double*, double*, kfc_stats*) {
-    return kfc_fail(op ? op->ctx : nullptr, KFC_EUNSUPPORTED, "kfc_chebyshev is not built yet");
-}
 int kfc_batch_transmons(kfc_ctx* ctx, int64_t, const int64_t*, const int64_t*, const double*, int32_t, int32_t, double,
                         int32_t, double*, double*, kfc_stats*) {
     return kfc_fail(ctx, KFC_EUNSUPPORTED, "kfc_batch_transmons is not built yet");

@@ -111,3 +111,34 @@ def test_dense_complex_periodic_vs_closed_form():
     s1 = np.array([sum(w1[j + h] * math.sin(j * k * D) for j in range(-h, h + 1)) / D for k in ks])
     s2 = np.array([-sum(w2[j + h] * math.cos(j * k * D) for j in range(-h, h + 1)) / D ** 2 for k in ks])
     assert np.allclose(E, np.sort(4 * EC * (s2 - 2 * ng * s1 + ng ** 2)), atol=1e-9)
+
+
+@pytest.mark.parametrize("ortho", ["svqb", "cholqr"])
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+def test_chebyshev_fluxonium_leaf_vs_dense(ortho, mode):
+    """ChebyshevFilter on a grid leaf (the device leaf solver, src/materialize.jl:557)."""
+    import kfc
+    b = kfc.PhaseGrid(-10 * np.pi, 10 * np.pi, 801, 8)
+    spec = kfc.NumericSpec(["varphi_1"], [[2.5]], [[0.5]], [0.5 * np.pi], [0.0], [(8.9, [1], 0.0)], 0.5 * np.pi ** 2 / 2)
+    node = kfc.Node([("varphi_1", b)], keep=6, solver=kfc.ChebyshevFilter(ortho=ortho, mode=mode, extra=20))
+    s = kfc.materialize(node, spec)
+    H = s.root.operator
+    Ho = oracle_from_kfc_terms(H)
+    w = np.linalg.eigvalsh(O.to_matrix(Ho))[:6]
+    assert np.max(np.abs(s.energies() - w)) < 1e-9 * max(1, np.max(np.abs(w)))
+    assert s.root.stats["max_residual"] <= max(1e-8, 10 * 2.2e-16 * H.gershgorin_bound())
+
+
+def test_chebyshev_complex_periodic_transmon():
+    import kfc
+    N, EJ, EC, ng = 128, 30.0, 1.2, 0.25
+    b = kfc.PeriodicGrid(N, 8)
+    H = kfc.KronFreeOperator([N], [kfc.AxisTerm(4 * EC, [(0, kfc.localop(b, ("n2", 0)))]),
+                                   kfc.AxisTerm(-8 * EC * ng, [(0, kfc.localop(b, ("n", 0)))]),
+                                   kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", 1)))]),
+                                   kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", -1)))])],
+                             constant=4 * EC * ng ** 2)
+    assert H.dtype == np.complex128
+    E, V = kfc.solve(kfc.ChebyshevFilter(extra=10), H, 5)
+    Ed, _ = kfc.solve(kfc.Dense(), H, 5)
+    assert np.allclose(E, Ed, atol=1e-9)
