@@ -293,6 +293,12 @@ static int build_device(kfc_op* op, DeviceData& dd) {
             dd.dp2[k] = upload<S>(op, dd, P.dprod[k].a2, &rc);
         }
     }
+    if (op->pairs.ok) {
+        if constexpr (!IsCx<S>::value) {
+            rc = pairs_upload<S>(op, dd);
+            if (rc != KFC_OK) return rc;
+        }
+    }
     if (!P.ok || P.need_full_diag) {
         op_host_diag(op);
         std::vector<S> h(op->N);
@@ -301,7 +307,7 @@ static int build_device(kfc_op* op, DeviceData& dd) {
         if (e == hipSuccess) e = hipMemcpy(dd.diag.p, h.data(), h.size() * sizeof(S), hipMemcpyHostToDevice);
         if (e != hipSuccess) return kfc_fail(op->ctx, KFC_ENOMEM, std::string("diagonal upload: ") + hipGetErrorString(e));
     }
-    if (!P.ok) {
+    if (!P.ok && !op->pairs.ok) {
         dd.fdev.assign(op->terms.size(), {});
         for (size_t t = 0; t < op->terms.size(); ++t) {
             for (const Factor& f : op->terms[t].factors) {
@@ -650,7 +656,13 @@ template <class S>
 static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
                    cd gamma, hipStream_t st) {
     if (op->plan.ok) return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
-    int rc = launch_chain<S>(op, X, Y, ncols, ld, st);
+    int rc;
+    if constexpr (!IsCx<S>::value) {
+        if (op->pairs.ok) rc = launch_pairs<S>(op, X, Y, ncols, ld, st);
+        else rc = launch_chain<S>(op, X, Y, ncols, ld, st);
+    } else {
+        rc = launch_chain<S>(op, X, Y, ncols, ld, st);
+    }
     if (rc != KFC_OK) return rc;
     if (alpha != cd(1) || beta != cd(0) || A) {
         const int64_t total = op->N * ncols;

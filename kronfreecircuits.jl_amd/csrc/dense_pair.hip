@@ -1,0 +1,283 @@
+// Dense kept-basis terms on gfx950 MFMA.
+//
+// A branch node's kept terms are products of two dense port matrices on two
+// axes (p, q): H x = D .* x + sum_t c_t (A_t (x) B_t) x.  Seen as slabs
+// X_s = x[l, :, m, :, r] (d_p x d_q), each group of terms on one axis pair is
+//   Y_s = [D .* X_s] + sum_t c_t A_t (X_s B_t^T),
+// two small GEMMs per term per slab, computed here with
+// v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32) tiles, the
+// slab and the intermediate staged in LDS, the port matrices read through L1
+// (they are shared by every slab).  No permutes: the reference moves a middle
+// axis to the front and back around its gemm (src/apply.jl:322-337).
+// Terms sharing one factor are merged on the host first
+// (c1 A(x)B1 + c2 A(x)B2 = A (x) (c1 B1 + c2 B2)).
+#include <algorithm>
+
+#include "kfc_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+template <class T> struct V4;
+template <> struct V4<double> { using type = d4; };
+template <> struct V4<float> { using type = f4; };
+
+constexpr int MAXT = 8;   // terms per group
+
+template <class T>
+struct PairArgs {
+    const T* X;
+    T* Y;
+    const T* D;           // full diagonal or null (accumulate into Y)
+    int64_t ld;
+    int dp, dq, Mp, Nq;   // sizes and their multiples of 16
+    int64_t L, M, R;      // slab decomposition
+    int64_t sp, sq, sr;   // strides of axis p, axis q, and the slab r index
+    int64_t nslab;        // L*M*R*ncols
+    int nterm;
+    T c[MAXT];
+    const T* A[MAXT];     // dp x dp column-major
+    const T* B[MAXT];     // dq x dq column-major
+};
+
+// P: padded extent (multiple of 16) bounding dp and dq.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
+    using v4 = typename V4<T>::type;
+    constexpr int NS = ((P / 16) * (P / 16) + 3) / 4;   // output tiles per wave
+    __shared__ T Xs[P * P];   // column-major, leading dimension P
+    __shared__ T Ts[P * P];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tm = a.Mp / 16, tn = a.Nq / 16, ntile = tm * tn;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int64_t perc = a.L * a.M * a.R;
+    for (int64_t s = blockIdx.x; s < a.nslab; s += gridDim.x) {
+        const int64_t col = s / perc;
+        int64_t rem = s - col * perc;
+        const int64_t l = rem % a.L;
+        rem /= a.L;
+        const int64_t m = rem % a.M;
+        const int64_t r = rem / a.M;
+        const int64_t base = col * a.ld + l + m * a.L * a.dp + r * a.sr;
+        __syncthreads();   // the previous slab's readers of Xs and Ts are done
+        for (int e = tid; e < P * P; e += 256) {
+            const int i = e % P, j = e / P;
+            Xs[e] = (i < a.dp && j < a.dq) ? a.X[base + i * a.sp + j * a.sq] : T(0);
+        }
+        v4 acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = v4{0, 0, 0, 0};
+        for (int t = 0; t < a.nterm; ++t) {
+            const T* __restrict__ At = a.A[t];
+            const T* __restrict__ Bt = a.B[t];
+            __syncthreads();   // Xs written (t == 0) / Ts free (t > 0)
+            // Ts = Xs B^T : (dp x dq)(dq x dq)
+            for (int tile = wave; tile < ntile; tile += 4) {
+                const int ti = tile % tm, tj = tile / tm;
+                v4 c = v4{0, 0, 0, 0};
+                const int row = ti * 16 + r16, bc = tj * 16 + r16;
+                for (int kk = 0; kk < a.Nq; kk += 4) {
+                    const int k = kk + k4;
+                    const T av = Xs[row + k * P];
+                    const T bv = (k < a.dq && bc < a.dq) ? Bt[bc + (int64_t)k * a.dq] : T(0);
+                    c = mfma(av, bv, c);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) Ts[(ti * 16 + k4 * 4 + i) + (tj * 16 + r16) * P] = c[i];
+            }
+            __syncthreads();
+            // acc += c_t A Ts : (dp x dp)(dp x dq)
+#pragma unroll
+            for (int slot = 0; slot < NS; ++slot) {
+                const int tile = wave + 4 * slot;
+                if (tile >= ntile) break;
+                const int ti = tile % tm, tj = tile / tm;
+                v4 c = v4{0, 0, 0, 0};
+                const int row = ti * 16 + r16, cc = tj * 16 + r16;
+                for (int kk = 0; kk < a.Mp; kk += 4) {
+                    const int k = kk + k4;
+                    const T av = (row < a.dp && k < a.dp) ? At[row + (int64_t)k * a.dp] : T(0);
+                    const T bv = Ts[k + cc * P];
+                    c = mfma(av, bv, c);
+                }
+                acc[slot] += a.c[t] * c;
+            }
+        }
+        // epilogue: Y_s = D .* X_s + acc   (or Y_s += acc)
+#pragma unroll
+        for (int slot = 0; slot < NS; ++slot) {
+            const int tile = wave + 4 * slot;
+            if (tile >= ntile) break;
+            const int ti = tile % tm, tj = tile / tm;
+            const int cc = tj * 16 + r16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = ti * 16 + k4 * 4 + i;
+                if (row < a.dp && cc < a.dq) {
+                    const int64_t idx = base + row * a.sp + cc * a.sq;
+                    const int64_t didx = idx - col * a.ld;
+                    const T y0 = a.D ? a.D[didx] * Xs[row + cc * P] : a.Y[idx];
+                    a.Y[idx] = y0 + acc[slot][i];
+                }
+            }
+        }
+    }
+}
+
+bool same(const std::vector<cd>& x, const std::vector<cd>& y) { return x == y; }
+
+}  // namespace
+
+// Host plan: every kept term is a product of exactly two factors on axes with
+// d <= 64 on a real operator; factors of any kind are densified.
+void dense_of_factor(const Factor& f, std::vector<cd>& M) {
+    const int64_t d = f.d;
+    M.assign(size_t(d) * d, cd(0));
+    if (f.kind == KFC_DENSE) M = f.matrix;
+    else if (f.kind == KFC_DIAG)
+        for (int64_t i = 0; i < d; ++i) M[i + i * d] = f.diag[i];
+    else if (f.kind == KFC_BANDED) {
+        for (size_t b = 0; b < f.offsets.size(); ++b)
+            for (int64_t i = 0; i < d; ++i) {
+                const int64_t j = i + f.offsets[b];
+                if (j >= 0 && j < d) M[i + j * d] += f.weights[i + b * d];
+            }
+    } else {
+        for (int64_t i = 0; i < d; ++i) {
+            const int64_t j = i + f.shift;
+            if (j >= 0 && j < d) M[j + i * d] = f.factor;
+        }
+    }
+}
+
+int op_build_pairs(kfc_op* op) {
+    PairPlan& P = op->pairs;
+    P = PairPlan();
+    if (op->cplx || op->plan.ok || op->terms.empty()) return KFC_OK;
+    for (const Term& t : op->terms) {
+        if (t.factors.size() != 2) return KFC_OK;
+        for (const Factor& f : t.factors)
+            if (f.d > 64) return KFC_OK;
+    }
+    for (const Term& t : op->terms) {
+        const int p = t.factors[0].axis, q = t.factors[1].axis;
+        PairGroup* g = nullptr;
+        for (PairGroup& x : P.groups)
+            if (x.p == p && x.q == q) g = &x;
+        if (!g) {
+            P.groups.push_back(PairGroup{});
+            g = &P.groups.back();
+            g->p = p;
+            g->q = q;
+            g->dp = t.factors[0].d;
+            g->dq = t.factors[1].d;
+        }
+        std::vector<cd> A, B;
+        dense_of_factor(t.factors[0], A);
+        dense_of_factor(t.factors[1], B);
+        bool merged = false;
+        for (size_t k = 0; k < g->A.size() && !merged; ++k) {
+            if (same(g->A[k], A)) {   // c1 A(x)B1 + c A(x)B = A (x) (c1 B1 + c B)
+                for (size_t i = 0; i < B.size(); ++i) g->B[k][i] = g->c[k] * g->B[k][i] + t.coeff * B[i];
+                g->c[k] = 1.0;
+                merged = true;
+            } else if (same(g->B[k], B)) {
+                for (size_t i = 0; i < A.size(); ++i) g->A[k][i] = g->c[k] * g->A[k][i] + t.coeff * A[i];
+                g->c[k] = 1.0;
+                merged = true;
+            }
+        }
+        if (!merged) {
+            g->A.push_back(std::move(A));
+            g->B.push_back(std::move(B));
+            g->c.push_back(t.coeff);
+        }
+    }
+    for (const PairGroup& g : P.groups)
+        if (g.A.size() > size_t(MAXT)) return KFC_OK;
+    P.ok = true;
+    return KFC_OK;
+}
+
+template <class T>
+static const T* up(kfc_op* op, DeviceData& dd, const std::vector<cd>& v) {
+    std::vector<T> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = T(v[i].real());
+    auto b = std::make_unique<DevBuf>();
+    if (b->ensure(h.size() * sizeof(T)) != hipSuccess) return nullptr;
+    if (hipMemcpy(b->p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    const T* p = b->as<T>();
+    dd.blobs.push_back(std::move(b));
+    return p;
+}
+
+template <class T>
+int pairs_upload(kfc_op* op, DeviceData& dd) {
+    dd.pair_ptr.clear();
+    for (const PairGroup& g : op->pairs.groups) {
+        std::vector<const void*> ptrs;
+        for (size_t k = 0; k < g.A.size(); ++k) {
+            const T* A = up<T>(op, dd, g.A[k]);
+            const T* B = up<T>(op, dd, g.B[k]);
+            if (!A || !B) return kfc_fail(op->ctx, KFC_ENOMEM, "dense pair upload failed");
+            ptrs.push_back(A);
+            ptrs.push_back(B);
+        }
+        dd.pair_ptr.push_back(ptrs);
+    }
+    return KFC_OK;
+}
+
+template int pairs_upload<double>(kfc_op*, DeviceData&);
+template int pairs_upload<float>(kfc_op*, DeviceData&);
+
+template <class T>
+int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st) {
+    const int prec = std::is_same<T, double>::value ? 0 : 1;
+    const DeviceData& dd = op->dd[prec];
+    std::vector<int64_t> stride(op->ndims + 1, 1);
+    for (int a = 1; a <= op->ndims; ++a) stride[a] = stride[a - 1] * op->dims[a - 1];
+    for (size_t gi = 0; gi < op->pairs.groups.size(); ++gi) {
+        const PairGroup& g = op->pairs.groups[gi];
+        PairArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        a.X = static_cast<const T*>(X);
+        a.Y = static_cast<T*>(Y);
+        a.D = gi == 0 ? dd.diag.as<T>() : nullptr;
+        a.ld = ld;
+        a.dp = int(g.dp);
+        a.dq = int(g.dq);
+        a.Mp = (a.dp + 15) / 16 * 16;
+        a.Nq = (a.dq + 15) / 16 * 16;
+        a.L = stride[g.p];
+        a.M = stride[g.q] / (stride[g.p] * g.dp);
+        a.R = op->N / stride[g.q + 1];
+        a.sp = stride[g.p];
+        a.sq = stride[g.q];
+        a.sr = stride[g.q + 1];
+        a.nslab = a.L * a.M * a.R * ncols;
+        a.nterm = int(g.A.size());
+        for (int t = 0; t < a.nterm; ++t) {
+            a.c[t] = T(g.c[t].real());
+            a.A[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t]);
+            a.B[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t + 1]);
+        }
+        const int Pm = std::max(a.Mp, a.Nq);
+        const unsigned grid = unsigned(std::min<int64_t>(a.nslab, 4096));
+        if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair<T, 16>), dim3(grid), dim3(256), 0, st, a);
+        else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair<T, 32>), dim3(grid), dim3(256), 0, st, a);
+        else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair<T, 48>), dim3(grid), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_dense_pair<T, 64>), dim3(grid), dim3(256), 0, st, a);
+        KFC_HIP(op->ctx, hipGetLastError());
+    }
+    if (op->pairs.groups.empty()) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "dense pair path without groups");
+    return KFC_OK;
+}
+
+template int launch_pairs<double>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t);
+template int launch_pairs<float>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t);

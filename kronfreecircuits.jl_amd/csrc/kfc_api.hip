@@ -149,6 +149,7 @@ int kfc_op_create(kfc_ctx* ctx, const kfc_op_desc* desc, kfc_op** out) {
         }
     }
     int rc = op_build_plan(op.get());
+    if (rc == KFC_OK) rc = op_build_pairs(op.get());
     if (rc != KFC_OK) return rc;
     *out = op.release();
     return KFC_OK;

@@ -155,6 +155,18 @@ struct StencilPlan {
     bool need_full_diag = false;
 };
 
+// Dense kept-basis terms grouped by axis pair: sum_t c_t A_t (x) B_t on axes (p, q).
+struct PairGroup {
+    int p = 0, q = 0;
+    int64_t dp = 0, dq = 0;
+    std::vector<cd> c;
+    std::vector<std::vector<cd>> A, B;
+};
+struct PairPlan {
+    bool ok = false;
+    std::vector<PairGroup> groups;
+};
+
 // Device data of one precision/scalar kind, built on first use.
 struct DeviceData {
     bool built = false;
@@ -169,6 +181,7 @@ struct DeviceData {
     const void* D0 = nullptr; const void* D1 = nullptr; const void* D2 = nullptr;
     const void* dp0[4] = {}; const void* dp1[4] = {}; const void* dp2[4] = {};
     DevBuf buf1, buf2;              // chain scratch
+    std::vector<std::vector<const void*>> pair_ptr;   // per pair group: A0, B0, A1, B1, ...
 };
 
 struct kfc_op {
@@ -184,6 +197,7 @@ struct kfc_op {
     bool hermitian = true;
     std::vector<cd> hdiag;          // host fused diagonal (built on demand)
     StencilPlan plan;
+    PairPlan pairs;
     DeviceData dd[2];               // [KFC_PREC_F64], [KFC_PREC_F32]
     double gershgorin = 0;
     bool have_gershgorin = false;
@@ -191,6 +205,9 @@ struct kfc_op {
 
 // Builders and launches (apply.hip)
 int op_build_plan(kfc_op* op);
+int op_build_pairs(kfc_op* op);
+template <class T> int pairs_upload(kfc_op* op, DeviceData& dd);
+template <class T> int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st);
 int op_host_diag(kfc_op* op);
 int op_ensure_device(kfc_op* op, int prec);
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st);

@@ -6,11 +6,12 @@ from .bases import ChargeBasis, FockBasis, PeriodicGrid, PhaseGrid, fd_weights, 
 from .localops import BandedOp, DenseOp, DiagonalOp, LocalOp, ShiftOp, dense, imagpart, realpart
 from .operator import AxisTerm, KronFreeOperator
 from .solvers import LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, residuals, resolve, solve, solve_stats
-from .tree import MaterializedNode, Node, NumericSpec, System, circuit_terms, constant_term, materialize, port_matrix
+from .tree import (MaterializedNode, Node, NumericSpec, System, circuit_terms, constant_term, materialize, operator,
+                   port_matrix)
 
 __all__ = ["ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
            "UnsupportedError", "context", "lib", "missing_symbols", "ChargeBasis", "FockBasis", "PeriodicGrid",
            "PhaseGrid", "fd_weights", "localop", "BandedOp", "DenseOp", "DiagonalOp", "LocalOp", "ShiftOp", "dense",
            "imagpart", "realpart", "AxisTerm", "KronFreeOperator", "LOBPCG", "Auto", "ChebyshevFilter", "Dense",
            "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_stats", "MaterializedNode", "Node",
-           "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "port_matrix"]
+           "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "operator", "port_matrix"]

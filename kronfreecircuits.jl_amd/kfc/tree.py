@@ -233,12 +233,28 @@ class System:
         return port_matrix(self.root, mode, port)
 
 
-def materialize(root, spec, previous=None, X0=None):
+def operator(root, spec):
+    """The root node's KronFreeOperator with its children materialized, not
+    solved (``operator(sys)``, src/materialize.jl:982-992)."""
+    routed = route(root, circuit_terms(spec))
+    kids = []
+    for ch in root.children:
+        if isinstance(ch, tuple):
+            kids.append(None)
+            continue
+        sub = materialize(ch, spec, _routed=routed, _is_root=False)
+        kids.append(sub.root)
+    dims = [ch[1].dim if isinstance(ch, tuple) else len(k.energies) for ch, k in zip(root.children, kids)]
+    ts = node_axisterms(root, kids, routed.get(id(root), []), root.real)
+    return KronFreeOperator(dims, ts, constant=constant_term(spec)), kids
+
+
+def materialize(root, spec, previous=None, X0=None, _routed=None, _is_root=True):
     """``materialize(root, spec, theta; previous)`` (src/materialize.jl:177-192, 713-731):
     children first, then each node's operator and solve on the device.
     ``X0`` (a dict node -> block) supplies LOBPCG warm starts directly."""
     import time
-    routed = route(root, circuit_terms(spec))
+    routed = _routed if _routed is not None else route(root, circuit_terms(spec))
     const = constant_term(spec)
     prev_map = {}
     if previous is not None:
@@ -270,4 +286,4 @@ def materialize(root, spec, previous=None, X0=None):
         keep = min(nd.keep, len(E))
         return MaterializedNode(nd, dims, np.asarray(E)[:keep], np.asarray(V)[:, :keep], kids, H, s, st,
                                 time.perf_counter() - t0)
-    return System(visit(root, True))
+    return System(visit(root, _is_root))

@@ -39,24 +39,33 @@ def test_chain_path_random(dims, cplx):
     assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
 
 
+def _fusable_terms(rng, dims, cplx):
+    """Kept terms whose stencil weights are rank-1 per position: constant band
+    weights everywhere, one band with per-row weights along axis 0, and
+    varying single-axis diagonal terms (fused into the diagonal)."""
+    r = (lambda: complex(*rng.standard_normal(2))) if cplx else (lambda: float(rng.standard_normal()))
+    K = len(dims)
+    terms = []
+    for a in range(K):
+        terms.append((r(), [(a, "diag", np.array([r() for _ in range(dims[a])]))]))
+        terms.append((r(), [(a, "banded", ([-1, 0, 1], [np.full(dims[a], r()) for _ in range(3)]))]))
+    for a in range(K):
+        for b in range(a + 1, K):
+            terms.append((r(), [(a, "banded", ([-1, 1], [np.full(dims[a], r()) for _ in range(2)])),
+                                (b, "shift", (int(rng.integers(-1, 2)), r()))]))
+    terms.append((r(), [(0, "banded", ([1], [np.array([r() for _ in range(dims[0])])]))]))
+    return terms
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("dims", [[70, 9, 11], [64, 4, 3], [130, 7], [200], [33, 17, 5]])
 def test_fused_path_random(dims, cplx):
     """Banded/shift/diagonal terms with offsets in [-1, 1] run as one fused pass."""
     rng = np.random.default_rng(hash((tuple(dims), cplx, 1)) % 2**32)
-    terms = random_ops(rng, dims, 7, cplx, kinds=("banded", "diag", "shift"), maxoff=1)
-    # varying weights on at most one axis per position: give every band constant weights
-    terms2 = []
-    for c, fs in terms:
-        fs2 = []
-        for a, kind, data in fs:
-            if kind == "banded":
-                data = (data[0], [np.full(dims[a], w[0]) for w in data[1]])
-            fs2.append((a, kind, data))
-        terms2.append((c, fs2))
-    H = to_kfc(dims, terms2, constant=-0.5)
+    terms = _fusable_terms(rng, dims, cplx)
+    H = to_kfc(dims, terms, constant=-0.5)
     assert H.fused
-    Ho = to_oracle(dims, terms2, constant=-0.5)
+    Ho = to_oracle(dims, terms, constant=-0.5)
     X = rng.standard_normal((H.N, 3)) + (1j * rng.standard_normal((H.N, 3)) if cplx else 0)
     assert _rel(_run(H, X), O.apply_chain(Ho, X)) < TOL64
     assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
@@ -120,3 +129,44 @@ def test_derived_quantities():
     P, Po = H.product_seed(6), O.product_seed(Ho, 6)
     # same product states up to each column's sign
     assert np.allclose(np.abs(np.sum(P * Po, axis=0)), 1.0, atol=1e-10)
+
+
+@pytest.mark.parametrize("dims", [[40, 40, 30], [7, 9, 5, 3], [16, 33, 2], [64, 3, 17]])
+def test_dense_pair_mfma_path(dims):
+    """Two-factor dense terms (a branch node's kept-basis ports) on MFMA tiles."""
+    import kfc
+    rng = np.random.default_rng(sum(dims))
+    K = len(dims)
+    terms = []
+    pairs = [(0, 1), (1, 2), (0, K - 1)]
+    for t in range(6):
+        p, q = pairs[t % len(pairs)]
+        A = rng.standard_normal((dims[p], dims[p]))
+        B = rng.standard_normal((dims[q], dims[q]))
+        if t >= 3:   # share a factor with an earlier term: merged on the host
+            A = terms[t - 3][1][0][2]
+        terms.append((float(rng.standard_normal()), [(p, "dense", A), (q, "dense", B)]))
+    terms.append((0.3, [(0, "diag", rng.standard_normal(dims[0]))]))
+    H = to_kfc(dims, terms, constant=0.25)
+    assert not H.fused
+    Ho = to_oracle(dims, terms, constant=0.25)
+    X = rng.standard_normal((H.N, 4))
+    Yo = O.apply_chain(Ho, X)
+    assert _rel(_run(H, X), Yo) < TOL64
+    assert _rel(_run(H, X, "f32"), Yo) < 20 * TOL32
+
+
+def test_zero_pi_parent_operator_parity():
+    """The zero-pi tree's parent [40, 40, 30] (dense ports) against the oracle chain."""
+    import kfc
+    from kfc import configs
+    from tests.helpers import oracle_from_kfc_terms
+    th = configs.zero_pi_theta(3)
+    spec, E_Cz = configs.zero_pi_spec(th["E_CS"], th["n_g"])
+    root = configs.zero_pi_tree(E_Cz)
+    H, kids = kfc.operator(root, spec)
+    assert H.dims == [40, 40, 30] and not H.fused
+    Ho = oracle_from_kfc_terms(H)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((H.N, 20))
+    assert _rel(_run(H, X), O.apply_chain(Ho, X)) < TOL64

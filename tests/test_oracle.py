@@ -206,3 +206,19 @@ def test_mlp_backward_by_finite_differences():
             ym, _ = O.mlp_forward(pp, X)
             fd = np.sum(dy * (yp - ym)) / (2 * h)
             assert abs(fd - g[idx]) < 1e-6 * max(1, abs(fd))
+
+
+def test_c_restatement_equals_numpy_chain():
+    from oracle import c_apply
+    if not c_apply.available():
+        pytest.skip("oracle/_build/liboracle_apply.so not built")
+    rng = np.random.default_rng(9)
+    H = O.three_fluxonia_operator([1.0, 0.9, 1.1], [0.5, 0.4, 0.6], [4, 5, 6], [0.1, 0.2, 0.3], N=11, order=2)
+    x = rng.standard_normal(H.N)
+    assert np.allclose(c_apply.CApply(H)(x), O.apply_chain(H, x), atol=1e-12)
+    dims = [4, 5, 3]
+    terms = [O.AxisTerm(0.7, [(0, O.DenseOp(rng.standard_normal((4, 4)))), (2, O.ShiftOp(3, 1, 0.5))]),
+             O.AxisTerm(-1.1, [(1, O.BandedOp(5, [-2, 1], [rng.standard_normal(5), rng.standard_normal(5)]))])]
+    H2 = O.kron_operator(dims, terms, 0.2)
+    x2 = rng.standard_normal(H2.N)
+    assert np.allclose(c_apply.CApply(H2)(x2), O.apply_chain(H2, x2), atol=1e-12)
