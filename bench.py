@@ -89,6 +89,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many restarts")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -105,7 +106,7 @@ def main():
 
     def step(mode):
         H = configs.three_fluxonia_operator(theta, N=args.N)
-        E, V, st = kfc.solve_stats(kfc.Lanczos(tol=args.tol, mode=mode), H, m)
+        E, V, st = kfc.solve_stats(kfc.Lanczos(tol=args.tol, mode=mode, verbosity=args.verbosity), H, m)
         return H, E, st
 
     # apply bandwidth (the dominant kernel)

@@ -145,9 +145,10 @@ int kfc_op_product_seed(kfc_op* op, int64_t k, double* X);
  * complex interleaved for KFC_C128) and fills stats (may be NULL). */
 
 /* Replaces lanczos_solve(::Lanczos, H, m) (src/solvers.jl:85-106):
- * thick-restart Lanczos, krylovdim = max(2m+10, krylovdim), residual tol. */
+ * thick-restart Lanczos, krylovdim = max(2m+10, krylovdim), residual tol;
+ * verbosity > 0 prints a progress line to stderr every `verbosity` restarts. */
 int kfc_lanczos(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxiter, uint64_t seed,
-                int32_t mode, double* E, double* V, kfc_stats* stats);
+                int32_t mode, int32_t verbosity, double* E, double* V, kfc_stats* stats);
 
 /* Replaces lobpcg_solve(::LOBPCG, H, m; X0) (src/lobpcg.jl:217-301).
  * precond: 0 = :ritz, 1 = :jacobi, 2 = :none; seed_kind: 0 = :product, 1 = :random.
@@ -157,6 +158,15 @@ enum kfc_seed { KFC_SEED_PRODUCT = 0, KFC_SEED_RANDOM = 1 };
 int kfc_lobpcg(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond,
                double eta, int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode,
                double* E, double* V, kfc_stats* stats);
+
+/* A parameter sweep of independent LOBPCG solves (as materialize runs one per
+ * sweep point, bench/common.jl:135-163): n operators, each created on one of a few
+ * contexts; one host thread per distinct context, the contexts' streams running
+ * concurrently.  X0s: NULL or n pointers (each NULL or prod(dims) x m); E[n x m];
+ * Vs: n host arrays; stats: NULL or n records. */
+int kfc_lobpcg_batch(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard, double tol, int64_t maxiter,
+                     int32_t precond, double eta, int32_t seed_kind, const double* const* X0s, uint64_t seed,
+                     int32_t mode, double* E, double* const* Vs, kfc_stats* stats);
 
 /* Replaces solve(::ChebyshevFilter, H, m) (src/chebyshev.jl:310-406).
  * ortho: 0 = :svqb, 1 = :cholqr; bound: a finite number, or NaN for :gershgorin. */

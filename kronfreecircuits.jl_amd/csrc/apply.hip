@@ -16,6 +16,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 // ------------------------------------------------------------------------------------------
 // Host: the fused diagonal (src/apply.jl:232-246, _add_term_diagonal! 438-447)
@@ -497,6 +501,140 @@ __global__ __launch_bounds__(256) void k_stencil(const StArgs<S> a) {
     }
 }
 
+// Variant without LDS or barriers: each thread owns one (i0, i1) column and marches
+// along axis 2 with a 4-slot register queue of its 3 x 3 lateral neighbourhood
+// (planes z-1, z, z+1 and the prefetched z+2), so every step issues 9 independent
+// loads and no synchronization.  CW: every position weight is a constant (the
+// per-axis vectors are all absent), so the weights stay scalar kernel arguments.
+template <class S, bool CW>
+struct St2 {
+    const StArgs<S>& a;
+    const S* __restrict__ Xc;
+    int64_t off[9];
+    bool ok[9];
+    S pre[27];
+    __device__ inline void load(S (&qq)[9], int z) const {
+        const bool zin = z >= 0 && z < a.d2;
+        const int64_t zo = (int64_t)z * a.s2;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) qq[k] = (zin && ok[k]) ? Xc[off[k] + zo] : s_zero(qq[k]);
+    }
+    __device__ inline S weight(int p, int z) const {
+        S w = CW ? a.c[p] : pre[p];
+        if (!CW && a.w[p]) w = s_mul(w, a.w[p][z]);
+        return w;
+    }
+};
+
+template <class S, bool CW>
+__device__ inline S st2_point(const St2<S, CW>& t, const S (&qm)[9], const S (&q0)[9], const S (&qp)[9], int z, S dv) {
+    S acc = s_mul(dv, q0[4]);
+#pragma unroll
+    for (int p = 0; p < 27; ++p) {
+        if ((t.a.used >> p) & 1u) {
+            const int dx = p % 3, dy = (p / 3) % 3, dz = p / 9;
+            const S& v = dz == 0 ? qm[dy * 3 + dx] : (dz == 1 ? q0[dy * 3 + dx] : qp[dy * 3 + dx]);
+            acc = s_fma(acc, t.weight(p, z), v);
+        }
+    }
+    return acc;
+}
+
+template <class S, bool CW>
+__global__ __launch_bounds__(256) void k_stencil_q(const StArgs<S> a) {
+    const int tid = threadIdx.x, lane = tid & 63, wy = tid >> 6;
+    const int nblk = gridDim.x;
+    const int per = nblk >> 3;
+    const int lg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lg >= a.nlog) return;
+    int rem = lg;
+    const int b0 = rem % a.nb0; rem /= a.nb0;
+    const int b1 = rem % a.nb1; rem /= a.nb1;
+    const int bz = rem % a.nbz; rem /= a.nbz;
+    const int col = rem;
+    const int i0 = b0 * ST_TX + lane, i1 = b1 * ST_TY + wy;
+    if (i0 >= a.d0 || i1 >= a.d1) return;   // no barriers in this kernel
+    const S one = s_one<S>(), zero = s_zero(one);
+    St2<S, CW> t{a, a.X + (int64_t)col * a.ld, {}, {}, {}};
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const int x = i0 + dx - 1, y = i1 + dy - 1;
+            t.ok[dy * 3 + dx] = x >= 0 && x < a.d0 && y >= 0 && y < a.d1;
+            t.off[dy * 3 + dx] = (int64_t)x + (int64_t)y * a.s1;
+        }
+    if (!CW) {
+#pragma unroll
+        for (int p = 0; p < 27; ++p) {
+            t.pre[p] = zero;
+            if ((a.used >> p) & 1u) {
+                S x = a.c[p];
+                if (a.u[p]) x = s_mul(x, a.u[p][i0]);
+                if (a.v[p]) x = s_mul(x, a.v[p][i1]);
+                t.pre[p] = x;
+            }
+        }
+    }
+    S d01 = zero;
+    if (a.D0) d01 = s_add(d01, a.D0[i0]);
+    if (a.D1) d01 = s_add(d01, a.D1[i1]);
+    S dpre[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        dpre[k] = zero;
+        if (k < a.ndp) {
+            S x = a.dpc[k];
+            if (a.dp0[k]) x = s_mul(x, a.dp0[k][i0]);
+            if (a.dp1[k]) x = s_mul(x, a.dp1[k][i1]);
+            dpre[k] = x;
+        }
+    }
+    S* __restrict__ Yc = a.Y + (int64_t)col * a.ld;
+    const S* __restrict__ Ac = a.A ? a.A + (int64_t)col * a.ld : nullptr;
+    const int z0 = bz * a.L, z1 = min(z0 + a.L, a.d2);
+    const int64_t base = (int64_t)i0 + (int64_t)i1 * a.s1;
+    auto emit = [&](const S (&qm)[9], const S (&q0)[9], const S (&qp)[9], int z) {
+        S dv = d01;
+        if (a.D2) dv = s_add(dv, a.D2[z]);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            if (kk < a.ndp) dv = s_fma(dv, dpre[kk], a.dp2[kk] ? a.dp2[kk][z] : one);
+        const int64_t idx = base + (int64_t)z * a.s2;
+        if (a.Dfull) dv = s_add(dv, a.Dfull[idx]);
+        S out = s_mul(a.alpha, st2_point<S, CW>(t, qm, q0, qp, z, dv));
+        out = s_fma(out, a.beta, q0[4]);
+        if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
+        Yc[idx] = out;
+    };
+    S qa[9], qb[9], qc[9], qd[9];
+    t.load(qa, z0 - 1);
+    t.load(qb, z0);
+    t.load(qc, z0 + 1);
+    int z = z0;
+    // four steps per iteration rotate the queue slots by name (no register moves)
+    for (; z + 3 < z1; z += 4) {
+        t.load(qd, z + 2);
+        emit(qa, qb, qc, z);
+        t.load(qa, z + 3);
+        emit(qb, qc, qd, z + 1);
+        t.load(qb, z + 4);
+        emit(qc, qd, qa, z + 2);
+        t.load(qc, z + 5);
+        emit(qd, qa, qb, z + 3);
+    }
+    // tail: at most three planes, queue (qa, qb, qc) = (z-1, z, z+1)
+    if (z < z1) {
+        t.load(qd, z + 2);
+        emit(qa, qb, qc, z);
+    }
+    if (z + 1 < z1) {
+        t.load(qa, z + 3);
+        emit(qb, qc, qd, z + 1);
+    }
+    if (z + 2 < z1) emit(qc, qd, qa, z + 2);
+}
+
 template <class S>
 static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld,
                           cd alpha, cd beta, cd gamma, hipStream_t st) {
@@ -539,7 +677,7 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     a.nb0 = int((a.d0 + ST_TX - 1) / ST_TX);
     a.nb1 = int((a.d1 + ST_TY - 1) / ST_TY);
     a.ncols = int(ncols);
-    int L = 64;
+    int L = op->stencil_march > 0 ? op->stencil_march : 64;
     while (L > 8 && (int64_t)a.nb0 * a.nb1 * ((a.d2 + L - 1) / L) * ncols < 8192) L >>= 1;
     L = std::min(L, a.d2);
     a.L = std::max(L, 1);
@@ -548,7 +686,16 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     if (nlog >= (1ll << 31) - 8) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "apply: grid too large");
     a.nlog = int(nlog);
     const int64_t nblk = (nlog + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    bool cw = true;
+    for (int p = 0; p < 27; ++p) cw &= !(a.u[p] || a.v[p] || a.w[p]);
+    const int variant = op->stencil_variant;
+    if (variant == 0) {
+        hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else if (cw) {
+        hipLaunchKernelGGL((k_stencil_q<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_stencil_q<S, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    }
     KFC_HIP(op->ctx, hipGetLastError());
     return KFC_OK;
 }
@@ -652,10 +799,86 @@ static int launch_chain(kfc_op* op, const void* Xv, void* Yv, int64_t ncols, int
     return KFC_OK;
 }
 
+// The stencil tuner: on the first application of an operator shape (dims, positions
+// used, scalar type, column count) every variant and march length is timed with HIP
+// events on the stream and the fastest is kept for every later call of that shape.
+struct TuneKey {
+    std::vector<int64_t> dims;
+    uint32_t used;
+    int scalar;
+    int64_t ncols;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(dims, used, scalar, ncols) < std::tie(o.dims, o.used, o.scalar, o.ncols);
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::pair<int, int>> g_tune;
+
+template <class S>
+static int tune_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, cd alpha,
+                        cd beta, cd gamma, hipStream_t st) {
+    uint32_t used = 0;
+    for (int p = 0; p < 27; ++p) used |= op->plan.pos[p].used ? (1u << p) : 0u;
+    TuneKey key{op->dims, used, int(sizeof(S)), ncols};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            op->stencil_variant = it->second.first;
+            op->stencil_march = it->second.second;
+            return KFC_OK;
+        }
+    }
+    if (const char* env = getenv("KFC_STENCIL_VARIANT")) {
+        op->stencil_variant = atoi(env);
+        return KFC_OK;
+    }
+    const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}};
+    hipEvent_t e0, e1;
+    KFC_HIP(op->ctx, hipEventCreate(&e0));
+    KFC_HIP(op->ctx, hipEventCreate(&e1));
+    float best = 1e30f;
+    std::pair<int, int> pick = cands[1];
+    for (const auto& c : cands) {
+        op->stencil_variant = c.first;
+        op->stencil_march = c.second;
+        int rc = launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);   // warm
+        if (rc != KFC_OK) return rc;
+        KFC_HIP(op->ctx, hipEventRecord(e0, st));
+        for (int r = 0; r < 3; ++r) {
+            rc = launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+            if (rc != KFC_OK) return rc;
+        }
+        KFC_HIP(op->ctx, hipEventRecord(e1, st));
+        KFC_HIP(op->ctx, hipEventSynchronize(e1));
+        float ms = 0;
+        KFC_HIP(op->ctx, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    op->stencil_variant = pick.first;
+    op->stencil_march = pick.second;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = pick;
+    return KFC_OK;
+}
+
 template <class S>
 static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
                    cd gamma, hipStream_t st) {
-    if (op->plan.ok) return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+    if (op->plan.ok) {
+        const int prec = std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1;
+        if (op->tuned_cols[prec] != ncols) {
+            int rc = tune_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+            if (rc != KFC_OK) return rc;
+            op->tuned_cols[prec] = ncols;
+        }
+        return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+    }
     int rc;
     if constexpr (!IsCx<S>::value) {
         if (op->pairs.ok) rc = launch_pairs<S>(op, X, Y, ncols, ld, st);

@@ -70,6 +70,7 @@ struct Cheb {
     CTimer tm;
     int64_t N;
     std::mt19937_64 rng;
+    double cutoff = SVQB_CUTOFF;   // relative Gram eigenvalue cutoff (raised for FP32 storage)
     S* col(S* b, int c) { return b + size_t(c) * N; }
 
     // Z <- Z - L (L^H Z), twice (_project_out!, src/chebyshev.jl:57-63)
@@ -114,7 +115,7 @@ struct Cheb {
         std::vector<int> keep;
         if (top > 0)
             for (int i = 0; i < nz; ++i)
-                if (lam[i] > SVQB_CUTOFF * top) keep.push_back(i);
+                if (lam[i] > cutoff * top) keep.push_back(i);
         k = int(keep.size());
         if (k == 0) return KFC_OK;
         std::vector<cd> M(size_t(nz) * k);
@@ -227,10 +228,11 @@ double partial_m(const std::vector<double>& th, int n, int m) {   // partialsort
     return v[m - 1];
 }
 
-int chebyshev_degree(int degree, double t1, double lcut, double lmax) {   // src/chebyshev.jl:169-174
+// The amplification cap 1/sqrt(eps) uses the storage precision's eps (src/chebyshev.jl:169-174).
+int chebyshev_degree(int degree, double t1, double lcut, double lmax, double eps) {
     const double x = std::fabs((t1 - (lmax + lcut) / 2) / ((lmax - lcut) / 2));
     if (!(x > 1)) return degree;
-    const double cap = std::acosh(1.0 / std::sqrt(EPS)) / std::acosh(x);
+    const double cap = std::acosh(1.0 / std::sqrt(eps)) / std::acosh(x);
     return cap >= degree ? degree : std::max(int(std::floor(cap)), 1);
 }
 
@@ -267,9 +269,11 @@ int cheb_run(kfc_op* op, const ChebCfg& c, const std::vector<cd>& X0h, uint64_t 
     const int64_t N = op->N;
     const int nb = int(std::min<int64_t>(c.m + c.extra, N));
     const int m = c.m;
+    const double eps_s = prec_of<S>() == 0 ? EPS : 1.1920928955078125e-07;
     const int g = std::max(std::min(std::min(c.extra - 1, CHEB_GUARDS), nb - 1 - m), 0);
     Cheb<S> ch{op, Work(), CTimer(), N, std::mt19937_64(seed)};
     ch.w.ctx = op->ctx;
+    if (prec_of<S>() == 1) ch.cutoff = 1e-6;
     DevBuf b0, b1, b2, bX, bHX;
     for (DevBuf* b : {&b0, &b1, &b2, &bX, &bHX}) KFC_HIP(op->ctx, b->ensure(sizeof(S) * size_t(N) * nb));
     S* bufs[3] = {b0.as<S>(), b1.as<S>(), b2.as<S>()};
@@ -312,7 +316,7 @@ int cheb_run(kfc_op* op, const ChebCfg& c, const std::vector<cd>& X0h, uint64_t 
         if (rc != KFC_OK) return rc;
         for (int j = 0; j < na; ++j) res[nlock + j] = rn[j];
         const bool filtered = it >= 1 || nb == N || seed_locked_ok;
-        const double floor = 10 * EPS * std::max(std::fabs(c.hbound), std::fabs(*std::min_element(th.begin(), th.end())));
+        const double floor = 10 * eps_s * std::max(std::fabs(c.hbound), std::fabs(*std::min_element(th.begin(), th.end())));
         while (filtered && nlock < nb && res[nlock] <= LOCK_FRACTION * std::max(c.tol, floor)) ++nlock;
         const int need = m + counted_guards(th, m, g, lcut, lmax, deg);
         if (chebyshev_done(th, nlock, m, need)) break;
@@ -337,7 +341,7 @@ int cheb_run(kfc_op* op, const ChebCfg& c, const std::vector<cd>& X0h, uint64_t 
         lcut = th[nb - 1];   // _chebyshev_cut (src/chebyshev.jl:157)
         if (!(lcut < lmax))
             return kfc_fail(op->ctx, KFC_EARG, "ChebyshevFilter: the block's largest Ritz value reaches the upper end lambda_max of the filter interval; the block spans the top of the spectrum, use Dense()");
-        deg = chebyshev_degree(c.degree, *std::min_element(th.begin(), th.end()), lcut, lmax);
+        deg = chebyshev_degree(c.degree, *std::min_element(th.begin(), th.end()), lcut, lmax, eps_s);
         // the filter (src/chebyshev.jl:38-54)
         const double cc = (lmax + lcut) / 2, e = (lmax - lcut) / 2;
         S* A = bufs[0];

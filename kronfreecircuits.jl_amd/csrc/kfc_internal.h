@@ -197,6 +197,9 @@ struct kfc_op {
     bool hermitian = true;
     std::vector<cd> hdiag;          // host fused diagonal (built on demand)
     StencilPlan plan;
+    int stencil_variant = 1;        // 0: LDS plane tile, 1: register queue (chosen by the tuner)
+    int stencil_march = 0;          // march chunk along the slowest axis (0: default)
+    int64_t tuned_cols[2] = {-1, -1};   // column count the stencil variant was tuned for, per precision
     PairPlan pairs;
     DeviceData dd[2];               // [KFC_PREC_F64], [KFC_PREC_F32]
     double gershgorin = 0;

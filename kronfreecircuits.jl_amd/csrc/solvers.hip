@@ -6,6 +6,7 @@
 #include <cmath>
 #include <numeric>
 #include <random>
+#include <thread>
 
 #include "blas.h"
 
@@ -181,7 +182,7 @@ struct LanczosOut {
 
 template <class S>
 int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter, const S* start, int64_t start_ld,
-                 uint64_t seed, Timer& tm, LanczosOut<S>& out, DevBuf& result) {
+                 uint64_t seed, Timer& tm, LanczosOut<S>& out, DevBuf& result, int verbosity) {
     const int64_t N = op->N;
     DevBuf bufA, bufB;
     KFC_HIP(op->ctx, bufA.ensure(sizeof(S) * size_t(N) * (kd + 1)));
@@ -217,8 +218,9 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             for (int i = 0; i <= j; ++i) hs[i] = scale[i] * h[i];
             rc = lz_update<S>(w, V, N, j + 1, hs, wv, N, &wn1);
             if (rc != KFC_OK) return rc;
-            // a second classical Gram-Schmidt pass when the first removed most of w
-            if (wn1 < 1e-4 * wn0) {
+            // classical Gram-Schmidt twice ("twice is enough"): a single pass loses
+            // orthogonality geometrically across thick restarts
+            {
                 rc = lz_multidot<S>(w, V, N, j + 1, scale.data(), wv, N, h2, nullptr);
                 if (rc != KFC_OK) return rc;
                 for (int i = 0; i <= j; ++i) hs[i] = scale[i] * h2[i], h[i] += h2[i];
@@ -255,6 +257,9 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
         for (int i = 0; i < n; ++i) resid[i] = out.invariant ? 0.0 : beta_last * std::abs(Y[(n - 1) + size_t(i) * n]);
         int conv = 0;
         while (conv < n && resid[conv] <= tol) ++conv;
+        if (verbosity > 0 && (rs % verbosity == 0))
+            fprintf(stderr, "[kfc lanczos] restart %lld matvec %lld converged %d resid[m-1] %.3e theta0 %.12f\n",
+                    (long long)rs, (long long)tm.cols, conv, resid[std::min(m, n) - 1], theta[0]);
         out.theta = theta;
         out.resid = resid;
         out.converged = conv;
@@ -290,7 +295,7 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
 
 template <class S>
 int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxiter, uint64_t seed, int mode,
-              double* Eout, double* Vout, kfc_stats* st) {
+              int verbosity, double* Eout, double* Vout, kfc_stats* st) {
     auto t0 = clk::now();
     const int64_t N = op->N;
     if (m < 1) return kfc_fail(op->ctx, KFC_EARG, "Lanczos requires m >= 1, got " + std::to_string(m));
@@ -313,7 +318,7 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
         const double tol32 = std::max(tol, 64.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
         LanczosOut<L> o32;
         DevBuf res32;
-        rc = lanczos_core<L>(op, w, int(m), kd, tol32, maxiter, nullptr, 0, seed, tm, o32, res32);
+        rc = lanczos_core<L>(op, w, int(m), kd, tol32, maxiter, nullptr, 0, seed, tm, o32, res32, verbosity);
         if (rc != KFC_OK) return rc;
         (void)hn;
         // FP64 continuation from the FP32 Ritz vectors combined into one start vector
@@ -330,10 +335,10 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
         KFC_HIP(op->ctx, st64.ensure(sizeof(S) * size_t(N)));
         rc = blas_gemm<S>(w, s64.as<S>(), N, q, comb, 1, st64.as<S>(), N, N, 0.0);
         if (rc != KFC_OK) return rc;
-        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, st64.as<S>(), N, seed, tm, out, res64);
+        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, st64.as<S>(), N, seed, tm, out, res64, verbosity);
         out.restarts += o32.restarts;
     } else {
-        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, nullptr, 0, seed, tm, out, res64);
+        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, nullptr, 0, seed, tm, out, res64, verbosity);
     }
     if (rc != KFC_OK) return rc;
     const int n = int(out.theta.size());
@@ -896,10 +901,10 @@ int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats) {
 }
 
 int kfc_lanczos(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxiter, uint64_t seed, int32_t mode,
-                double* E, double* V, kfc_stats* stats) {
+                int32_t verbosity, double* E, double* V, kfc_stats* stats) {
     if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_lanczos: null argument");
-    return op->cplx ? lanczos_t<Cx<double>>(op, m, krylovdim, tol, maxiter, seed, mode, E, V, stats)
-                    : lanczos_t<double>(op, m, krylovdim, tol, maxiter, seed, mode, E, V, stats);
+    return op->cplx ? lanczos_t<Cx<double>>(op, m, krylovdim, tol, maxiter, seed, mode, verbosity, E, V, stats)
+                    : lanczos_t<double>(op, m, krylovdim, tol, maxiter, seed, mode, verbosity, E, V, stats);
 }
 
 int kfc_lobpcg(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond, double eta,
@@ -916,3 +921,43 @@ int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, doubl
 }
 
 }  // extern "C"
+
+// Independent LOBPCG solves of several operators, one host thread per distinct
+// context: each context's stream carries its operators' solves in order, and the
+// contexts' streams run concurrently on the device (a parameter sweep).
+extern "C" int kfc_lobpcg_batch(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard, double tol, int64_t maxiter,
+                                int32_t precond, double eta, int32_t seed_kind, const double* const* X0s,
+                                uint64_t seed, int32_t mode, double* E, double* const* Vs, kfc_stats* stats) {
+    if (n < 1 || !ops || !E || !Vs) return kfc_fail(n > 0 && ops && ops[0] ? ops[0]->ctx : nullptr, KFC_EARG,
+                                                    "kfc_lobpcg_batch: bad arguments");
+    std::vector<kfc_ctx*> ctxs;
+    for (int i = 0; i < n; ++i) {
+        if (!ops[i]) return kfc_fail(ops[0]->ctx, KFC_EARG, "kfc_lobpcg_batch: null operator");
+        if (std::find(ctxs.begin(), ctxs.end(), ops[i]->ctx) == ctxs.end()) ctxs.push_back(ops[i]->ctx);
+    }
+    std::vector<int> rcs(n, KFC_OK);
+    std::vector<std::thread> th;
+    for (kfc_ctx* c : ctxs) {
+        th.emplace_back([&, c]() {
+            (void)hipSetDevice(c->device);
+            for (int i = 0; i < n; ++i) {
+                if (ops[i]->ctx != c) continue;
+                kfc_op* op = ops[i];
+                rcs[i] = op->cplx
+                             ? lobpcg_t<Cx<double>>(op, m, guard, tol, maxiter, precond, eta, seed_kind,
+                                                    X0s ? X0s[i] : nullptr, seed + i, mode, E + i * m, Vs[i],
+                                                    stats ? stats + i : nullptr)
+                             : lobpcg_t<double>(op, m, guard, tol, maxiter, precond, eta, seed_kind,
+                                                X0s ? X0s[i] : nullptr, seed + i, mode, E + i * m, Vs[i],
+                                                stats ? stats + i : nullptr);
+            }
+        });
+    }
+    for (auto& t : th) t.join();
+    for (int i = 0; i < n; ++i)
+        if (rcs[i] != KFC_OK) {
+            std::string msg = "instance " + std::to_string(i) + ": " + ops[i]->ctx->err;
+            return kfc_fail(ops[0]->ctx, rcs[i], msg);
+        }
+    return KFC_OK;
+}

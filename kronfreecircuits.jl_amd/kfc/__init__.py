@@ -1,17 +1,18 @@
 """kfc: the Kronecker-free apply and eigensolver loops of KronFreeCircuits.jl on
 the MI355X, over the C ABI of libkfc.so (include/kfc.h)."""
-from ._lib import (ArgumentError, BreakdownError, ConvergenceError, DeviceError, DimensionMismatch,
+from ._lib import (Context, ArgumentError, BreakdownError, ConvergenceError, DeviceError, DimensionMismatch,
                    UnsupportedError, context, lib, missing_symbols)
 from .bases import ChargeBasis, FockBasis, PeriodicGrid, PhaseGrid, fd_weights, localop
 from .localops import BandedOp, DenseOp, DiagonalOp, LocalOp, ShiftOp, dense, imagpart, realpart
 from .operator import AxisTerm, KronFreeOperator
-from .solvers import LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, residuals, resolve, solve, solve_stats
+from .solvers import (LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, residuals, resolve, solve, solve_batch,
+                      solve_stats)
 from .tree import (MaterializedNode, Node, NumericSpec, System, circuit_terms, constant_term, materialize, operator,
                    port_matrix)
 
-__all__ = ["ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
+__all__ = ["Context", "ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
            "UnsupportedError", "context", "lib", "missing_symbols", "ChargeBasis", "FockBasis", "PeriodicGrid",
            "PhaseGrid", "fd_weights", "localop", "BandedOp", "DenseOp", "DiagonalOp", "LocalOp", "ShiftOp", "dense",
            "imagpart", "realpart", "AxisTerm", "KronFreeOperator", "LOBPCG", "Auto", "ChebyshevFilter", "Dense",
-           "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_stats", "MaterializedNode", "Node",
+           "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_batch", "solve_stats", "MaterializedNode", "Node",
            "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "operator", "port_matrix"]

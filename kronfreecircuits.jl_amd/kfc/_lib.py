@@ -93,10 +93,13 @@ SIGNATURES = {
     "kfc_op_diagonal": (C.c_int, [_P, _D]),
     "kfc_op_gershgorin": (C.c_int, [_P, _D]),
     "kfc_op_product_seed": (C.c_int, [_P, C.c_int64, _D]),
-    "kfc_lanczos": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_int32, _D, _D,
+    "kfc_lanczos": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, _D, _D,
                               C.POINTER(Stats)]),
     "kfc_lobpcg": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int32, C.c_double, C.c_int32, _D,
                              C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
+    "kfc_lobpcg_batch": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int32,
+                                   C.c_double, C.c_int32, C.POINTER(_D), C.c_uint64, C.c_int32, _D, C.POINTER(_D),
+                                   C.POINTER(Stats)]),
     "kfc_chebyshev": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_double,
                                 C.c_int32, C.c_double, C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
     "kfc_dense": (C.c_int, [_P, C.c_int64, _D, _D, C.POINTER(Stats)]),

@@ -47,7 +47,7 @@ class KronFreeOperator:
     by the library; the element type is complex when any coefficient, the
     constant or any factor's data is complex (src/apply.jl:232-233)."""
 
-    def __init__(self, dims, terms, constant=0.0, block_capacity=1, hermitian=True, device=0):
+    def __init__(self, dims, terms, constant=0.0, block_capacity=1, hermitian=True, device=0, ctx=None):
         dims = [int(d) for d in dims]
         if any(d < 1 for d in dims):
             raise L.ArgumentError(f"KronFreeOperator: dims must be positive, got {dims}")
@@ -68,7 +68,7 @@ class KronFreeOperator:
         self.terms = terms
         self.constant = constant
         self.hermitian = hermitian
-        self.ctx = L.context(device)
+        self.ctx = ctx if ctx is not None else L.context(device)
         keep = []
         tdescs = (L.TermDesc * max(1, len(terms)))()
         for ti, t in enumerate(terms):

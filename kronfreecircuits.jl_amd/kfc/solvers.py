@@ -140,7 +140,8 @@ def solve_stats(s, H, m, X0=None):
     E, V = _out(H, m)
     if isinstance(s, Lanczos):
         rc = lib.kfc_lanczos(H.handle, int(m), int(s.krylovdim), float(s.tol), int(s.maxiter), int(s.seed),
-                             _mode(s.mode), E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
+                             _mode(s.mode), int(s.verbosity), E.ctypes.data_as(L._D), V.ctypes.data_as(L._D),
+                             C.byref(st))
     elif isinstance(s, LOBPCG):
         x0 = None
         if X0 is not None:
@@ -185,3 +186,33 @@ def residuals(H, E, V):
     L.check(L.lib().kfc_residuals(H.handle, m, E.ctypes.data_as(L._D), Vb.ctypes.data_as(L._D),
                                   out.ctypes.data_as(L._D)), H.ctx.handle)
     return out
+
+
+def solve_batch(s, Hs, m, X0s=None):
+    """Independent LOBPCG solves of a parameter sweep (``kfc_lobpcg_batch``):
+    operators built on different contexts (``kfc._lib.Context``) run their solves
+    concurrently on the device.  Returns lists of (E, V) and the stats."""
+    if not isinstance(s, LOBPCG):
+        raise L.ArgumentError("solve_batch serves LOBPCG sweeps")
+    n = len(Hs)
+    ops = (C.c_void_p * n)(*[H.handle.value for H in Hs])
+    E = np.empty(n * m)
+    Vb = [np.empty(H.N * m * (2 if H.dtype == np.complex128 else 1)) for H in Hs]
+    Vp = (L._D * n)(*[v.ctypes.data_as(L._D) for v in Vb])
+    x0p, keep = None, []
+    if X0s is not None:
+        x0p = (L._D * n)()
+        for i, (H, X0) in enumerate(zip(Hs, X0s)):
+            if X0 is None:
+                x0p[i] = L._D()
+            else:
+                b = pack_block(np.asarray(X0), H.dtype)
+                keep.append(b)
+                x0p[i] = b.ctypes.data_as(L._D)
+    st = (L.Stats * n)()
+    L.check(L.lib().kfc_lobpcg_batch(n, ops, int(m), int(s.guard), float(s.tol), int(s.maxiter),
+                                     L.KFC_PRECOND[s.preconditioner], float(s.eta), L.KFC_SEED[s.seed], x0p,
+                                     int(s.rng_seed), _mode(s.mode), E.ctypes.data_as(L._D), Vp, st),
+            Hs[0].ctx.handle)
+    out = [(E[i * m:(i + 1) * m].copy(), unpack_block(Vb[i], Hs[i].N, m, Hs[i].dtype)) for i in range(n)]
+    return out, [x.as_dict() for x in st]

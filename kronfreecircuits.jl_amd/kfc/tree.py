@@ -220,6 +220,45 @@ def route(root, terms):
     return routed
 
 
+def positive_mask(node, dims):
+    """The phase mask of a node's product basis (src/materialize.jl:470-472,
+    src/bases.jl positive_mask): a basis child's coordinates >= 0, all of a node
+    child's kept states; axis 0 fastest."""
+    mask = np.ones(1, dtype=bool)
+    for ch, d in zip(node.children, dims):
+        if isinstance(ch, tuple):
+            b = ch[1]
+            if hasattr(b, "coordinates"):
+                m = b.coordinates() >= 0
+            elif hasattr(b, "ncut"):
+                m = np.arange(-b.ncut, b.ncut + 1) >= 0
+            else:
+                m = np.ones(d, dtype=bool)
+        else:
+            m = np.ones(d, dtype=bool)
+        mask = np.kron(m, mask).astype(bool)
+    return mask
+
+
+def fix_phases(V, mask):
+    """``fix_phases!(V, :positive_sector, mask)`` (src/phase.jl:12-33): the
+    largest-modulus entry among the mask made real and positive (the global
+    largest when the column has no weight there)."""
+    V = np.array(V)
+    sector = np.flatnonzero(mask)
+    for j in range(V.shape[1]):
+        col = V[:, j]
+        i = sector[np.argmax(np.abs(col[sector]))] if len(sector) else 0
+        if len(sector) == 0 or abs(col[i]) < 1e-14:
+            i = int(np.argmax(np.abs(col)))
+        x = col[i]
+        if np.iscomplexobj(V):
+            V[:, j] = col * np.exp(-1j * np.angle(x))
+        elif np.real(x) < 0:
+            V[:, j] = -col
+    return V
+
+
 class System:
     """A materialized tree: ``energies(sys)``, ``port(sys, mode, port)``."""
 
@@ -284,6 +323,6 @@ def materialize(root, spec, previous=None, X0=None, _routed=None, _is_root=True)
                 x0 = np.concatenate([pv[:, :k], np.random.default_rng(0).standard_normal((pv.shape[0], m - k))], 1)
         E, V, st = solve_stats(s, H, m if not isinstance(s, Dense) else m, x0)
         keep = min(nd.keep, len(E))
-        return MaterializedNode(nd, dims, np.asarray(E)[:keep], np.asarray(V)[:, :keep], kids, H, s, st,
-                                time.perf_counter() - t0)
+        V = fix_phases(np.asarray(V)[:, :keep], positive_mask(nd, dims))
+        return MaterializedNode(nd, dims, np.asarray(E)[:keep], V, kids, H, s, st, time.perf_counter() - t0)
     return System(visit(root, _is_root))

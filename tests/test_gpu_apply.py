@@ -116,7 +116,7 @@ def test_matrix_equals_kron_reference():
     H = to_kfc(dims, terms, 1.5)
     Ho = to_oracle(dims, terms, 1.5)
     M = _run(H, np.eye(H.N, dtype=complex))
-    assert np.allclose(M, O.kron_reference(dims, Ho.terms + Ho.diagterms, 0.0), atol=1e-12)
+    assert np.allclose(M, O.kron_reference(dims, Ho.terms + Ho.diagterms, 1.5), atol=1e-12)
 
 
 def test_derived_quantities():
