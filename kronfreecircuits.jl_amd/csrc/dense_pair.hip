@@ -23,6 +23,13 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ inline f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
+// Row of the 16x16 output tile held in accumulator register i of a lane in
+// quarter k4 (lane / 16); the column is lane % 16.  The f64 instruction
+// interleaves rows across the quarters, the f32 one gives each quarter four
+// consecutive rows.
+__device__ inline int out_row(double, int k4, int i) { return k4 + 4 * i; }
+__device__ inline int out_row(float, int k4, int i) { return 4 * k4 + i; }
+
 template <class T> struct V4;
 template <> struct V4<double> { using type = d4; };
 template <> struct V4<float> { using type = f4; };
@@ -88,7 +95,7 @@ __global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
                     c = mfma(av, bv, c);
                 }
 #pragma unroll
-                for (int i = 0; i < 4; ++i) Ts[(ti * 16 + k4 * 4 + i) + (tj * 16 + r16) * P] = c[i];
+                for (int i = 0; i < 4; ++i) Ts[(ti * 16 + out_row(T(0), k4, i)) + (tj * 16 + r16) * P] = c[i];
             }
             __syncthreads();
             // acc += c_t A Ts : (dp x dp)(dp x dq)
@@ -117,7 +124,7 @@ __global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
             const int cc = tj * 16 + r16;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int row = ti * 16 + k4 * 4 + i;
+                const int row = ti * 16 + out_row(T(0), k4, i);
                 if (row < a.dp && cc < a.dq) {
                     const int64_t idx = base + row * a.sp + cc * a.sq;
                     const int64_t didx = idx - col * a.ld;

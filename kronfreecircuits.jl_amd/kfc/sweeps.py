@@ -1,0 +1,183 @@
+"""Parameter sweeps of the zero-pi tree and its learned warm start.
+
+A sweep point is one draw of (E_CS, n_g) of the zero-pi tree (the workload
+described in kfc.configs); its parent node [40, 40, 30] is solved by LOBPCG for
+the lowest 16 pairs.  Three ways to run a sweep:
+
+* plain: one point after another, FP64 throughout, product-state seed (the
+  reference's materialize per point);
+* batched: the parent solves of many points in flight at once on several
+  device contexts (kfc_lobpcg_batch), FP32 bulk iterations with FP64
+  Rayleigh-Ritz and refinement (mode="mixed");
+* learned: batched, plus an initial block X0 predicted from the parameters by
+  the warm-start network.
+
+The warm start works in a canonical frame of the parent basis that does not
+move with the parameters: the theta leaf's kept states are charge states
+|n>, whose energy order changes with n_g, so a parent vector is re-indexed by
+(charge n, phi level, zeta level) over a fixed window.  A fixed 64-vector basis
+B of that frame is taken from the training eigenvectors (their leading
+principal directions); the network maps the six circuit parameters to the 64 x
+16 coefficients of the lowest 16 parent eigenvectors on B, and
+X0 = B C mapped back to the parent basis of the point."""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import configs
+from .solvers import LOBPCG, solve_batch, solve_stats
+from .tree import materialize_child, operator
+
+WINDOW = (-12, 14)       # charges n in [-12, 14) of the theta leaf
+PHI_LEVELS = 20
+ZETA_LEVELS = 12
+
+
+def features(theta):
+    """The six network inputs: (E_CS, n_g, E_J, E_L, dE_J, varphi_ext / 2pi), each
+    mapped to about [-1, 1] over its range (E_J, E_L, dE_J and varphi_ext are
+    held at their fixed values in these sweeps)."""
+    return np.array([(theta["E_CS"] - 0.11) / 0.09, (theta["n_g"] - 0.5) / 0.5, (theta.get("E_J", 10.0) - 10.0),
+                     (theta.get("E_L", 0.05) - 0.05) / 0.01, (theta.get("dE_J", 0.05) - 0.05) / 0.02,
+                     (theta.get("varphi_ext", 2 * math.pi * 0.23) / (2 * math.pi) - 0.23) / 0.05], dtype=np.float32)
+
+
+class ZeroPiSweep:
+    def __init__(self, ncut=40, Nphi=256, Nzeta=30, keep=40, m=16, contexts=4):
+        self.ncut, self.Nphi, self.Nzeta, self.keep, self.m = ncut, Nphi, Nzeta, keep, m
+        self.ctxs = [L.context(0)] + [L.Context(0) for _ in range(max(0, contexts - 1))]
+        self._phi = None
+
+    def tree(self, theta):
+        spec, E_Cz = configs.zero_pi_spec(theta["E_CS"], theta["n_g"])
+        root = configs.zero_pi_tree(E_Cz, ncut=self.ncut, Nphi=self.Nphi, Nzeta=self.Nzeta, keep=self.keep, m=self.m)
+        return spec, root
+
+    def parent(self, theta, ctx=None):
+        """The parent operator of one sweep point (the phi leaf, which no swept
+        parameter touches, is materialized once)."""
+        spec, root = self.tree(theta)
+        th = materialize_child(root, root.children[0], spec)
+        if self._phi is None:
+            self._phi = materialize_child(root, root.children[1], spec)
+        ph = self._phi
+        ph.node = root.children[1]
+        ze = materialize_child(root, root.children[2], spec)
+        H, kids = operator(root, spec, kids=[th, ph, ze], ctx=ctx, block_capacity=self.m + 4)
+        return H, kids
+
+    # -- the canonical frame ---------------------------------------------------------------
+    def frame_index(self, kids):
+        """Parent-basis flat index of every canonical (n, phi level, zeta level) in the
+        window, or -1 when that charge is not among the kept theta states."""
+        Vt = kids[0].vectors
+        charge = np.argmax(np.abs(Vt), axis=0) - self.ncut          # kept index -> charge n
+        pos = {int(n): k for k, n in enumerate(charge)}
+        d0, d1 = Vt.shape[1], kids[1].vectors.shape[1]
+        idx = []
+        for iz in range(ZETA_LEVELS):
+            for ip in range(PHI_LEVELS):
+                for n in range(*WINDOW):
+                    k = pos.get(n, -1)
+                    idx.append(-1 if k < 0 else k + d0 * (ip + d1 * iz))
+        return np.array(idx)
+
+    def to_frame(self, V, idx):
+        out = np.zeros((len(idx), V.shape[1]))
+        ok = idx >= 0
+        out[ok] = V[idx[ok]]
+        return out
+
+    def from_frame(self, Xf, idx, N):
+        X = np.zeros((N, Xf.shape[1]))
+        ok = idx >= 0
+        X[idx[ok]] = Xf[ok]
+        return X
+
+    # -- sweeps ----------------------------------------------------------------------------
+    def run_plain(self, thetas, tol=1e-8):
+        """One point after another, FP64, product seed."""
+        out, t0 = [], time.perf_counter()
+        for th in thetas:
+            H, kids = self.parent(th)
+            E, V, st = solve_stats(LOBPCG(tol=tol, mode="f64"), H, self.m)
+            out.append((E, V, st, kids))
+        return out, time.perf_counter() - t0
+
+    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, chunk=None):
+        """All points' parent solves in flight on the contexts (round-robin)."""
+        t0 = time.perf_counter()
+        Hs, kidss = [], []
+        for i, th in enumerate(thetas):
+            H, kids = self.parent(th, ctx=self.ctxs[i % len(self.ctxs)])
+            Hs.append(H)
+            kidss.append(kids)
+        X0s = None
+        if x0_fn is not None:
+            X0s = x0_fn(thetas, Hs, kidss)
+        res, st = solve_batch(LOBPCG(tol=tol, mode=mode), Hs, self.m, X0s)
+        wall = time.perf_counter() - t0
+        return [(E, V, s, k) for (E, V), s, k in zip(res, st, kidss)], wall
+
+
+class WarmStart:
+    """The learned warm start: a fixed 64-vector basis of the canonical frame and
+    the network mapping parameters to the 64 x 16 coefficients on it."""
+
+    def __init__(self, sweep, nbasis=64, hidden=256, seed=0):
+        from .mlp import MLP
+        self.sw, self.nb, self.m = sweep, nbasis, sweep.m
+        self.net = MLP((6, hidden, hidden, nbasis * sweep.m), seed=seed)
+        self.B = None
+        self.signref = None
+
+    def fit_basis(self, frames):
+        """B: the leading principal directions of the stacked training eigenvectors."""
+        Z = np.concatenate(frames, axis=1)
+        G = Z.T @ Z
+        w, U = np.linalg.eigh(G)
+        top = np.argsort(w)[::-1][:self.nb]
+        B = Z @ U[:, top] / np.sqrt(np.maximum(w[top], 1e-300))
+        Q, _ = np.linalg.qr(B)
+        self.B = Q
+
+    def coeffs(self, Vf):
+        C = self.B.T @ Vf                       # nb x m
+        if self.signref is None:
+            self.signref = np.argmax(np.abs(C), axis=0)
+        s = np.sign(C[self.signref, np.arange(C.shape[1])])
+        s[s == 0] = 1
+        return C * s
+
+    def train(self, thetas, Vframes, steps=3000, lr=1e-3, batch=256):
+        self.fit_basis(Vframes)
+        X = np.stack([features(t) for t in thetas])
+        T = np.stack([self.coeffs(Vf).reshape(-1, order="F") for Vf in Vframes]).astype(np.float32)
+        self.scale = float(np.max(np.abs(T))) or 1.0
+        loss = self.net.train(X, T / self.scale, batch=min(batch, len(X)), steps=steps, lr=lr)
+        return loss
+
+    def predict_frames(self, thetas):
+        X = np.stack([features(t) for t in thetas])
+        Y = self.net.forward(X) * self.scale
+        return [self.B @ y.reshape(self.nb, self.m, order="F") for y in Y]
+
+    def x0_fn(self, thetas, Hs, kidss):
+        out = []
+        for Xf, H, kids in zip(self.predict_frames(thetas), Hs, kidss):
+            X0 = self.sw.from_frame(Xf, self.sw.frame_index(kids), H.N)
+            bad = np.linalg.norm(X0, axis=0) == 0
+            if np.any(bad):
+                X0[:, bad] = np.random.default_rng(0).standard_normal((H.N, int(bad.sum())))
+            out.append(X0)
+        return out
+
+
+def subspace_capture(X0, V):
+    """The weight of each true eigenvector in span(X0) (1 = captured)."""
+    Q, _ = np.linalg.qr(X0)
+    return np.sum((Q.T @ V) ** 2, axis=0)

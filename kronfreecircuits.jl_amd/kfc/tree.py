@@ -272,20 +272,29 @@ class System:
         return port_matrix(self.root, mode, port)
 
 
-def operator(root, spec):
+def operator(root, spec, kids=None, ctx=None, block_capacity=1):
     """The root node's KronFreeOperator with its children materialized, not
-    solved (``operator(sys)``, src/materialize.jl:982-992)."""
+    solved (``operator(sys)``, src/materialize.jl:982-992).  ``kids`` may pass
+    already materialized children (a sweep reuses the ones that do not depend
+    on the swept parameters); ``ctx`` picks the device context."""
     routed = route(root, circuit_terms(spec))
-    kids = []
-    for ch in root.children:
-        if isinstance(ch, tuple):
-            kids.append(None)
-            continue
-        sub = materialize(ch, spec, _routed=routed, _is_root=False)
-        kids.append(sub.root)
+    if kids is None:
+        kids = []
+        for ch in root.children:
+            if isinstance(ch, tuple):
+                kids.append(None)
+                continue
+            sub = materialize(ch, spec, _routed=routed, _is_root=False)
+            kids.append(sub.root)
     dims = [ch[1].dim if isinstance(ch, tuple) else len(k.energies) for ch, k in zip(root.children, kids)]
     ts = node_axisterms(root, kids, routed.get(id(root), []), root.real)
-    return KronFreeOperator(dims, ts, constant=constant_term(spec)), kids
+    return KronFreeOperator(dims, ts, constant=constant_term(spec), ctx=ctx, block_capacity=block_capacity), kids
+
+
+def materialize_child(root, child, spec):
+    """One child node of ``root`` materialized with the terms ``route`` gives it."""
+    routed = route(root, circuit_terms(spec))
+    return materialize(child, spec, _routed=routed, _is_root=False).root
 
 
 def materialize(root, spec, previous=None, X0=None, _routed=None, _is_root=True):
