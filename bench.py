@@ -78,6 +78,114 @@ def cpu_baseline(theta, N, n_matvec_gpu, m):
             "seconds_per_apply": round(t, 4)}
 
 
+def sweep_bench(args):
+    """Zero-pi parameter sweep (parent dims [40, 40, 30], LOBPCG lowest 16, tol 1e-8):
+    plain FP64 point after point against batched solves with and without the
+    learned warm start.  Counted pairs have an FP64 true residual <= 1e-8."""
+    import numpy as np
+
+    from kfc import configs
+    from kfc.sweeps import WarmStart, ZeroPiSweep, subspace_capture
+
+    sw = ZeroPiSweep(contexts=args.contexts)
+    m = sw.m
+    test = [configs.zero_pi_theta(50_000 + i) for i in range(args.test_points)]
+    train = [configs.zero_pi_theta(i) for i in range(args.train_points)]
+
+    def counted(out):
+        return sum(int(np.sum(np.asarray(o[2]["max_residual"]) <= 1e-8)) * m for o in out)
+
+    # warm-up: one point through every path (compiles, allocates, tunes)
+    sw.run_plain(test[:1])
+    sw.run_batched(test[:2], mode=args.sweep_mode)
+    plain, t_plain = sw.run_plain(test[:args.plain_points])
+    rate_plain = counted(plain) / t_plain
+    log(f"[sweep] plain f64: {args.plain_points} points {t_plain:.2f} s -> {rate_plain:.1f} pairs/s, "
+        f"iters {[o[2]['iterations'] for o in plain]}")
+    t0 = time.perf_counter()
+    gen, t_gen = sw.run_batched(train, mode="f64")
+    frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
+    log(f"[sweep] training data: {len(train)} points ({len(train) * m} eigenpairs) in {t_gen:.1f} s")
+    ws = WarmStart(sw, seed=1)
+    t1 = time.perf_counter()
+    loss = ws.train(train, frames, steps=args.train_steps, lr=1e-3)
+    t_train = time.perf_counter() - t1
+    log(f"[sweep] network trained in {t_train:.1f} s, loss {loss[0]:.4g} -> {loss[-1]:.4g}")
+    res = {}
+    for name, mode, x0 in (("batched_f64", "f64", None), ("batched_mixed", "mixed", None),
+                           ("batched_f64_learned", "f64", ws.x0_fn), ("batched_mixed_learned", "mixed", ws.x0_fn)):
+        out, wall = sw.run_batched(test, mode=mode, x0_fn=x0)
+        res[name] = {"seconds": round(wall, 3), "eigenpairs_per_s": round(counted(out) / wall, 2),
+                     "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in out])), 1),
+                     "max_residual": max(o[2]["max_residual"] for o in out)}
+        log(f"[sweep] {name}: {wall:.2f} s, {res[name]['eigenpairs_per_s']} pairs/s, "
+            f"mean iters {res[name]['mean_iterations']}")
+        if name == "batched_f64":
+            ref = out
+    # agreement with the plain path and what the warm start captures
+    dE = max(float(np.max(np.abs(a[0] - b[0]) / np.abs(b[0]))) for a, b in zip(ref[:args.plain_points], plain))
+    Xf = ws.predict_frames(test[:8])
+    caps = []
+    for (E, V, st, kids), xf in zip(ref[:8], Xf):
+        X0 = sw.from_frame(xf, sw.frame_index(kids), V.shape[0])
+        caps.append(float(np.mean(subspace_capture(X0, V))))
+    best = max(res, key=lambda k: res[k]["eigenpairs_per_s"])
+    out = {"metric": "eigenpairs/s at resid<=1e-8 (zero-pi sweep, parent [40,40,30], LOBPCG lowest 16)",
+           "value": res["batched_mixed_learned"]["eigenpairs_per_s"], "unit": "eigenpairs/s", "n_gpus": 1,
+           "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * res["batched_mixed_learned"]["seconds"], 1),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "zero_pi_sweep", "dims": [40, 40, 30], "m": m, "tol": 1e-8,
+                      "test_points": args.test_points, "train_points": args.train_points,
+                      "contexts": args.contexts},
+           "plain_fp64": {"points": args.plain_points, "seconds": round(t_plain, 3),
+                          "eigenpairs_per_s": round(rate_plain, 2),
+                          "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in plain])), 1)},
+           **res,
+           "speedup_vs_plain": {k: round(v["eigenpairs_per_s"] / rate_plain, 2) for k, v in res.items()},
+           "best": best, "max_rel_dE_batched_vs_plain": dE,
+           "warm_start": {"train_seconds": round(t_train, 2), "data_seconds": round(t_gen, 2),
+                          "final_loss": float(loss[-1]), "mean_subspace_capture": round(float(np.mean(caps)), 4)}}
+    print(json.dumps(out))
+
+
+def transmon_bench(args):
+    """512 coupled-transmon instances on periodic grids ({256..768} points per
+    mode, complex leaves keeping 8, tops keeping 20 = 10 240 eigenpairs): plain
+    instance after instance against all leaves batched."""
+    import numpy as np
+
+    from kfc import transmon_sweep as T
+
+    insts = T.draw(args.instances, seed=args.seed)
+    sw = T.TransmonSweep(contexts=args.contexts)
+    sw.run_batched(insts[:2], mode=args.sweep_mode)      # warm-up
+    sw.run_plain(insts[:1])
+    plain, t_plain = sw.run_plain(insts[:args.plain_points])
+    n_plain = sum(len(t.energies) for t in plain
+                  if all(l.stats["max_residual"] <= 1e-8 for l in t.children))
+    rate_plain = n_plain / t_plain
+    log(f"[transmons] plain f64: {args.plain_points} instances {t_plain:.2f} s -> {rate_plain:.1f} pairs/s")
+    tops, wall, t_leaves = sw.run_batched(insts, mode=args.sweep_mode)
+    ok = [t for t in tops if all(l.stats["max_residual"] <= 1e-8 for l in t.children)]
+    value = sum(len(t.energies) for t in ok) / wall
+    log(f"[transmons] batched {args.sweep_mode}: {len(insts)} instances {wall:.2f} s (leaves {t_leaves:.2f} s) "
+        f"-> {value:.1f} pairs/s")
+    dE = max(float(np.max(np.abs(a.energies - b.energies) / np.abs(a.energies))) for a, b in zip(plain, tops))
+    out = {"metric": "eigenpairs/s at resid<=1e-8 (512 coupled transmons on periodic grids, tops keep 20)",
+           "value": round(value, 2), "unit": "eigenpairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64" if args.sweep_mode == "mixed" else "f64",
+           "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "transmon_sweep", "instances": len(insts), "grid_sizes": list(T.GRID_SIZES),
+                      "order": 8, "k_leaf": 8, "k_top": 20, "contexts": args.contexts, "mode": args.sweep_mode},
+           "plain_fp64": {"instances": args.plain_points, "seconds": round(t_plain, 3),
+                          "eigenpairs_per_s": round(rate_plain, 2)},
+           "speedup_vs_plain": round(value / rate_plain, 2), "seconds_leaves": round(t_leaves, 3),
+           "max_rel_dE_batched_vs_plain": dE}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +197,24 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
-    ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many restarts")
+    ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many cycles")
+    ap.add_argument("--workload", default="three_fluxonia",
+                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep"])
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--contexts", type=int, default=8, help="device contexts (streams) of a batched sweep")
+    ap.add_argument("--test-points", type=int, default=64)
+    ap.add_argument("--plain-points", type=int, default=8)
+    ap.add_argument("--train-points", type=int, default=256)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--sweep-mode", default="mixed", choices=["f64", "mixed"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU: this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "zero_pi_sweep":
+        return sweep_bench(args)
+    if args.workload == "transmon_sweep":
+        return transmon_bench(args)
     import numpy as np
     import torch
 

@@ -121,6 +121,13 @@ int kfc_op_destroy(kfc_op* op);
 int64_t kfc_op_size(const kfc_op* op);
 /* 1 when every application of op runs as one fused stencil pass */
 int kfc_op_fused(const kfc_op* op);
+/* Fused-stencil launch shape (no counterpart in the reference: the choice of
+ * the project's autotuner).  variant 0 = LDS tile, 1 = register queue,
+ * 2 = plane sweep (constant weights only); march = planes per block along the
+ * last axis.  Setting variant -1 returns the choice to the autotuner, which
+ * times the candidates on the first application of each column count. */
+int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march);
+int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march);
 
 /* Replaces LinearAlgebra.mul!(y, H, x) (src/apply.jl:416-432):
  * Y = H X for ncols columns with leading dimension ld (>= prod(dims)).
@@ -182,17 +189,6 @@ int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats);
 /* Fresh FP64 true residuals ||H v_j - E_j v_j|| of host pairs (E, V[N x m]):
  * _worst_true_residual per column (src/solvers.jl:65-69). */
 int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid);
-
-/* ---- batched sweeps (no reference counterpart; the closest is bench/common.jl:135-163).
- * A batch of independent two-mode instances of the coupled-transmon form on
- * periodic grids, solved in persistent launches.  n_inst instances; per instance i:
- * grid sizes N1[i], N2[i] and parameters P[i*8 .. i*8+7] =
- * (E_J1, E_C1, n_g1, E_J2, E_C2, n_g2, g, order).  Each leaf keeps k_leaf states,
- * the top keeps k_top: E_out[i*k_top ..], resid_out[i*k_top ..] (FP64 true
- * residuals of the top pairs). */
-int kfc_batch_transmons(kfc_ctx* ctx, int64_t n_inst, const int64_t* N1, const int64_t* N2, const double* P,
-                        int32_t k_leaf, int32_t k_top, double tol, int32_t mode, double* E_out,
-                        double* resid_out, kfc_stats* stats);
 
 /* ---- learned warm start (no reference counterpart; feeds X0 of kfc_lobpcg).
  * A 6 -> 256 -> 256 -> 1024 GELU MLP with BF16 weights and FP32 accumulation. */

@@ -635,6 +635,166 @@ __global__ __launch_bounds__(256) void k_stencil_q(const StArgs<S> a) {
     if (z + 2 < z1) emit(qc, qd, qa, z + 2);
 }
 
+// Plane-sweep variant for constant stencil weights: a block owns a 64 x 16 patch of
+// (axis 0, axis 1) and marches along axis 2.  Each plane of the patch (with a
+// one-point halo) is read from HBM once into LDS; every thread then takes the
+// plane's contributions to the outputs one plane below, at and one plane above
+// (dz = +1, 0, -1 weights) for four consecutive rows, keeping the three partial
+// sums in registers.  One global read per point instead of nine, one barrier per
+// plane, the next plane prefetched into registers during the current one.
+constexpr int SP_TX = 64, SP_TY = 16, SP_R = SP_TY / 4, SP_W = SP_TX + 2, SP_H = SP_TY + 2,
+              SP_TILE = SP_W * SP_H, SP_NL = (SP_TILE + 255) / 256;
+
+template <class S, bool DP>
+__global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
+    __shared__ S tile[2][SP_TILE];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nblk = gridDim.x;
+    const int per = nblk >> 3;
+    const int lg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lg >= a.nlog) return;   // padding blocks leave before any barrier
+    int rem = lg;
+    const int b0 = rem % a.nb0; rem /= a.nb0;
+    const int b1 = rem % a.nb1; rem /= a.nb1;
+    const int bz = rem % a.nbz; rem /= a.nbz;
+    const int col = rem;
+    const int x0 = b0 * SP_TX, y0 = b1 * SP_TY;
+    const int i0 = x0 + lane;
+    const int r0 = wv * SP_R;   // first patch row of this thread
+    const S one = s_one<S>(), zero = s_zero(one);
+    const S* __restrict__ Xc = a.X + (int64_t)col * a.ld;
+    S* __restrict__ Yc = a.Y + (int64_t)col * a.ld;
+    const S* __restrict__ Ac = a.A ? a.A + (int64_t)col * a.ld : nullptr;
+    // this thread's share of a patch plane (with halo)
+    // offsets inside one plane (a plane holds fewer than 2^31 points)
+    int goff[SP_NL];
+    bool gok[SP_NL];
+#pragma unroll
+    for (int j = 0; j < SP_NL; ++j) {
+        const int e = tid + 256 * j;
+        const int r = e / SP_W, c = e - r * SP_W;
+        const int gx = x0 - 1 + c, gy = y0 - 1 + r;
+        gok[j] = e < SP_TILE && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1;
+        goff[j] = gok[j] ? gx + gy * int(a.s1) : 0;
+    }
+    const int z0 = bz * a.L, z1 = min(z0 + a.L, a.d2);
+    S pre[SP_NL];
+    auto fetch = [&](int z) {
+        const bool zin = z >= 0 && z < a.d2;
+        const S* __restrict__ Xz = Xc + (int64_t)z * a.s2;
+#pragma unroll
+        for (int j = 0; j < SP_NL; ++j) pre[j] = (zin && gok[j]) ? Xz[goff[j]] : zero;
+    };
+    auto stash = [&](S* t) {
+#pragma unroll
+        for (int j = 0; j < SP_NL; ++j)
+            if (tid + 256 * j < SP_TILE) t[tid + 256 * j] = pre[j];
+    };
+    // per-row diagonal pieces that do not depend on axis 2
+    constexpr int NDP = DP ? 4 : 1;   // products of per-axis diagonals (none in most operators)
+    S d01[SP_R];
+    S dp01[NDP][SP_R];
+    const bool xin = i0 < a.d0;
+#pragma unroll
+    for (int r = 0; r < SP_R; ++r) {
+        const int y = y0 + r0 + r;
+        const bool in = xin && y < a.d1;
+        S d = zero;
+        if (in && a.D0) d = s_add(d, a.D0[i0]);
+        if (in && a.D1) d = s_add(d, a.D1[y]);
+        d01[r] = d;
+#pragma unroll
+        for (int k = 0; k < NDP; ++k) {
+            S x = zero;
+            if (DP && in && k < a.ndp) {
+                x = a.dpc[k];
+                if (a.dp0[k]) x = s_mul(x, a.dp0[k][i0]);
+                if (a.dp1[k]) x = s_mul(x, a.dp1[k][y]);
+            }
+            dp01[k][r] = x;
+        }
+    }
+    S accM[SP_R], acc0[SP_R], accP[SP_R], ctrM[SP_R], ctr0[SP_R];
+#pragma unroll
+    for (int r = 0; r < SP_R; ++r) accM[r] = acc0[r] = accP[r] = ctrM[r] = ctr0[r] = zero;
+    int buf = 0;
+    fetch(z0 - 1);
+    stash(tile[0]);
+    __syncthreads();
+    for (int zp = z0 - 1; zp <= z1; ++zp) {
+        if (zp + 1 <= z1) fetch(zp + 1);
+        const S* t = tile[buf];
+        // rows r0 .. r0 + SP_R + 1 of the patch, columns lane .. lane + 2
+        S v[SP_R + 2][3];
+#pragma unroll
+        for (int rr = 0; rr < SP_R + 2; ++rr)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) v[rr][dx] = t[(r0 + rr) * SP_W + lane + dx];
+        // plane zp feeds output zp + 1 (dz = 0 weights), zp (dz = 1) and zp - 1 (dz = 2)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int dy = q / 3, dx = q % 3;
+            if ((a.used >> q) & 1u) {
+                const S w = a.c[q];
+#pragma unroll
+                for (int r = 0; r < SP_R; ++r) accP[r] = s_fma(accP[r], w, v[r + dy][dx]);
+            }
+            if ((a.used >> (9 + q)) & 1u) {
+                const S w = a.c[9 + q];
+#pragma unroll
+                for (int r = 0; r < SP_R; ++r) acc0[r] = s_fma(acc0[r], w, v[r + dy][dx]);
+            }
+            if ((a.used >> (18 + q)) & 1u) {
+                const S w = a.c[18 + q];
+#pragma unroll
+                for (int r = 0; r < SP_R; ++r) accM[r] = s_fma(accM[r], w, v[r + dy][dx]);
+            }
+        }
+        // the diagonal of output zp
+        if (zp >= z0 && zp < z1) {
+            const S d2 = a.D2 ? a.D2[zp] : zero;
+            S dp2[NDP];
+#pragma unroll
+            for (int k = 0; k < NDP; ++k) dp2[k] = (DP && k < a.ndp) ? (a.dp2[k] ? a.dp2[k][zp] : one) : zero;
+#pragma unroll
+            for (int r = 0; r < SP_R; ++r) {
+                S dv = s_add(d01[r], d2);
+#pragma unroll
+                for (int k = 0; k < NDP; ++k)
+                    if (DP && k < a.ndp) dv = s_fma(dv, dp01[k][r], dp2[k]);
+                const int y = y0 + r0 + r;
+                if (a.Dfull && xin && y < a.d1) dv = s_add(dv, a.Dfull[(int64_t)i0 + (int64_t)y * a.s1 + (int64_t)zp * a.s2]);
+                ctr0[r] = v[r + 1][1];
+                acc0[r] = s_fma(acc0[r], dv, ctr0[r]);
+            }
+        }
+        // output zp - 1 is complete
+        if (zp - 1 >= z0 && xin) {
+            const int64_t zo = (int64_t)(zp - 1) * a.s2;
+#pragma unroll
+            for (int r = 0; r < SP_R; ++r) {
+                const int y = y0 + r0 + r;
+                if (y < a.d1) {
+                    const int64_t idx = (int64_t)i0 + (int64_t)y * a.s1 + zo;
+                    S out = s_fma(s_mul(a.alpha, accM[r]), a.beta, ctrM[r]);
+                    if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
+                    Yc[idx] = out;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < SP_R; ++r) {
+            accM[r] = acc0[r];
+            acc0[r] = accP[r];
+            accP[r] = zero;
+            ctrM[r] = ctr0[r];
+        }
+        if (zp + 1 <= z1) stash(tile[buf ^ 1]);
+        __syncthreads();
+        buf ^= 1;
+    }
+}
+
 template <class S>
 static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld,
                           cd alpha, cd beta, cd gamma, hipStream_t st) {
@@ -674,11 +834,16 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     a.alpha = s_from<S>(alpha);
     a.beta = s_from<S>(beta);
     a.gamma = s_from<S>(gamma);
+    bool cw = true;
+    for (int p = 0; p < 27; ++p) cw &= !(a.u[p] || a.v[p] || a.w[p]);
+    const int variant = (op->stencil_variant == 2 && !cw) ? 1 : op->stencil_variant;
+    const int ty = variant == 2 ? SP_TY : ST_TY;
     a.nb0 = int((a.d0 + ST_TX - 1) / ST_TX);
-    a.nb1 = int((a.d1 + ST_TY - 1) / ST_TY);
+    a.nb1 = int((a.d1 + ty - 1) / ty);
     a.ncols = int(ncols);
     int L = op->stencil_march > 0 ? op->stencil_march : 64;
-    while (L > 8 && (int64_t)a.nb0 * a.nb1 * ((a.d2 + L - 1) / L) * ncols < 8192) L >>= 1;
+    const int64_t min_blocks = variant == 2 ? 2048 : 8192;   // the plane sweep wants long marches
+    while (L > 8 && (int64_t)a.nb0 * a.nb1 * ((a.d2 + L - 1) / L) * ncols < min_blocks) L >>= 1;
     L = std::min(L, a.d2);
     a.L = std::max(L, 1);
     a.nbz = (a.d2 + a.L - 1) / a.L;
@@ -686,10 +851,10 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     if (nlog >= (1ll << 31) - 8) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "apply: grid too large");
     a.nlog = int(nlog);
     const int64_t nblk = (nlog + 7) / 8 * 8;
-    bool cw = true;
-    for (int p = 0; p < 27; ++p) cw &= !(a.u[p] || a.v[p] || a.w[p]);
-    const int variant = op->stencil_variant;
-    if (variant == 0) {
+    if (variant == 2) {
+        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_p<S, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else if (variant == 0) {
         hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (cw) {
         hipLaunchKernelGGL((k_stencil_q<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
@@ -833,7 +998,7 @@ static int tune_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64
         op->stencil_variant = atoi(env);
         return KFC_OK;
     }
-    const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}};
+    const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}, {2, 16}, {2, 32}, {2, 64}, {2, 128}};
     hipEvent_t e0, e1;
     KFC_HIP(op->ctx, hipEventCreate(&e0));
     KFC_HIP(op->ctx, hipEventCreate(&e1));
@@ -872,7 +1037,7 @@ static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t nc
                    cd gamma, hipStream_t st) {
     if (op->plan.ok) {
         const int prec = std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1;
-        if (op->tuned_cols[prec] != ncols) {
+        if (!op->stencil_pinned && op->tuned_cols[prec] != ncols) {
             int rc = tune_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
             if (rc != KFC_OK) return rc;
             op->tuned_cols[prec] = ncols;

@@ -14,6 +14,7 @@ struct Work {
     DevBuf part;      // reduction partials
     DevBuf small;     // small matrices / vectors uploaded to the device
     DevBuf small2;
+    DevBuf tmp;       // N x q scratch of the general tall-skinny product
     std::vector<unsigned char> host;
 };
 
@@ -24,6 +25,12 @@ int blas_gram(Work& w, const S* A, int64_t lda, int p, const S* B, int64_t ldb, 
 template <class S>
 int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M, int q, S* C, int64_t ldc, int64_t N,
               double beta);
+// Y[:, 0:q] = X[:, 0:p] M (M p x q host, column-major), Y may be columns of X (each row
+// of X is read before that row of Y is written).  gram = 1 also returns G = X[:, 0:pz]^H Y
+// (pz x q), gram = 2 returns G = Y^H Y (q x q), over all N rows of the result.
+template <class S>
+int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M, int q, S* Y, int64_t ldy, int64_t N,
+              int gram, int pz, std::vector<cd>* G);
 // Column 2-norms of A[:, 0:p] (host).
 template <class S>
 int blas_colnorms(Work& w, const S* A, int64_t lda, int p, int64_t N, std::vector<double>& nrm);

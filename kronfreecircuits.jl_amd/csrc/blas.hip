@@ -92,23 +92,261 @@ __global__ __launch_bounds__(256) void k_gram(const S* __restrict__ Am, int64_t 
         }
         __syncthreads();
     }
-    A* out = part + ((int64_t)chunk * gridDim.y * gridDim.z + (int64_t)pt * gridDim.z + qt) * 4096;
+    // this chunk's partial of the p x q product, column-major
+    A* out = part + (int64_t)chunk * p * q;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) out[(ti * 4 + i) + (tj * 4 + j) * 64] = acc[i][j];
+        for (int j = 0; j < 4; ++j) {
+            const int gi = pt * 64 + ti * 4 + i, gj = qt * 64 + tj * 4 + j;
+            if (gi < p && gj < q) out[gi + (int64_t)gj * p] = acc[i][j];
+        }
 }
 
+// G[e] = sum over chunks of part[c][e]: one block per entry, threads stride the chunks
 template <class A>
-__global__ void k_gram_reduce(const A* __restrict__ part, int nchunk, int ntp, int ntq, int p, int q, A* __restrict__ G) {
-    const int64_t total = (int64_t)p * q;
-    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int i = int(x % p), j = int(x / p);
-        const int pt = i >> 6, qt = j >> 6, ii = i & 63, jj = j & 63;
-        A s = acc_zero<A>();
-        for (int c = 0; c < nchunk; ++c) s = s_add(s, part[((int64_t)c * ntp * ntq + pt * ntq + qt) * 4096 + ii + jj * 64]);
-        G[i + (int64_t)j * p] = s;
+__global__ __launch_bounds__(256) void k_gram_reduce(const A* __restrict__ part, int nchunk, int64_t pq,
+                                                     A* __restrict__ G) {
+    __shared__ A red[256];
+    const int64_t e = blockIdx.x;
+    A s = acc_zero<A>();
+    for (int c = threadIdx.x; c < nchunk; c += 256) s = s_add(s, part[(int64_t)c * pq + e]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = s_add(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
     }
+    if (threadIdx.x == 0) G[e] = red[0];
+}
+
+// Gram of real blocks on the matrix cores: G = A^T B for p, q <= 64.  Each
+// 64-row slab of A and B is staged in LDS as FP64 (one coalesced 512-byte load
+// per column and wave) and contracted with v_mfma_f64_16x16x4f64 on TP x TQ
+// output tiles of 16 x 16; the four waves take alternate 4-row k-steps of the
+// slab and their tiles are summed in LDS at the end.  FP64 accumulation for
+// FP32 blocks too.
+typedef double gd4 __attribute__((ext_vector_type(4)));
+
+template <class S, int TP, int TQ>
+__global__ __launch_bounds__(256) void k_gram_mfma(const S* __restrict__ Am, int64_t lda, int p,
+                                                   const S* __restrict__ Bm, int64_t ldb, int q, int64_t N,
+                                                   int64_t rows_per, double* __restrict__ part) {
+    constexpr int LDR = 65;   // odd row stride (in doubles) against bank conflicts
+    __shared__ double smem[(TP + TQ) * 16 * LDR];
+    double* sA = smem;
+    double* sB = smem + TP * 16 * LDR;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t r0 = blockIdx.x * rows_per, r1 = min(N, r0 + rows_per);
+    gd4 acc[TP][TQ];
+#pragma unroll
+    for (int i = 0; i < TP; ++i)
+#pragma unroll
+        for (int j = 0; j < TQ; ++j) acc[i][j] = gd4{0, 0, 0, 0};
+    for (int64_t rb = r0; rb < r1; rb += 64) {
+        const int64_t row = rb + lane;
+        const bool rok = row < r1;
+        __syncthreads();   // the previous slab has been consumed
+#pragma unroll
+        for (int c = wv; c < TP * 16; c += 4) sA[c * LDR + lane] = (rok && c < p) ? double(Am[row + c * lda]) : 0.0;
+#pragma unroll
+        for (int c = wv; c < TQ * 16; c += 4) sB[c * LDR + lane] = (rok && c < q) ? double(Bm[row + c * ldb]) : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int rr = 4 * (wv + 4 * u) + (lane >> 4);   // slab row of this lane's k index
+            double a[TP], b[TQ];
+#pragma unroll
+            for (int i = 0; i < TP; ++i) a[i] = sA[(16 * i + (lane & 15)) * LDR + rr];
+#pragma unroll
+            for (int j = 0; j < TQ; ++j) b[j] = sB[(16 * j + (lane & 15)) * LDR + rr];
+#pragma unroll
+            for (int i = 0; i < TP; ++i)
+#pragma unroll
+                for (int j = 0; j < TQ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    // sum the four waves' tiles in LDS: red[tile][row][col]
+    double* red = smem;
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int i = 0; i < TP; ++i)
+#pragma unroll
+                for (int j = 0; j < TQ; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        // f64 accumulator layout: register r of lane l holds row l/16 + 4r, column l%16
+                        const int e = (i * TQ + j) * 256 + ((lane >> 4) + 4 * r) * 16 + (lane & 15);
+                        red[e] = (w == 0 ? 0.0 : red[e]) + acc[i][j][r];
+                    }
+        }
+        __syncthreads();
+    }
+    double* out = part + (int64_t)blockIdx.x * p * q;
+    for (int e = tid; e < TP * TQ * 256; e += 256) {
+        const int tile = e >> 8, rc = e & 255, i = tile / TQ, j = tile % TQ;
+        const int gi = 16 * i + (rc >> 4), gj = 16 * j + (rc & 15);
+        if (gi < p && gj < q) out[gi + (int64_t)gj * p] = red[e];
+    }
+}
+
+template <class S, int TP, int TQ>
+void launch_gram_mfma(int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q,
+                      int64_t N, int64_t rows_per, double* part) {
+    hipLaunchKernelGGL((k_gram_mfma<S, TP, TQ>), dim3(nchunk), dim3(256), 0, st, A, lda, p, B, ldb, q, N, rows_per,
+                       part);
+}
+
+template <class S, int TP>
+void gram_mfma_q(int tq, int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q,
+                 int64_t N, int64_t rows_per, double* part) {
+    switch (tq) {
+        case 1: launch_gram_mfma<S, TP, 1>(nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+        case 2: launch_gram_mfma<S, TP, 2>(nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+        case 3: launch_gram_mfma<S, TP, 3>(nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+        default: launch_gram_mfma<S, TP, 4>(nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+    }
+}
+
+template <class S>
+void gram_mfma(int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q, int64_t N,
+               int64_t rows_per, double* part) {
+    const int tp = (p + 15) / 16, tq = (q + 15) / 16;
+    switch (tp) {
+        case 1: gram_mfma_q<S, 1>(tq, nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+        case 2: gram_mfma_q<S, 2>(tq, nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+        case 3: gram_mfma_q<S, 3>(tq, nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+        default: gram_mfma_q<S, 4>(tq, nchunk, st, A, lda, p, B, ldb, q, N, rows_per, part); break;
+    }
+}
+
+// Tall-skinny product on the matrix cores, with an optional Gram of the result:
+// Y[:, 0:q] = X[:, 0:p] M for real blocks, p <= 80, q <= 64.  A block stages a
+// 64-row slab of X in LDS (as FP64), each wave forms the 16 x q rows of Y of its
+// quarter as Y^T = M^T X^T with v_mfma_f64_16x16x4f64 (so each store writes 16
+// consecutive rows of a column), and with GR = 1 or 2 also contracts the slab's
+// rows into G = X[:, 0:pz]^T Y or Y^T Y (q <= 16) while the slab is in LDS.  Y may
+// be columns of X: a slab is read completely before any of its rows is written.
+constexpr int TS_LDR = 65;
+
+template <class S, int TQ, int GR, int TPZ>
+__global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t ldx, int p, int ppad,
+                                              const double* __restrict__ M, int q, S* Y, int64_t ldy, int64_t N,
+                                              int64_t rows_per, int pz, double* __restrict__ part) {
+    constexpr int QP = 16 * TQ;
+    extern __shared__ double sm[];
+    double* sX = sm;                      // ppad columns x 64 rows (stride TS_LDR)
+    double* sM = sX + ppad * TS_LDR;      // ppad x QP, row k at sM[k * QP]
+    double* sY = sM + ppad * QP;          // 16 columns x 64 rows (GR != 0)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lo = lane & 15, hi = lane >> 4;
+    for (int e = tid; e < ppad * QP; e += 256) {
+        const int k = e / QP, j = e - k * QP;
+        sM[e] = (k < p && j < q) ? M[k + (int64_t)j * p] : 0.0;
+    }
+    const int64_t r0 = blockIdx.x * rows_per, r1 = min(N, r0 + rows_per);
+    gd4 g[TPZ];
+#pragma unroll
+    for (int i = 0; i < TPZ; ++i) g[i] = gd4{0, 0, 0, 0};
+    for (int64_t rb = r0; rb < r1; rb += 64) {
+        const int64_t row = rb + lane;
+        const bool rok = row < r1;
+        __syncthreads();   // the previous slab's readers are done (and sM is in place)
+        for (int c = wv; c < ppad; c += 4) sX[c * TS_LDR + lane] = (rok && c < p) ? double(X[row + c * ldx]) : 0.0;
+        __syncthreads();
+        gd4 acc[TQ];
+#pragma unroll
+        for (int j = 0; j < TQ; ++j) acc[j] = gd4{0, 0, 0, 0};
+        for (int t = 0; t < ppad; t += 4) {
+            const int k = t + hi;
+            const double b = sX[k * TS_LDR + 16 * wv + lo];
+#pragma unroll
+            for (int j = 0; j < TQ; ++j)
+                acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[k * QP + 16 * j + lo], b, acc[j], 0, 0, 0);
+        }
+        // register r of lane l holds Y[16 wv + l % 16][16 j + l / 16 + 4 r]
+        const int64_t yrow = rb + 16 * wv + lo;
+#pragma unroll
+        for (int j = 0; j < TQ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * j + hi + 4 * r;
+                if (yrow < r1 && col < q) Y[yrow + col * ldy] = S(acc[j][r]);
+                if (GR != 0 && j == 0) sY[col * TS_LDR + 16 * wv + lo] = (yrow < r1 && col < q) ? acc[0][r] : 0.0;
+            }
+        if (GR != 0) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int rr = 4 * (wv + 4 * u) + hi;
+                const double b = sY[lo * TS_LDR + rr];
+                if (GR == 1) {
+#pragma unroll
+                    for (int z = 0; z < TPZ; ++z)
+                        g[z] = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[(16 * z + lo) * TS_LDR + rr], b, g[z], 0, 0, 0);
+                } else {
+                    g[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b, b, g[0], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (GR == 0) return;
+    // sum the waves' Gram tiles in LDS
+    double* red = sm;
+    const int gp = GR == 1 ? pz : q;
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int z = 0; z < TPZ; ++z)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = z * 256 + (hi + 4 * r) * 16 + lo;
+                    red[e] = (w == 0 ? 0.0 : red[e]) + g[z][r];
+                }
+        }
+        __syncthreads();
+    }
+    double* out = part + (int64_t)blockIdx.x * gp * q;
+    for (int e = tid; e < TPZ * 256; e += 256) {
+        const int z = e >> 8, rc = e & 255;
+        const int gi = 16 * z + (rc >> 4), gj = rc & 15;
+        if (gi < gp && gj < q) out[gi + (int64_t)gj * gp] = red[e];
+    }
+}
+
+template <class S, int TQ, int GR, int TPZ>
+void launch_tsmm(int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p, int ppad, const double* M,
+                 int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz, double* part) {
+    hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N,
+                       rows_per, pz, part);
+}
+
+template <class S>
+void tsmm_dispatch(int gr, int tq, int tpz, int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p,
+                   int ppad, const double* M, int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz,
+                   double* part) {
+#define KFC_TS(TQv, GRv, TPZv) \
+    launch_tsmm<S, TQv, GRv, TPZv>(nchunk, shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N, rows_per, pz, part)
+    if (gr == 0) {
+        switch (tq) {
+            case 1: KFC_TS(1, 0, 1); break;
+            case 2: KFC_TS(2, 0, 1); break;
+            case 3: KFC_TS(3, 0, 1); break;
+            default: KFC_TS(4, 0, 1); break;
+        }
+    } else if (gr == 2) {
+        KFC_TS(1, 2, 1);
+    } else {
+        switch (tpz) {
+            case 1: KFC_TS(1, 1, 1); break;
+            case 2: KFC_TS(1, 1, 2); break;
+            case 3: KFC_TS(1, 1, 3); break;
+            default: KFC_TS(1, 1, 4); break;
+        }
+    }
+#undef KFC_TS
 }
 
 template <class S>
@@ -117,14 +355,23 @@ int blas_gram(Work& w, const S* A, int64_t lda, int p, const S* B, int64_t ldb, 
     G.assign(size_t(p) * q, cd(0));
     if (p == 0 || q == 0) return KFC_OK;
     const int ntp = (p + 63) / 64, ntq = (q + 63) / 64;
-    const int nchunk = nchunks_for(N, 8192, 256);
+    // enough row chunks to fill the chip several times over
+    const int nchunk = nchunks_for(N, 4096, std::max(64, 2048 / (ntp * ntq)));
     const int64_t rows_per = (N + nchunk - 1) / nchunk;
-    KFC_HIP(w.ctx, w.part.ensure(sizeof(Acc) * 4096 * size_t(nchunk) * ntp * ntq));
+    KFC_HIP(w.ctx, w.part.ensure(sizeof(Acc) * size_t(nchunk) * p * q));
     KFC_HIP(w.ctx, w.small.ensure(sizeof(Acc) * size_t(p) * q));
-    hipLaunchKernelGGL((k_gram<S, Acc>), dim3(nchunk, ntp, ntq), dim3(256), 0, w.ctx->stream, A, lda, p, B, ldb, q, N,
-                       rows_per, w.part.as<Acc>());
-    hipLaunchKernelGGL(k_gram_reduce<Acc>, dim3(grid1(int64_t(p) * q)), dim3(256), 0, w.ctx->stream,
-                       w.part.as<Acc>(), nchunk, ntp, ntq, p, q, w.small.as<Acc>());
+    if constexpr (!IsCx<S>::value) {
+        if (p <= 64 && q <= 64)
+            gram_mfma<S>(nchunk, w.ctx->stream, A, lda, p, B, ldb, q, N, rows_per, w.part.as<double>());
+        else
+            hipLaunchKernelGGL((k_gram<S, Acc>), dim3(nchunk, ntp, ntq), dim3(256), 0, w.ctx->stream, A, lda, p, B, ldb,
+                               q, N, rows_per, w.part.as<Acc>());
+    } else {
+        hipLaunchKernelGGL((k_gram<S, Acc>), dim3(nchunk, ntp, ntq), dim3(256), 0, w.ctx->stream, A, lda, p, B, ldb, q,
+                           N, rows_per, w.part.as<Acc>());
+    }
+    hipLaunchKernelGGL(k_gram_reduce<Acc>, dim3(unsigned(int64_t(p) * q)), dim3(256), 0, w.ctx->stream,
+                       w.part.as<Acc>(), nchunk, int64_t(p) * q, w.small.as<Acc>());
     KFC_HIP(w.ctx, hipGetLastError());
     std::vector<Acc> h(size_t(p) * q);
     KFC_HIP(w.ctx, hipMemcpyAsync(h.data(), w.small.p, h.size() * sizeof(Acc), hipMemcpyDeviceToHost, w.ctx->stream));
@@ -188,6 +435,59 @@ int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M,
     KFC_HIP(w.ctx, hipGetLastError());
     // the host staging vector dies here: wait for the copy to finish
     KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));
+    return KFC_OK;
+}
+
+template <class S>
+int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M, int q, S* Y, int64_t ldy, int64_t N,
+              int gram, int pz, std::vector<cd>* G) {
+    if (q == 0 || N == 0) {
+        if (G) G->assign(size_t(gram == 1 ? pz : q) * q, cd(0));
+        return KFC_OK;
+    }
+    bool mfma = !IsCx<S>::value && p <= 80 && q <= 64 && (gram == 0 || (q <= 16 && pz <= 64 && pz <= p));
+    if (!mfma) {
+        // general path: product into scratch, copy, then the Gram
+        KFC_HIP(w.ctx, w.tmp.ensure(sizeof(S) * size_t(N) * q));
+        int rc = blas_gemm<S>(w, X, ldx, p, M, q, w.tmp.as<S>(), N, N, 0.0);
+        if (rc == KFC_OK) rc = blas_axpby<S>(w, w.tmp.as<S>(), N, Y, ldy, q, N, 1.0, 0.0);
+        if (rc == KFC_OK && gram == 1) rc = blas_gram<S>(w, X, ldx, pz, Y, ldy, q, N, *G);
+        if (rc == KFC_OK && gram == 2) rc = blas_gram<S>(w, Y, ldy, q, Y, ldy, q, N, *G);
+        return rc;
+    }
+    if constexpr (!IsCx<S>::value) {
+        const int tq = (q + 15) / 16, ppad = (p + 3) / 4 * 4, tpz = (pz + 15) / 16;
+        const int gp = gram == 1 ? pz : q;
+        const int nchunk = nchunks_for(N, 4096, 2048);
+        const int64_t rows_per = (N + nchunk - 1) / nchunk;
+        std::vector<double> h(size_t(p) * q);
+        for (size_t i = 0; i < h.size(); ++i) h[i] = M[i].real();
+        KFC_HIP(w.ctx, w.small2.ensure(sizeof(double) * h.size()));
+        KFC_HIP(w.ctx, hipMemcpyAsync(w.small2.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                                      w.ctx->stream));
+        if (gram) {
+            KFC_HIP(w.ctx, w.part.ensure(sizeof(double) * size_t(nchunk) * gp * q));
+            KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(gp) * q));
+        }
+        size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 * tq + (gram ? 16 * TS_LDR : 0);
+        words = std::max<size_t>(words, size_t(gram == 1 ? tpz : 1) * 256);
+        tsmm_dispatch<S>(gram, tq, tpz, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
+                         w.small2.as<double>(), q, Y, ldy, N, rows_per, pz, gram ? w.part.as<double>() : nullptr);
+        KFC_HIP(w.ctx, hipGetLastError());
+        if (gram) {
+            hipLaunchKernelGGL(k_gram_reduce<double>, dim3(unsigned(gp * q)), dim3(256), 0, w.ctx->stream,
+                               w.part.as<double>(), nchunk, int64_t(gp) * q, w.small.as<double>());
+            KFC_HIP(w.ctx, hipGetLastError());
+            std::vector<double> hg(size_t(gp) * q);
+            KFC_HIP(w.ctx, hipMemcpyAsync(hg.data(), w.small.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                          w.ctx->stream));
+            KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));
+            G->resize(hg.size());
+            for (size_t i = 0; i < hg.size(); ++i) (*G)[i] = cd(hg[i], 0.0);
+        } else {
+            KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));   // the staged M dies with this call
+        }
+    }
     return KFC_OK;
 }
 
@@ -600,6 +900,8 @@ int lz_update(Work& w, const S* V, int64_t ldv, int k, const std::vector<cd>& hs
     template int blas_gram<S>(Work&, const S*, int64_t, int, const S*, int64_t, int, int64_t, std::vector<cd>&);     \
     template int blas_gemm<S>(Work&, const S*, int64_t, int, const std::vector<cd>&, int, S*, int64_t, int64_t,      \
                               double);                                                                              \
+    template int blas_tsmm<S>(Work&, const S*, int64_t, int, const std::vector<cd>&, int, S*, int64_t, int64_t, int, \
+                              int, std::vector<cd>*);                                                               \
     template int blas_colnorms<S>(Work&, const S*, int64_t, int, int64_t, std::vector<double>&);                    \
     template int blas_scale_cols<S>(Work&, S*, int64_t, int, int64_t, const std::vector<double>&);                  \
     template int blas_residual<S>(Work&, S*, int64_t, const S*, int64_t, const S*, int64_t, int, int64_t,            \

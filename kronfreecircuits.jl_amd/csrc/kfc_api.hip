@@ -165,6 +165,28 @@ int kfc_op_destroy(kfc_op* op) {
 int64_t kfc_op_size(const kfc_op* op) { return op ? op->N : 0; }
 int kfc_op_fused(const kfc_op* op) { return op && op->plan.ok ? 1 : 0; }
 
+int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march) {
+    if (!op || !variant || !march) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_op_stencil_get: null argument");
+    *variant = op->stencil_variant;
+    *march = op->stencil_march;
+    return KFC_OK;
+}
+
+int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march) {
+    if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_op_stencil_set: null operator");
+    if (variant < -1 || variant > 2) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: variant must lie in -1:2");
+    if (march < 0 || march > 4096) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: march must lie in 0:4096");
+    if (variant < 0) {
+        op->tuned_cols[0] = op->tuned_cols[1] = -1;
+        op->stencil_pinned = false;
+        return KFC_OK;
+    }
+    op->stencil_variant = variant;
+    op->stencil_march = march;
+    op->stencil_pinned = true;
+    return KFC_OK;
+}
+
 int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision) {
     if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_apply: null operator");
     if (precision != KFC_PREC_F64 && precision != KFC_PREC_F32)
@@ -249,9 +271,9 @@ extern "C" int kfc_op_gershgorin(kfc_op* op, double* out) {
             for (const Factor& f : t.factors) p *= infnorm_of(f);
             off += p;
         }
-        double dmax, imax = 0;
+        double dmax, dmin, imax = 0;
         if (!op->cplx && separable_diag(op)) {
-            // max over a sum of per-axis vectors is the sum of the per-axis maxima
+            // extrema over a sum of per-axis vectors are the sums of the per-axis extrema
             std::vector<std::vector<double>> ax(op->ndims);
             for (int a = 0; a < op->ndims; ++a) ax[a].assign(op->dims[a], 0.0);
             double c = 0;
@@ -260,21 +282,34 @@ extern "C" int kfc_op_gershgorin(kfc_op* op, double* out) {
                 const Factor& f = t.factors[0];
                 for (int64_t i = 0; i < f.d; ++i) ax[f.axis][i] += (t.coeff * f.diag[i]).real();
             }
-            dmax = c;
-            for (auto& v : ax) dmax += *std::max_element(v.begin(), v.end());
+            dmax = dmin = c;
+            for (auto& v : ax) {
+                dmax += *std::max_element(v.begin(), v.end());
+                dmin += *std::min_element(v.begin(), v.end());
+            }
         } else {
             op_host_diag(op);
             dmax = -INFINITY;
+            dmin = INFINITY;
             for (const cd& x : op->hdiag) {
                 dmax = std::max(dmax, x.real());
+                dmin = std::min(dmin, x.real());
                 imax = std::max(imax, std::fabs(x.imag()));
             }
         }
+        op->gershgorin_lo = dmin - imax - off;
         op->gershgorin = dmax + imax + off;
         op->have_gershgorin = true;
     }
     *out = op->gershgorin;
     return KFC_OK;
+}
+
+int op_spectral_interval(kfc_op* op, double* lo, double* hi) {
+    // Gershgorin discs: every eigenvalue lies in [min d_i - r, max d_i + r]
+    int rc = kfc_op_gershgorin(op, hi);
+    if (rc == KFC_OK) *lo = op->gershgorin_lo;
+    return rc;
 }
 
 int op_full_diagonal(kfc_op* op, std::vector<double>& D) {

@@ -199,12 +199,17 @@ struct kfc_op {
     StencilPlan plan;
     int stencil_variant = 1;        // 0: LDS plane tile, 1: register queue (chosen by the tuner)
     int stencil_march = 0;          // march chunk along the slowest axis (0: default)
+    bool stencil_pinned = false;    // launch shape set by the caller: no autotuning
     int64_t tuned_cols[2] = {-1, -1};   // column count the stencil variant was tuned for, per precision
     PairPlan pairs;
     DeviceData dd[2];               // [KFC_PREC_F64], [KFC_PREC_F32]
     double gershgorin = 0;
+    double gershgorin_lo = 0;       // lower end of the Gershgorin interval
     bool have_gershgorin = false;
 };
+
+// Gershgorin interval [lo, hi] holding the spectrum (kfc_api.hip)
+int op_spectral_interval(kfc_op* op, double* lo, double* hi);
 
 // Builders and launches (apply.hip)
 int op_build_plan(kfc_op* op);

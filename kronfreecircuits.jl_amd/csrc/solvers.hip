@@ -77,6 +77,9 @@ struct Timer {
     double apply = 0, apply_gpu = 0;
     int64_t cols = 0, cols32 = 0, calls = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // event pairs of applies not yet read back (read at the next flush, so that an
+    // apply inside a Krylov block does not stall the host)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
     Timer() {
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
@@ -84,9 +87,41 @@ struct Timer {
     ~Timer() {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        for (auto& pr : pending) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        for (auto& pr : pool) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
     }
     Timer(const Timer&) = delete;
     Timer& operator=(const Timer&) = delete;
+    hipError_t pair(std::pair<hipEvent_t, hipEvent_t>& pr) {
+        if (!pool.empty()) {
+            pr = pool.back();
+            pool.pop_back();
+            return hipSuccess;
+        }
+        hipError_t e = hipEventCreate(&pr.first);
+        if (e == hipSuccess) e = hipEventCreate(&pr.second);
+        return e;
+    }
+    // add the device time of every pending apply (waits for the last one)
+    hipError_t flush() {
+        for (auto& pr : pending) {
+            hipError_t e = hipEventSynchronize(pr.second);
+            if (e != hipSuccess) return e;
+            float ms = 0;
+            e = hipEventElapsedTime(&ms, pr.first, pr.second);
+            if (e != hipSuccess) return e;
+            apply_gpu += 1e-3 * ms;
+            pool.push_back(pr);
+        }
+        pending.clear();
+        return hipSuccess;
+    }
 };
 
 template <class S>
@@ -167,129 +202,410 @@ int dense_t(kfc_op* op, int64_t m, double* Eout, double* Vout, kfc_stats* st) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Thick-restart Lanczos (Wu & Simon's restart with full reorthogonalization).
-// The basis columns are stored unnormalized with a host scale s_i, so that the
-// normalization never costs a pass over the vector: v_i = s_i * V[:, i].
+// Thick-restart Lanczos (Wu & Simon's restart, full reorthogonalization), computed
+// s steps at a time.
+//
+// The reference's Lanczos (KrylovKit's, src/solvers.jl:85-106) extends the Krylov
+// basis by one vector per matvec and orthogonalizes it against the whole basis,
+// which reads the basis twice per matvec.  Here each cycle extends it by s vectors
+// at once: K = [q, T1(Hs) q, ..., Ts(Hs) q] with Ts the Chebyshev polynomials of H
+// scaled to the spectral interval [lo, hi] (s matvecs), then K is orthogonalized
+// against the basis Q in one block pass (block classical Gram-Schmidt, twice) and
+// within itself (Cholesky QR, twice).  The projected matrix of the extended basis
+// follows from the Chebyshev recurrence coefficients and the two factorizations
+// without further matvecs (the s-step Krylov formulation of Hoemmen's thesis,
+// 2010, ch. 4), so the basis is read three times per s matvecs.  The subspace is
+// the one the one-vector recurrence builds; only the order of the arithmetic
+// differs.
+//
+// Between cycles the Krylov decomposition H Q = Q T + q b^T is kept, with Q
+// orthonormal (k columns), q a unit vector orthogonal to Q, T Hermitian.
 
 template <class S>
 struct LanczosOut {
     std::vector<double> theta;   // Ritz values of the last projection
-    std::vector<double> resid;   // |beta y_k|
+    std::vector<double> resid;   // |b^T y|, the Ritz residual norms
     int converged = 0;
     int restarts = 0;
+    int cycles = 0;
     bool invariant = false;
 };
 
+// Upper Cholesky factor R (n x n) of the leading block of a Hermitian G (leading
+// dimension ldg): G = R^H R.  Returns the first column whose pivot is not above
+// `dep` times its diagonal entry (n when every pivot passes); the off-diagonal
+// entries of that column are filled before its pivot is tested.
+int chol_upper(int n, const std::vector<cd>& G, int ldg, std::vector<cd>& R, double dep, double& pivot_rel) {
+    R.assign(size_t(n) * n, cd(0));
+    pivot_rel = 1.0;
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < j; ++i) {
+            cd s = G[i + size_t(j) * ldg];
+            for (int l = 0; l < i; ++l) s -= std::conj(R[l + size_t(i) * n]) * R[l + size_t(j) * n];
+            R[i + size_t(j) * n] = s / R[i + size_t(i) * n];
+        }
+        const double gjj = G[j + size_t(j) * ldg].real();
+        double d = gjj;
+        for (int l = 0; l < j; ++l) d -= std::norm(R[l + size_t(j) * n]);
+        if (!(gjj > 0) || !(d > dep * gjj)) {
+            pivot_rel = gjj > 0 ? d / gjj : 0.0;
+            return j;
+        }
+        R[j + size_t(j) * n] = std::sqrt(d);
+    }
+    return n;
+}
+
+// Inverse of an upper triangular n x n matrix (column-major, leading dimension n).
+std::vector<cd> inv_upper(int n, const std::vector<cd>& R) {
+    std::vector<cd> X(size_t(n) * n, cd(0));
+    for (int j = 0; j < n; ++j) {
+        X[j + size_t(j) * n] = 1.0 / R[j + size_t(j) * n];
+        for (int i = j - 1; i >= 0; --i) {
+            cd s = 0;
+            for (int l = i + 1; l <= j; ++l) s += R[i + size_t(l) * n] * X[l + size_t(j) * n];
+            X[i + size_t(j) * n] = -s / R[i + size_t(i) * n];
+        }
+    }
+    return X;
+}
+
+// C (r x c) = A (r x t) B (t x c), all column-major with leading dimensions r, r, t.
+std::vector<cd> mm(int r, int t, int c, const std::vector<cd>& A, const std::vector<cd>& B) {
+    std::vector<cd> C(size_t(r) * c, cd(0));
+    for (int j = 0; j < c; ++j)
+        for (int l = 0; l < t; ++l) {
+            const cd b = B[l + size_t(j) * t];
+            if (b == cd(0)) continue;
+            for (int i = 0; i < r; ++i) C[i + size_t(j) * r] += A[i + size_t(l) * r] * b;
+        }
+    return C;
+}
+
+template <class S>
+int apply_cheb_timed(kfc_op* op, const S* B, const S* A, S* C, double a, double b, double g, Timer& t) {
+    auto t0 = clk::now();
+    std::pair<hipEvent_t, hipEvent_t> pr;
+    KFC_HIP(op->ctx, t.pair(pr));
+    KFC_HIP(op->ctx, hipEventRecord(pr.first, op->ctx->stream));
+    int rc = op_apply_cheb(op, B, A, C, 1, op->N, prec_of<S>(), a, b, g, op->ctx->stream);
+    if (rc != KFC_OK) return rc;
+    KFC_HIP(op->ctx, hipEventRecord(pr.second, op->ctx->stream));
+    t.pending.push_back(pr);
+    t.apply += secs(t0);
+    t.cols += 1;
+    if (prec_of<S>() == 1) t.cols32 += 1;
+    t.calls += 1;
+    return KFC_OK;
+}
+
+constexpr int LZ_SMAX = 12;   // Chebyshev block length: the basis condition stays near 1e3
+
+// |b^T y_i| for the columns of Y (n x n): the Ritz residual norms of H V = V T + q b^T.
+std::vector<double> ritz_residuals(int n, const std::vector<cd>& b, const std::vector<cd>& Y) {
+    std::vector<double> r(n);
+    for (int i = 0; i < n; ++i) {
+        cd acc = 0;
+        for (int j = 0; j < n; ++j) acc += b[j] * Y[j + size_t(i) * n];
+        r[i] = std::abs(acc);
+    }
+    return r;
+}
+
+// max-norm condition estimate of an upper triangular s x s matrix
+double cond_upper(int s, const std::vector<cd>& R) {
+    if (s <= 0) return 1.0;
+    const std::vector<cd> Ri = inv_upper(s, R);
+    double a = 0, b = 0;
+    for (size_t i = 0; i < R.size(); ++i) {
+        a = std::max(a, std::abs(R[i]));
+        b = std::max(b, std::abs(Ri[i]));
+    }
+    return double(s) * a * b;
+}
+
 template <class S>
 int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter, const S* start, int64_t start_ld,
-                 uint64_t seed, Timer& tm, LanczosOut<S>& out, DevBuf& result, int verbosity) {
+                 uint64_t seed, double glo, double ghi, Timer& tm, LanczosOut<S>& out, DevBuf& result,
+                 int verbosity, int s_max = LZ_SMAX) {
+    using Rl = typename RealOf<S>::type;
+    constexpr bool single = std::is_same<Rl, float>::value;
+    // a column whose part outside the span of the others is below ~1e-10 (FP64
+    // storage) or ~1e-5 (FP32) of its norm counts as dependent: the Krylov space
+    // is then invariant
+    const double dep = single ? 1e-10 : 1e-20;
+    const double eps_s = single ? 5.96e-8 : 1.11e-16;
     const int64_t N = op->N;
-    DevBuf bufA, bufB;
+    DevBuf bufA, bufB, bufR;
     KFC_HIP(op->ctx, bufA.ensure(sizeof(S) * size_t(N) * (kd + 1)));
     KFC_HIP(op->ctx, bufB.ensure(sizeof(S) * size_t(N) * (kd + 1)));
     S* V = bufA.as<S>();
     S* V2 = bufB.as<S>();
-    std::vector<double> scale(kd + 1, 1.0);
     int rc;
     if (start) rc = blas_convert<S, S>(w, start, start_ld, V, N, 1, N);
     else rc = blas_randn<S>(w, V, N, 1, N, seed);
     if (rc != KFC_OK) return rc;
-    std::vector<double> nrm;
-    rc = blas_colnorms<S>(w, V, N, 1, N, nrm);
-    if (rc != KFC_OK) return rc;
-    if (!(nrm[0] > 0)) return kfc_fail(op->ctx, KFC_EBREAKDOWN, "Lanczos: zero start vector");
-    scale[0] = 1.0 / nrm[0];
-    std::vector<cd> T(size_t(kd) * kd, cd(0));
+    {
+        std::vector<double> nrm;
+        rc = blas_colnorms<S>(w, V, N, 1, N, nrm);
+        if (rc != KFC_OK) return rc;
+        if (!(nrm[0] > 0)) return kfc_fail(op->ctx, KFC_EBREAKDOWN, "Lanczos: zero start vector");
+        rc = blas_scale_cols<S>(w, V, N, 1, N, std::vector<double>{1.0 / nrm[0]});
+        if (rc != KFC_OK) return rc;
+    }
+    const double hi = ghi;
+    double lo = std::min(glo, ghi - std::max(1e-300, 1e-12 * std::fabs(ghi)));
+    bool ritz_known = false;
     int k = 0;
-    double beta_last = 0;
-    std::vector<double> theta;
-    std::vector<cd> Y;
-    for (int64_t rs = 0;; ++rs) {
-        int jend = kd;
-        for (int j = k; j < kd; ++j) {
-            S* wv = V + size_t(j + 1) * N;
-            rc = apply_timed<S>(op, V + size_t(j) * N, wv, 1, N, tm, scale[j]);
+    std::vector<cd> T, b;   // k x k and k
+    int64_t restarts = 0;
+    int s_limit = std::max(1, std::min(s_max, LZ_SMAX)), s_cap = s_limit, anchors = 0;
+    for (int64_t cyc = 0;; ++cyc) {
+        int s = std::min(s_cap, kd - k);
+        if (!ritz_known) s = std::min(s, 4);   // the first block runs on the loose interval
+        S* K = V + size_t(k) * N;
+        const double cc = 0.5 * (hi + lo), ee = 0.5 * (hi - lo);
+        for (int i = 0; i < s; ++i) {
+            rc = i == 0 ? apply_cheb_timed<S>(op, K, nullptr, K + N, 1.0 / ee, -cc / ee, 0.0, tm)
+                        : apply_cheb_timed<S>(op, K + size_t(i) * N, K + size_t(i - 1) * N, K + size_t(i + 1) * N,
+                                              2.0 / ee, -2.0 * cc / ee, -1.0, tm);
             if (rc != KFC_OK) return rc;
-            std::vector<cd> h, h2;
-            double wn0 = 0, wn1 = 0;
-            rc = lz_multidot<S>(w, V, N, j + 1, scale.data(), wv, N, h, &wn0);
-            if (rc != KFC_OK) return rc;
-            std::vector<cd> hs(j + 1);
-            for (int i = 0; i <= j; ++i) hs[i] = scale[i] * h[i];
-            rc = lz_update<S>(w, V, N, j + 1, hs, wv, N, &wn1);
-            if (rc != KFC_OK) return rc;
-            // classical Gram-Schmidt twice ("twice is enough"): a single pass loses
-            // orthogonality geometrically across thick restarts
-            {
-                rc = lz_multidot<S>(w, V, N, j + 1, scale.data(), wv, N, h2, nullptr);
-                if (rc != KFC_OK) return rc;
-                for (int i = 0; i <= j; ++i) hs[i] = scale[i] * h2[i], h[i] += h2[i];
-                rc = lz_update<S>(w, V, N, j + 1, hs, wv, N, &wn1);
-                if (rc != KFC_OK) return rc;
-            }
-            for (int i = 0; i <= j; ++i) {
-                T[i + size_t(j) * kd] = h[i];
-                T[j + size_t(i) * kd] = std::conj(h[i]);
-            }
-            T[j + size_t(j) * kd] = cd(h[j].real(), 0);
-            const double beta = std::sqrt(std::max(wn1, 0.0));
-            beta_last = beta;
-            if (j + 1 < kd) {
-                T[(j + 1) + size_t(j) * kd] = 0;   // filled by the next step's projection
-                T[j + size_t(j + 1) * kd] = 0;
-            }
-            double hn = 0;
-            for (auto& x : h) hn += std::norm(x);
-            if (beta <= 1e-13 * std::sqrt(hn + wn0)) {   // an invariant subspace
-                jend = j + 1;
-                out.invariant = true;
-                scale[j + 1] = 0.0;
-                break;
-            }
-            scale[j + 1] = 1.0 / beta;
         }
-        const int n = jend;
-        std::vector<cd> Tn(size_t(n) * n);
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i) Tn[i + size_t(j) * n] = T[i + size_t(j) * kd];
-        small_eig(n, Tn, theta, Y);
-        std::vector<double> resid(n);
-        for (int i = 0; i < n; ++i) resid[i] = out.invariant ? 0.0 : beta_last * std::abs(Y[(n - 1) + size_t(i) * n]);
+        const int nb0 = s + 1;
+        // block classical Gram-Schmidt against Q, twice (K <- K - Q Cm), the second
+        // and third reads of [Q K] fused with the next Gram: C1 = Q^H K;
+        // K -= Q C1 with C2 = Q^H K; K -= Q C2 with G = K^H K
+        std::vector<cd> Cm(size_t(k) * nb0, cd(0)), Gk;
+        if (k > 0) {
+            std::vector<cd> C1, C2;
+            rc = blas_gram<S>(w, V, N, k, K, N, nb0, N, C1);
+            if (rc != KFC_OK) return rc;
+            auto upd = [&](const std::vector<cd>& C) {   // [[-C]; [I]] on [Q K]
+                std::vector<cd> M(size_t(k + nb0) * nb0, cd(0));
+                for (int j = 0; j < nb0; ++j) {
+                    for (int i = 0; i < k; ++i) M[i + size_t(j) * (k + nb0)] = -C[i + size_t(j) * k];
+                    M[k + j + size_t(j) * (k + nb0)] = 1.0;
+                }
+                return M;
+            };
+            rc = blas_tsmm<S>(w, V, N, k + nb0, upd(C1), nb0, K, N, N, 1, k, &C2);
+            if (rc != KFC_OK) return rc;
+            rc = blas_tsmm<S>(w, V, N, k + nb0, upd(C2), nb0, K, N, N, 2, 0, &Gk);
+            if (rc != KFC_OK) return rc;
+            for (size_t i = 0; i < Cm.size(); ++i) Cm[i] = C1[i] + C2[i];
+        } else {
+            rc = blas_gram<S>(w, K, N, nb0, K, N, nb0, N, Gk);
+            if (rc != KFC_OK) return rc;
+        }
+        // Cholesky QR, twice, on the independent leading columns:
+        // K_orig[:, j] = Q Cm[:, j] + Qn[:, 0:nind] Rt[0:nind, j]
+        int nind = nb0;
+        bool dep_last = false;   // column s lies in the span of the columns before it
+        const int ldr = nb0;
+        std::vector<cd> Rt(size_t(nb0) * nb0, cd(0));
+        for (int i = 0; i < nb0; ++i) Rt[i + size_t(i) * ldr] = 1.0;
+        std::vector<cd> G = Gk;   // Gram of the current K[:, 0:nind] (leading dimension ldg)
+        int ldg = nb0;
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<cd> Rp, dcoef;
+            for (;;) {
+                double prel = 1;
+                const int n0 = nind;
+                const int fc = chol_upper(nind, G, ldg, Rp, dep, prel);
+                if (fc == nind) break;
+                if (pass == 0 && fc >= 1 && (std::fabs(prel) <= dep || fc == 1)) {
+                    dcoef.assign(Rp.begin() + size_t(fc) * n0, Rp.begin() + size_t(fc) * n0 + fc);
+                    dep_last = true;
+                    s = fc;
+                    nind = fc;
+                } else if (fc >= 2) {   // an ill-conditioned block: drop its tail
+                    s = fc - 1;
+                    nind = fc;
+                    dep_last = false;
+                } else {
+                    return kfc_fail(op->ctx, KFC_EBREAKDOWN, "Lanczos: the Krylov block lost rank");
+                }
+            }
+            const int nb = s + 1;
+            // K[:, 0:nind] <- K[:, 0:nind] Rp^{-1}, in place (with the next pass's Gram)
+            const std::vector<cd> Ri = inv_upper(nind, Rp);
+            std::vector<cd> Gn;
+            rc = blas_tsmm<S>(w, K, N, nind, Ri, nind, K, N, N, pass == 0 ? 2 : 0, 0, pass == 0 ? &Gn : nullptr);
+            if (rc != KFC_OK) return rc;
+            G.swap(Gn);
+            ldg = nind;
+            // Rt <- Rp Rt (the dependent column's coefficients come from the partial factor)
+            std::vector<cd> Rn(size_t(nb0) * nb0, cd(0));
+            for (int j = 0; j < nb; ++j)
+                for (int i = 0; i < nind; ++i) {
+                    if (pass == 0 && dep_last && j == nind) {
+                        Rn[i + size_t(j) * ldr] = dcoef[i];
+                        continue;
+                    }
+                    cd acc = 0;
+                    for (int l = i; l < nind; ++l) acc += Rp[i + size_t(l) * nind] * Rt[l + size_t(j) * ldr];
+                    Rn[i + size_t(j) * ldr] = acc;
+                }
+            Rt.swap(Rn);
+        }
+        KFC_HIP(op->ctx, tm.flush());   // the block's applies finished before the Grams came back
+        const int nb = s + 1, n = k + s, rows = k + nb;
+        // coordinates in [Q, Qn] (rows x ...): H Q = [Q Qn] Lm, K = [Q Qn] CR, H K[:, 0:s] = K Bh
+        std::vector<cd> Lm(size_t(rows) * k, cd(0)), CR(size_t(rows) * nb, cd(0)), Bh(size_t(nb) * s, cd(0));
+        for (int j = 0; j < k; ++j) {
+            for (int i = 0; i < k; ++i) Lm[i + size_t(j) * rows] = T[i + size_t(j) * k] + Cm[i] * b[j];
+            for (int i = 0; i < nb; ++i) Lm[k + i + size_t(j) * rows] = Rt[i] * b[j];
+        }
+        for (int j = 0; j < nb; ++j) {
+            for (int i = 0; i < k; ++i) CR[i + size_t(j) * rows] = Cm[i + size_t(j) * k];
+            for (int i = 0; i < nb; ++i) CR[k + i + size_t(j) * rows] = Rt[i + size_t(j) * ldr];
+        }
+        for (int i = 0; i < s; ++i) {
+            Bh[i + size_t(i) * nb] = cc;
+            Bh[i + 1 + size_t(i) * nb] = i == 0 ? ee : 0.5 * ee;
+            if (i > 0) Bh[i - 1 + size_t(i) * nb] = 0.5 * ee;
+        }
+        // right block: (CR Bh - Lm Cm[:, 0:s]) Rss^{-1}
+        std::vector<cd> Cs(size_t(k) * s);
+        for (int j = 0; j < s; ++j)
+            for (int i = 0; i < k; ++i) Cs[i + size_t(j) * k] = Cm[i + size_t(j) * k];
+        std::vector<cd> Z = mm(rows, nb, s, CR, Bh);
+        if (k > 0) {
+            const std::vector<cd> LC = mm(rows, k, s, Lm, Cs);
+            for (size_t i = 0; i < Z.size(); ++i) Z[i] -= LC[i];
+        }
+        std::vector<cd> Rss(size_t(s) * s);
+        for (int j = 0; j < s; ++j)
+            for (int i = 0; i < s; ++i) Rss[i + size_t(j) * s] = Rt[i + size_t(j) * ldr];
+        const std::vector<cd> right = mm(rows, s, s, Z, inv_upper(s, Rss));
+        // the (k + s) x (k + s) projection and the residual row
+        std::vector<cd> Tn(size_t(n) * n), bnew(n);
+        for (int j = 0; j < n; ++j) {
+            const cd* col = j < k ? &Lm[size_t(j) * rows] : &right[size_t(j - k) * rows];
+            for (int i = 0; i < n; ++i) Tn[i + size_t(j) * n] = col[i];
+            bnew[j] = dep_last ? cd(0) : col[n];
+        }
+        std::vector<double> theta;
+        std::vector<cd> Y;
+        small_eig(n, Tn, theta, Y);   // symmetrizes Tn
+        std::vector<double> resid = ritz_residuals(n, bnew, Y);
         int conv = 0;
         while (conv < n && resid[conv] <= tol) ++conv;
-        if (verbosity > 0 && (rs % verbosity == 0))
-            fprintf(stderr, "[kfc lanczos] restart %lld matvec %lld converged %d resid[m-1] %.3e theta0 %.12f\n",
-                    (long long)rs, (long long)tm.cols, conv, resid[std::min(m, n) - 1], theta[0]);
+        // rounding in the projection grows with the block's condition number
+        // (eps |H| cond(Rss) per cycle): long blocks while the wanted residuals are
+        // far above the tolerance, short ones for the last digits
+        {
+            const double far = resid[std::min(m, n) - 1];
+            const double kap = cond_upper(s, Rss);
+            const double noise = eps_s * std::max(std::fabs(hi), std::fabs(lo)) * kap;
+            int want = far > 1e3 * tol ? LZ_SMAX : (far > 30 * tol ? 6 : 3);
+            if (noise > far * 1e-2 || kap > 1e5) want = std::max(1, std::min(want, s - 2));
+            s_cap = std::min(want, s_limit);
+        }
+        bool stop = conv >= m || dep_last || (n >= kd && restarts >= maxiter);
+        if (stop && !dep_last) {
+            // confirm with an explicit Rayleigh-Ritz on the basis: W = H V gives the
+            // exact projection and the true residuals of the wanted Ritz pairs
+            S* W = V2;
+            rc = apply_timed<S>(op, V, W, n, N, tm);
+            if (rc != KFC_OK) return rc;
+            std::vector<cd> Tx, bx;
+            rc = blas_gram<S>(w, V, N, n, W, N, n, N, Tx);
+            if (rc == KFC_OK) rc = blas_gram<S>(w, V + size_t(n) * N, N, 1, W, N, n, N, bx);
+            if (rc != KFC_OK) return rc;
+            small_eig(n, Tx, theta, Y);
+            const int q = std::min(m, n);
+            std::vector<cd> M(size_t(n) * q);
+            for (int cI = 0; cI < q; ++cI)
+                for (int i = 0; i < n; ++i) M[i + size_t(cI) * n] = Y[i + size_t(cI) * n];
+            KFC_HIP(op->ctx, result.ensure(sizeof(S) * size_t(N) * std::max(q, 1)));
+            rc = blas_tsmm<S>(w, V, N, n, M, q, result.as<S>(), N, N, 0, 0, nullptr);
+            std::vector<double> tr;
+            KFC_HIP(op->ctx, bufR.ensure(sizeof(S) * size_t(N) * std::max(q, 1)));
+            if (rc == KFC_OK) rc = blas_tsmm<S>(w, W, N, n, M, q, bufR.as<S>(), N, N, 0, 0, nullptr);
+            if (rc == KFC_OK) rc = blas_residual<S>(w, bufR.as<S>(), N, bufR.as<S>(), N, result.as<S>(), N, q, N,
+                                                  std::vector<double>(theta.begin(), theta.begin() + q), tr);
+            if (rc != KFC_OK) return rc;
+            // the convergence test reads the residuals of the exact Krylov relation
+            // H V = V T + q b^T, as KrylovKit's normres does, and asks the true
+            // residuals of the returned vectors to sit within the rounding floor of
+            // H v (10 eps max(|H|, |E_1|), src/solvers.jl:75) of the tolerance: a
+            // basis whose Krylov relation was corrupted by an ill-conditioned block
+            // fails here and goes on with short blocks
+            resid = ritz_residuals(n, bx, Y);
+            const double floor_ = 10.0 * eps_s * std::max(std::max(std::fabs(hi), std::fabs(lo)), std::fabs(theta[0]));
+            const double tbar = 10.0 * std::max(tol, floor_);
+            conv = 0;
+            while (conv < n && resid[conv] <= tol && (conv >= q || tr[conv] <= tbar)) ++conv;
+            bool polluted = false;
+            for (int i = 0; i < q; ++i) polluted |= resid[i] <= tol && tr[i] > tbar;
+            if (polluted) s_limit = 1;
+            ++anchors;
+            out.theta = theta;
+            out.resid = resid;
+            out.converged = conv;
+            if (verbosity > 0)
+                fprintf(stderr, "[kfc lanczos] explicit check %d: converged %d of %d, resid[m-1] %.3e, true max %.3e\n",
+                        anchors, conv, m, resid[q - 1], *std::max_element(tr.begin(), tr.end()));
+            if (conv >= m || (n >= kd && restarts >= maxiter) || anchors >= 16) return KFC_OK;
+            // not there yet: continue from the exact decomposition with shorter blocks
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) Tn[i + size_t(j) * n] = 0.5 * (Tx[i + size_t(j) * n] + std::conj(Tx[j + size_t(i) * n]));
+            bnew = bx;
+            s_limit = std::max(1, std::min(s_limit, s) / 2);
+            s_cap = std::min(s_cap, s_limit);
+            stop = false;
+        }
         out.theta = theta;
         out.resid = resid;
         out.converged = conv;
-        out.restarts = int(rs);
-        if (conv >= m || rs >= maxiter || out.invariant || n < kd) {
-            // Ritz vectors of the lowest min(m, n) values
+        out.restarts = int(restarts);
+        out.cycles = int(cyc + 1);
+        out.invariant = dep_last;
+        if (verbosity > 0 && cyc % verbosity == 0)
+            fprintf(stderr, "[kfc lanczos] cycle %lld restart %lld matvec %lld s %d converged %d resid[m-1] %.3e theta0 %.12f\n",
+                    (long long)cyc, (long long)restarts, (long long)tm.cols, s, conv, resid[std::min(m, n) - 1],
+                    theta[0]);
+        if (stop) {   // an invariant subspace: its Ritz pairs are exact
             const int q = std::min(m, n);
             std::vector<cd> M(size_t(n) * q);
-            for (int c = 0; c < q; ++c)
-                for (int i = 0; i < n; ++i) M[i + size_t(c) * n] = scale[i] * Y[i + size_t(c) * n];
+            for (int cI = 0; cI < q; ++cI)
+                for (int i = 0; i < n; ++i) M[i + size_t(cI) * n] = Y[i + size_t(cI) * n];
             KFC_HIP(op->ctx, result.ensure(sizeof(S) * size_t(N) * std::max(q, 1)));
-            rc = blas_gemm<S>(w, V, N, n, M, q, result.as<S>(), N, N, 0.0);
-            return rc;
+            return blas_gemm<S>(w, V, N, n, M, q, result.as<S>(), N, N, 0.0);
         }
-        // restart: keep the lowest `keep` Ritz vectors and the residual direction
-        int keep = (3 * kd + 2 * conv) / 5;
-        keep = std::max(keep, std::min(m + 1, kd - 2));
-        keep = std::min(keep, kd - 2);
-        std::vector<cd> M(size_t(kd) * keep);
-        for (int c = 0; c < keep; ++c)
-            for (int i = 0; i < kd; ++i) M[i + size_t(c) * kd] = scale[i] * Y[i + size_t(c) * kd];
-        rc = blas_gemm<S>(w, V, N, kd, M, keep, V2, N, N, 0.0);
-        if (rc != KFC_OK) return rc;
-        rc = blas_axpby<S>(w, V + size_t(kd) * N, N, V2 + size_t(keep) * N, N, 1, N, scale[kd], 0.0);
-        if (rc != KFC_OK) return rc;
-        std::swap(V, V2);
-        std::fill(scale.begin(), scale.end(), 1.0);
-        std::fill(T.begin(), T.end(), cd(0));
-        for (int i = 0; i < keep; ++i) T[i + size_t(i) * kd] = theta[i];
-        k = keep;
+        ritz_known = true;
+        lo = theta[0] - std::max(resid[0], 1e-3 * (hi - theta[0]));
+        if (n >= kd) {
+            // thick restart: keep the lowest Ritz vectors and the residual direction
+            int keep = (3 * kd + 2 * conv) / 5;
+            keep = std::max(keep, std::min(m + 1, kd - 2));
+            keep = std::min(keep, kd - 2);
+            // [V2 Ritz vectors, q] = [V q] [[Y_keep, 0]; [0, 1]] in one pass
+            std::vector<cd> M(size_t(n + 1) * (keep + 1), cd(0));
+            for (int cI = 0; cI < keep; ++cI)
+                for (int i = 0; i < n; ++i) M[i + size_t(cI) * (n + 1)] = Y[i + size_t(cI) * n];
+            M[n + size_t(keep) * (n + 1)] = 1.0;
+            rc = blas_tsmm<S>(w, V, N, n + 1, M, keep + 1, V2, N, N, 0, 0, nullptr);
+            if (rc != KFC_OK) return rc;
+            std::swap(V, V2);
+            T.assign(size_t(keep) * keep, cd(0));
+            b.assign(keep, cd(0));
+            for (int i = 0; i < keep; ++i) {
+                T[i + size_t(i) * keep] = theta[i];
+                cd r = 0;
+                for (int j = 0; j < n; ++j) r += bnew[j] * Y[j + size_t(i) * n];
+                b[i] = r;
+            }
+            k = keep;
+            ++restarts;
+        } else {
+            T.assign(size_t(n) * n, cd(0));
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) T[i + size_t(j) * n] = 0.5 * (Tn[i + size_t(j) * n] + std::conj(Tn[j + size_t(i) * n]));
+            b = bnew;
+            k = n;
+        }
     }
 }
 
@@ -302,25 +618,24 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
     if (m > N) return kfc_fail(op->ctx, KFC_EARG, "Lanczos: m = " + std::to_string(m) + " exceeds the operator dimension " + std::to_string(N));
     int kd = int(std::max<int64_t>(2 * m + 10, krylovdim));   // src/solvers.jl:92
     kd = int(std::min<int64_t>(kd, N));
-    if (kd > 63) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "Lanczos: krylovdim above 63 is not served");
+    if (kd > 512) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "Lanczos: krylovdim above 512 is not served");
     Work w;
     w.ctx = op->ctx;
     Timer tm;
     DevBuf res64;
     LanczosOut<S> out;
     int rc;
+    double glo = 0, hb = 0;
+    rc = op_spectral_interval(op, &glo, &hb);   // the Chebyshev interval of the first block
+    if (rc != KFC_OK) return rc;
     if (mode == KFC_MODE_MIXED) {
         using L = typename Lower<S>::type;
-        double hb = 0;
-        kfc_op_gershgorin(op, &hb);
-        double hn = 0;
         // the FP32 floor: a few units of roundoff of the largest eigenvalue scale
         const double tol32 = std::max(tol, 64.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
         LanczosOut<L> o32;
         DevBuf res32;
-        rc = lanczos_core<L>(op, w, int(m), kd, tol32, maxiter, nullptr, 0, seed, tm, o32, res32, verbosity);
+        rc = lanczos_core<L>(op, w, int(m), kd, tol32, maxiter, nullptr, 0, seed, glo, hb, tm, o32, res32, verbosity);
         if (rc != KFC_OK) return rc;
-        (void)hn;
         // FP64 continuation from the FP32 Ritz vectors combined into one start vector
         const int q = int(std::min<int64_t>(m, N));
         DevBuf s64;
@@ -335,10 +650,13 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
         KFC_HIP(op->ctx, st64.ensure(sizeof(S) * size_t(N)));
         rc = blas_gemm<S>(w, s64.as<S>(), N, q, comb, 1, st64.as<S>(), N, N, 0.0);
         if (rc != KFC_OK) return rc;
-        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, st64.as<S>(), N, seed, tm, out, res64, verbosity);
+        // the start vector lies almost in an invariant subspace: a long Chebyshev block
+        // on it would be ill-conditioned, so the continuation runs short blocks
+        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, st64.as<S>(), N, seed, glo, hb, tm, out, res64, verbosity,
+                             2);
         out.restarts += o32.restarts;
     } else {
-        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, nullptr, 0, seed, tm, out, res64, verbosity);
+        rc = lanczos_core<S>(op, w, int(m), kd, tol, maxiter, nullptr, 0, seed, glo, hb, tm, out, res64, verbosity);
     }
     if (rc != KFC_OK) return rc;
     const int n = int(out.theta.size());

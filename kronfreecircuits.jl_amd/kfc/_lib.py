@@ -88,6 +88,8 @@ SIGNATURES = {
     "kfc_op_destroy": (C.c_int, [_P]),
     "kfc_op_size": (C.c_int64, [_P]),
     "kfc_op_fused": (C.c_int, [_P]),
+    "kfc_op_stencil_get": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "kfc_op_stencil_set": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "kfc_apply": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_int32]),
     "kfc_sync": (C.c_int, [_P]),
     "kfc_op_diagonal": (C.c_int, [_P, _D]),
@@ -104,8 +106,6 @@ SIGNATURES = {
                                 C.c_int32, C.c_double, C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
     "kfc_dense": (C.c_int, [_P, C.c_int64, _D, _D, C.POINTER(Stats)]),
     "kfc_residuals": (C.c_int, [_P, C.c_int64, _D, _D, _D]),
-    "kfc_batch_transmons": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _D, C.c_int32,
-                                      C.c_int32, C.c_double, C.c_int32, _D, _D, C.POINTER(Stats)]),
     "kfc_mlp_create": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.c_uint64, C.POINTER(_P)]),
     "kfc_mlp_destroy": (C.c_int, [_P]),
     "kfc_mlp_set_params": (C.c_int, [_P, C.POINTER(C.c_float)]),

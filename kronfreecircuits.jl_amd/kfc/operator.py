@@ -131,6 +131,18 @@ class KronFreeOperator:
         return bool(L.lib().kfc_op_fused(self.handle))
 
     @property
+    def stencil(self):
+        """The fused stencil's launch shape ``(variant, march)`` (the autotuner's
+        pick after the first application)."""
+        v, m = C.c_int32(), C.c_int32()
+        L.check(L.lib().kfc_op_stencil_get(self.handle, C.byref(v), C.byref(m)), self.ctx.handle)
+        return v.value, m.value
+
+    def set_stencil(self, variant, march=0):
+        """Pin the fused stencil's launch shape; ``variant=-1`` hands it back to the autotuner."""
+        L.check(L.lib().kfc_op_stencil_set(self.handle, int(variant), int(march)), self.ctx.handle)
+
+    @property
     def shape(self):
         return (self.N, self.N)
 

@@ -77,10 +77,10 @@ class ZeroPiSweep:
         Vt = kids[0].vectors
         charge = np.argmax(np.abs(Vt), axis=0) - self.ncut          # kept index -> charge n
         pos = {int(n): k for k, n in enumerate(charge)}
-        d0, d1 = Vt.shape[1], kids[1].vectors.shape[1]
+        d0, d1, d2 = Vt.shape[1], kids[1].vectors.shape[1], kids[2].vectors.shape[1]
         idx = []
-        for iz in range(ZETA_LEVELS):
-            for ip in range(PHI_LEVELS):
+        for iz in range(min(ZETA_LEVELS, d2)):
+            for ip in range(min(PHI_LEVELS, d1)):
                 for n in range(*WINDOW):
                     k = pos.get(n, -1)
                     idx.append(-1 if k < 0 else k + d0 * (ip + d1 * iz))
@@ -158,8 +158,9 @@ class WarmStart:
         X = np.stack([features(t) for t in thetas])
         T = np.stack([self.coeffs(Vf).reshape(-1, order="F") for Vf in Vframes]).astype(np.float32)
         self.scale = float(np.max(np.abs(T))) or 1.0
+        loss0 = float(np.mean((self.net.forward(X) - T / self.scale) ** 2))
         loss = self.net.train(X, T / self.scale, batch=min(batch, len(X)), steps=steps, lr=lr)
-        return loss
+        return loss0, loss
 
     def predict_frames(self, thetas):
         X = np.stack([features(t) for t in thetas])

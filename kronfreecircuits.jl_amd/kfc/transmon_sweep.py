@@ -1,0 +1,109 @@
+"""Batched sweep of coupled-transmon instances on periodic phase grids.
+
+An instance is the two-transmon circuit (test/fixtures/two_transmons.json form:
+E_C matrix [[E_C1, g/8], [g/8, E_C2]], cosines E_J1, E_J2, offset charges n_g1,
+n_g2) with each mode on PeriodicGrid(N_i, order), N_i drawn from {256, 384, 512,
+640, 768}: two complex leaves (the offset charge makes n complex) keeping 8
+states, and a top node [8, 8] keeping 20.  There is no reference counterpart
+for the batch (the closest is the warm-started sweep of bench/common.jl:135-163);
+each instance is the tree that materialize (src/materialize.jl:713-731) would
+solve, and its energies are the ones it returns.
+
+Two ways to run a batch:
+
+* plain: instance after instance, every leaf by FP64 LOBPCG from the product
+  seed, the top Dense (what materialize does per point on the device);
+* batched: the leaves of all instances in flight together on several device
+  contexts (kfc_lobpcg_batch) in FP32 with FP64 refinement (mode="mixed"),
+  then the tops."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .bases import PeriodicGrid
+from .operator import KronFreeOperator
+from .solvers import LOBPCG, Dense, solve_batch, solve_stats
+from .tree import (MaterializedNode, Node, circuit_terms, constant_term, fix_phases, materialize, node_axisterms,
+                   operator, positive_mask, route)
+from .configs import two_transmons_spec
+
+GRID_SIZES = (256, 384, 512, 640, 768)
+# the ranges the instance parameters are drawn from (GHz): about the fixture's
+# values (E_J 25-30, E_C 1.0-1.2, g 0.3)
+RANGES = dict(E_J=(20.0, 35.0), E_C=(0.8, 1.4), n_g=(0.0, 1.0), g=(0.1, 0.5))
+
+
+def draw(n, seed=20261203):
+    """n instances: grid sizes and parameters, seeded."""
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        u = lambda k: float(r.uniform(*RANGES[k]))
+        out.append(dict(N1=int(r.choice(GRID_SIZES)), N2=int(r.choice(GRID_SIZES)), E_J1=u("E_J"), E_C1=u("E_C"),
+                        n_g1=u("n_g"), E_J2=u("E_J"), E_C2=u("E_C"), n_g2=u("n_g"), g=u("g")))
+    return out
+
+
+def spec_of(inst):
+    return two_transmons_spec(inst["E_J1"], inst["E_C1"], inst["n_g1"], inst["E_J2"], inst["E_C2"], inst["n_g2"],
+                              inst["g"])
+
+
+def tree_of(inst, order=8, k_leaf=8, k_top=20, leaf_solver=None):
+    # the offset charges make every node complex (n on a periodic grid is imaginary)
+    leaves = [Node([(f"varphi_{i}", PeriodicGrid(inst[f"N{i}"], order))], keep=k_leaf, solver=leaf_solver,
+                   real=False) for i in (1, 2)]
+    return Node(leaves, keep=k_top, solver=Dense(), real=False)
+
+
+class TransmonSweep:
+    def __init__(self, order=8, k_leaf=8, k_top=20, contexts=8):
+        self.order, self.k_leaf, self.k_top = order, k_leaf, k_top
+        self.ctxs = [L.context(0)] + [L.Context(0) for _ in range(max(0, contexts - 1))]
+
+    def run_plain(self, insts, tol=1e-8):
+        """Instance after instance: materialize with FP64 LOBPCG leaves and a Dense top."""
+        out, t0 = [], time.perf_counter()
+        for inst in insts:
+            root = tree_of(inst, self.order, self.k_leaf, self.k_top, LOBPCG(tol=tol, mode="f64"))
+            sysm = materialize(root, spec_of(inst))
+            out.append(sysm.root)
+        return out, time.perf_counter() - t0
+
+    def run_batched(self, insts, mode="mixed", tol=1e-8):
+        """All leaves of all instances solved together, then every top."""
+        t0 = time.perf_counter()
+        roots, specs, routed, leaf_ops, leaf_nodes = [], [], [], [], []
+        c = 0
+        for inst in insts:
+            root = tree_of(inst, self.order, self.k_leaf, self.k_top)
+            spec = spec_of(inst)
+            rt = route(root, circuit_terms(spec))
+            roots.append(root)
+            specs.append(spec)
+            routed.append(rt)
+            for leaf in root.children:
+                dims = [leaf.children[0][1].dim]
+                ts = node_axisterms(leaf, [None], rt.get(id(leaf), []), leaf.real)
+                H = KronFreeOperator(dims, ts, constant=0.0, block_capacity=self.k_leaf + 4,
+                                     ctx=self.ctxs[c % len(self.ctxs)])
+                c += 1
+                leaf_ops.append(H)
+                leaf_nodes.append(leaf)
+        res, st = solve_batch(LOBPCG(tol=tol, mode=mode), leaf_ops, self.k_leaf)
+        t_leaves = time.perf_counter() - t0
+        mats = []
+        for (E, V), s, H, leaf in zip(res, st, leaf_ops, leaf_nodes):
+            V = fix_phases(np.asarray(V), positive_mask(leaf, H.dims))
+            mats.append(MaterializedNode(leaf, list(H.dims), np.asarray(E), V, [None], H, None, s))
+        tops = []
+        for i, (root, spec) in enumerate(zip(roots, specs)):
+            kids = mats[2 * i:2 * i + 2]
+            Ht, _ = operator(root, spec, kids=kids)
+            E, V, s = solve_stats(Dense(), Ht, self.k_top)
+            tops.append(MaterializedNode(root, list(Ht.dims), np.asarray(E)[:self.k_top],
+                                         np.asarray(V)[:, :self.k_top], kids, Ht, Dense(), s))
+        return tops, time.perf_counter() - t0, t_leaves

@@ -71,12 +71,38 @@ def test_fused_path_random(dims, cplx):
     assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
 
 
+STENCIL_SHAPES = [(0, 64), (1, 16), (1, 64), (2, 16), (2, 64), (2, 128)]
+
+
+@pytest.mark.parametrize("variant,march", STENCIL_SHAPES)
+@pytest.mark.parametrize("dims,cplx", [([70, 19, 11], False), ([64, 33, 40], False), ([33, 17, 5], True),
+                                       ([130, 7], False), ([200], True), ([65, 20, 150], False)])
+def test_stencil_variants(dims, cplx, variant, march):
+    """Every launch shape of the fused pass on constant-weight stencils (the plane
+    sweep's domain), with separable and product diagonals, ragged patch edges."""
+    rng = np.random.default_rng(hash((tuple(dims), cplx, 2)) % 2**32)
+    terms = _fusable_terms(rng, dims, cplx)[:-1]
+    if len(dims) >= 2:
+        terms.append((0.7, [(0, "diag", rng.standard_normal(dims[0])), (1, "diag", rng.standard_normal(dims[1]))]))
+    H = to_kfc(dims, terms, constant=0.25)
+    assert H.fused
+    H.set_stencil(variant, march)
+    Ho = to_oracle(dims, terms, constant=0.25)
+    X = rng.standard_normal((H.N, 3)) + (1j * rng.standard_normal((H.N, 3)) if cplx else 0)
+    Yo = O.apply_chain(Ho, X)
+    assert _rel(_run(H, X), Yo) < TOL64
+    assert _rel(_run(H, X, "f32"), Yo) < 50 * TOL32
+    assert H.stencil == (variant, march)
+
+
+@pytest.mark.parametrize("variant,march", [(-1, 0)] + STENCIL_SHAPES)
 @pytest.mark.parametrize("N", [16, 33, 64, 100])
-def test_three_fluxonia_fused_parity(N):
+def test_three_fluxonia_fused_parity(N, variant, march):
     from kfc import configs
     th = configs.three_fluxonia_theta(11)
     H = configs.three_fluxonia_operator(th, N=N)
     assert H.fused
+    H.set_stencil(variant, march)
     Ho = O.three_fluxonia_operator(th["E_C"], th["E_L"], th["E_J"], th["phi_ext"], N=N, order=2)
     rng = np.random.default_rng(N)
     X = rng.standard_normal((H.N, 2))

@@ -142,3 +142,42 @@ def test_chebyshev_complex_periodic_transmon():
     E, V = kfc.solve(kfc.ChebyshevFilter(extra=10), H, 5)
     Ed, _ = kfc.solve(kfc.Dense(), H, 5)
     assert np.allclose(E, Ed, atol=1e-9)
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+@pytest.mark.parametrize("krylovdim", [0, 48])
+def test_lanczos_complex_periodic_transmon(mode, krylovdim):
+    """Complex Hermitian Lanczos (the general block path) against Dense, plus the
+    Krylov-dimension keyword."""
+    import kfc
+    N, EJ, EC, ng = 300, 25.0, 1.0, 0.31
+    b = kfc.PeriodicGrid(N, 8)
+    H = kfc.KronFreeOperator([N], [kfc.AxisTerm(4 * EC, [(0, kfc.localop(b, ("n2", 0)))]),
+                                   kfc.AxisTerm(-8 * EC * ng, [(0, kfc.localop(b, ("n", 0)))]),
+                                   kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", 1)))]),
+                                   kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", -1)))])],
+                             constant=4 * EC * ng ** 2)
+    assert H.dtype == np.complex128
+    E, V, st = kfc.solve_stats(kfc.Lanczos(tol=1e-9, krylovdim=krylovdim, mode=mode), H, 6)
+    Ed, _ = kfc.solve(kfc.Dense(), H, 6)
+    assert np.max(np.abs(E - Ed)) < 1e-9 * max(1.0, np.max(np.abs(Ed)))
+    assert np.allclose(V.conj().T @ V, np.eye(6), atol=1e-9)
+    assert st["max_residual"] < 1e-8
+
+
+@pytest.mark.parametrize("N", [14, 18])
+def test_lanczos_real_block_paths_vs_dense(N):
+    """Real Lanczos at several Krylov dimensions: short and long Chebyshev blocks,
+    thick restarts and the explicit check all agree with Dense."""
+    import kfc
+    from kfc import configs
+    th = configs.three_fluxonia_theta(5)
+    H = configs.three_fluxonia_operator(th, N=N)
+    Ho = oracle_from_kfc_terms(H)
+    w = np.linalg.eigvalsh(O.to_matrix(Ho))
+    for kd in (0, 40, 64):
+        E, V, st = kfc.solve_stats(kfc.Lanczos(tol=1e-10, krylovdim=kd), H, 8)
+        r = O.true_residuals(Ho, E, V)
+        assert np.max(r) < 1e-8
+        assert np.max(np.abs(E - w[:8])) < 1e-9 * max(1.0, np.max(np.abs(w[:8])))
+        assert np.allclose(V.T @ V, np.eye(8), atol=1e-10)
