@@ -103,9 +103,11 @@ def sweep_bench(args):
     log(f"[sweep] plain f64: {args.plain_points} points {t_plain:.2f} s -> {rate_plain:.1f} pairs/s, "
         f"iters {[o[2]['iterations'] for o in plain]}")
     t0 = time.perf_counter()
-    gen, t_gen = sw.run_batched(train, mode="f64")
+    # training data: the lowest m + g pairs (the guards too) from the plain solver
+    nev = sw.width(40 * 40 * 30)
+    gen, t_gen = sw.run_batched(train, mode="f64", nev=nev)
     frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
-    log(f"[sweep] training data: {len(train)} points ({len(train) * m} eigenpairs) in {t_gen:.1f} s")
+    log(f"[sweep] training data: {len(train)} points ({len(train) * nev} eigenpairs) in {t_gen:.1f} s")
     ws = WarmStart(sw, seed=1)
     t1 = time.perf_counter()
     loss = ws.train(train, frames, steps=args.train_steps, lr=1e-3)
@@ -128,7 +130,7 @@ def sweep_bench(args):
     caps = []
     for (E, V, st, kids), xf in zip(ref[:8], Xf):
         X0 = sw.from_frame(xf, sw.frame_index(kids), V.shape[0])
-        caps.append(float(np.mean(subspace_capture(X0, V))))
+        caps.append(float(np.mean(subspace_capture(X0, V[:, :m]))))
     best = max(res, key=lambda k: res[k]["eigenpairs_per_s"])
     out = {"metric": "eigenpairs/s at resid<=1e-8 (zero-pi sweep, parent [40,40,30], LOBPCG lowest 16)",
            "value": res["batched_mixed_learned"]["eigenpairs_per_s"], "unit": "eigenpairs/s", "n_gpus": 1,
@@ -145,7 +147,9 @@ def sweep_bench(args):
            "speedup_vs_plain": {k: round(v["eigenpairs_per_s"] / rate_plain, 2) for k, v in res.items()},
            "best": best, "max_rel_dE_batched_vs_plain": dE,
            "warm_start": {"train_seconds": round(t_train, 2), "data_seconds": round(t_gen, 2),
-                          "final_loss": float(loss[-1]), "mean_subspace_capture": round(float(np.mean(caps)), 4)}}
+                          "initial_loss": float(loss[0]), "final_loss": float(loss[-1]),
+                          "outputs": f"{ws.nb} x {ws.m}", "training_eigenpairs": len(train) * nev,
+                          "mean_subspace_capture": round(float(np.mean(caps)), 4)}}
     print(json.dumps(out))
 
 
@@ -159,26 +163,28 @@ def transmon_bench(args):
 
     insts = T.draw(args.instances, seed=args.seed)
     sw = T.TransmonSweep(contexts=args.contexts)
-    sw.run_batched(insts[:2], mode=args.sweep_mode)      # warm-up
+    mode = args.leaf_mode
+    sw.run_batched(insts[:2], mode=mode)      # warm-up
     sw.run_plain(insts[:1])
     plain, t_plain = sw.run_plain(insts[:args.plain_points])
     n_plain = sum(len(t.energies) for t in plain
                   if all(l.stats["max_residual"] <= 1e-8 for l in t.children))
     rate_plain = n_plain / t_plain
     log(f"[transmons] plain f64: {args.plain_points} instances {t_plain:.2f} s -> {rate_plain:.1f} pairs/s")
-    tops, wall, t_leaves = sw.run_batched(insts, mode=args.sweep_mode)
+    tops, wall, t_leaves = sw.run_batched(insts, mode=mode)
     ok = [t for t in tops if all(l.stats["max_residual"] <= 1e-8 for l in t.children)]
     value = sum(len(t.energies) for t in ok) / wall
-    log(f"[transmons] batched {args.sweep_mode}: {len(insts)} instances {wall:.2f} s (leaves {t_leaves:.2f} s) "
+    log(f"[transmons] batched {mode}: {len(insts)} instances {wall:.2f} s (leaves {t_leaves:.2f} s) "
         f"-> {value:.1f} pairs/s")
     dE = max(float(np.max(np.abs(a.energies - b.energies) / np.abs(a.energies))) for a, b in zip(plain, tops))
     out = {"metric": "eigenpairs/s at resid<=1e-8 (512 coupled transmons on periodic grids, tops keep 20)",
            "value": round(value, 2), "unit": "eigenpairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
            "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32+f64" if args.sweep_mode == "mixed" else "f64",
+           "dtype": "f32+f64" if mode == "mixed" else "f64",
            "data": "synthetic (seeded circuit parameters)",
            "config": {"workload": "transmon_sweep", "instances": len(insts), "grid_sizes": list(T.GRID_SIZES),
-                      "order": 8, "k_leaf": 8, "k_top": 20, "contexts": args.contexts, "mode": args.sweep_mode},
+                      "order": 8, "k_leaf": 8, "k_top": 20, "contexts": args.contexts, "mode": mode,
+                      "leaf_solver": "Lanczos tol 1e-9"},
            "plain_fp64": {"instances": args.plain_points, "seconds": round(t_plain, 3),
                           "eigenpairs_per_s": round(rate_plain, 2)},
            "speedup_vs_plain": round(value / rate_plain, 2), "seconds_leaves": round(t_leaves, 3),
@@ -192,7 +198,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=512)
-    ap.add_argument("--mode", default="mixed", choices=["f64", "mixed"])
+    ap.add_argument("--mode", default="f64", choices=["f64", "mixed"])
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
@@ -207,6 +213,7 @@ def main():
     ap.add_argument("--train-points", type=int, default=256)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--sweep-mode", default="mixed", choices=["f64", "mixed"])
+    ap.add_argument("--leaf-mode", default="f64", choices=["f64", "mixed"], help="transmon sweep leaf solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path", file=sys.stderr)
@@ -255,16 +262,24 @@ def main():
     wall = time.perf_counter() - t0
     value = counted / wall
     st = stats[-1]
-    apply_calls = st["n_apply_calls"]
-    # roofline of the dominant kernel: live HIP-event time of the solve's applies
-    avg_apply = st["seconds_apply_gpu"] / max(1, apply_calls)
-    prec_bytes = 4 if args.mode == "mixed" else 8
-    bytes_per_launch = 2 * (args.N ** 3) * prec_bytes
-    achieved = bytes_per_launch / avg_apply / 1e9 if args.mode == "f64" else gbs32
-    roof = {"bound": "hbm", "achieved": round(gbs64 if args.mode == "f64" else gbs32, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round((gbs64 if args.mode == "f64" else gbs32) / HBM_PEAK_GBS, 4),
-            "traffic": None, "kernel": "k_stencil", "in_solve_avg_apply_us": round(avg_apply * 1e6, 1),
-            "in_solve_achieved_gbs": round(achieved, 1)}
+    # roofline of the solve's apply kernel (k_stencil_p): HIP-event time of every
+    # apply on the library's stream over the timed solve, per column.  Nearly all
+    # are Chebyshev steps y = a Hx + b x + g z: algorithmic bytes 3 N s (read x and
+    # z, write y); the explicit check's block apply and the acceptance residuals
+    # are a few dozen columns of 2 N s, counted at the same rate.
+    cols = st["n_matvec"] + m
+    avg_col = st["seconds_apply_gpu"] / max(1, cols)
+    s_bytes = 8 if args.mode == "f64" else 4
+    bytes_per_launch = 3 * (args.N ** 3) * s_bytes
+    achieved = bytes_per_launch / avg_col / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"stencil_traffic_{args.N}.json")
+    if os.path.exists(tf) and args.mode == "f64":
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_stencil_p",
+            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_col * 1e6, 1)}
     extra = {}
     if not args.skip_plain and args.mode != "f64":
         _, E64, st64 = step("f64")

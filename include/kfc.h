@@ -161,7 +161,11 @@ int kfc_lanczos(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t ma
  * precond: 0 = :ritz, 1 = :jacobi, 2 = :none; seed_kind: 0 = :product, 1 = :random.
  * X0: NULL or host prod(dims) x m (the warm start, src/materialize.jl:656-666). */
 enum kfc_precond { KFC_PRECOND_RITZ = 0, KFC_PRECOND_JACOBI = 1, KFC_PRECOND_NONE = 2 };
-enum kfc_seed { KFC_SEED_PRODUCT = 0, KFC_SEED_RANDOM = 1 };
+/* seed = :product / :random (src/tree.jl:66-87).  KFC_SEED_WARM_GUARDS (no
+ * counterpart in the reference) says X0 holds m + g columns, the guard columns
+ * included, instead of m columns with product-state guards: a learned warm
+ * start supplies all the columns the solver converges. */
+enum kfc_seed { KFC_SEED_PRODUCT = 0, KFC_SEED_RANDOM = 1, KFC_SEED_WARM_GUARDS = 2 };
 int kfc_lobpcg(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond,
                double eta, int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode,
                double* E, double* V, kfc_stats* stats);

@@ -635,6 +635,15 @@ __global__ __launch_bounds__(256) void k_stencil_q(const StArgs<S> a) {
     if (z + 2 < z1) emit(qc, qd, qa, z + 2);
 }
 
+// Streaming store of an output point (the output is not read again by this kernel).
+__device__ inline void store_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ inline void store_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
+template <class R>
+__device__ inline void store_stream(Cx<R>* p, Cx<R> v) {
+    __builtin_nontemporal_store(v.re, &p->re);
+    __builtin_nontemporal_store(v.im, &p->im);
+}
+
 // Plane-sweep variant for constant stencil weights: a block owns a 64 x 16 patch of
 // (axis 0, axis 1) and marches along axis 2.  Each plane of the patch (with a
 // one-point halo) is read from HBM once into LDS; every thread then takes the
@@ -778,7 +787,7 @@ __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
                     const int64_t idx = (int64_t)i0 + (int64_t)y * a.s1 + zo;
                     S out = s_fma(s_mul(a.alpha, accM[r]), a.beta, ctrM[r]);
                     if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
-                    Yc[idx] = out;
+                    store_stream(Yc + idx, out);
                 }
             }
         }

@@ -143,15 +143,31 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const S* __restrict__ Am, int
     for (int i = 0; i < TP; ++i)
 #pragma unroll
         for (int j = 0; j < TQ; ++j) acc[i][j] = gd4{0, 0, 0, 0};
-    for (int64_t rb = r0; rb < r1; rb += 64) {
+    // the next slab is fetched into registers while the current one is contracted
+    double pa[TP * 4], pb[TQ * 4];
+    auto fetch = [&](int64_t rb) {
         const int64_t row = rb + lane;
         const bool rok = row < r1;
+#pragma unroll
+        for (int j = 0; j < TP * 4; ++j) {
+            const int c = wv + 4 * j;
+            pa[j] = (rok && c < p) ? double(Am[row + c * lda]) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < TQ * 4; ++j) {
+            const int c = wv + 4 * j;
+            pb[j] = (rok && c < q) ? double(Bm[row + c * ldb]) : 0.0;
+        }
+    };
+    if (r0 < r1) fetch(r0);
+    for (int64_t rb = r0; rb < r1; rb += 64) {
         __syncthreads();   // the previous slab has been consumed
 #pragma unroll
-        for (int c = wv; c < TP * 16; c += 4) sA[c * LDR + lane] = (rok && c < p) ? double(Am[row + c * lda]) : 0.0;
+        for (int j = 0; j < TP * 4; ++j) sA[(wv + 4 * j) * LDR + lane] = pa[j];
 #pragma unroll
-        for (int c = wv; c < TQ * 16; c += 4) sB[c * LDR + lane] = (rok && c < q) ? double(Bm[row + c * ldb]) : 0.0;
+        for (int j = 0; j < TQ * 4; ++j) sB[(wv + 4 * j) * LDR + lane] = pb[j];
         __syncthreads();
+        if (rb + 64 < r1) fetch(rb + 64);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int rr = 4 * (wv + 4 * u) + (lane >> 4);   // slab row of this lane's k index
@@ -249,12 +265,28 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
     gd4 g[TPZ];
 #pragma unroll
     for (int i = 0; i < TPZ; ++i) g[i] = gd4{0, 0, 0, 0};
-    for (int64_t rb = r0; rb < r1; rb += 64) {
+    // the next slab is fetched into registers while the current one is contracted
+    constexpr int NPF = 20;   // ppad <= 80 columns over 4 waves
+    double pf[NPF];
+    auto fetch = [&](int64_t rb) {
         const int64_t row = rb + lane;
         const bool rok = row < r1;
+#pragma unroll
+        for (int j = 0; j < NPF; ++j) {
+            const int c = wv + 4 * j;
+            pf[j] = (rok && c < p) ? double(X[row + c * ldx]) : 0.0;
+        }
+    };
+    if (r0 < r1) fetch(r0);
+    for (int64_t rb = r0; rb < r1; rb += 64) {
         __syncthreads();   // the previous slab's readers are done (and sM is in place)
-        for (int c = wv; c < ppad; c += 4) sX[c * TS_LDR + lane] = (rok && c < p) ? double(X[row + c * ldx]) : 0.0;
+#pragma unroll
+        for (int j = 0; j < NPF; ++j) {
+            const int c = wv + 4 * j;
+            if (c < ppad) sX[c * TS_LDR + lane] = pf[j];
+        }
         __syncthreads();
+        if (rb + 64 < r1) fetch(rb + 64);
         gd4 acc[TQ];
 #pragma unroll
         for (int j = 0; j < TQ; ++j) acc[j] = gd4{0, 0, 0, 0};

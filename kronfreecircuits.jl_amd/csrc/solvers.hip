@@ -1047,20 +1047,24 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
     if (precond == KFC_PRECOND_RITZ && !(eta > 0 && std::isfinite(eta)))
         return kfc_fail(op->ctx, KFC_EARG, "eta must be positive and finite");
     if (guard < 0) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: guard must be nonnegative");
+    if (seed_kind == KFC_SEED_WARM_GUARDS && !X0)
+        return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: the warm-guards seed needs an X0 block");
+    const int g = int(std::max<int64_t>(0, std::min<int64_t>(guard, (N - 5) / 3 - m)));   // src/lobpcg.jl:243
+    const int W = int(m) + g;
+    // X0 holds m columns, or m + g (the guards too) with the warm-guards seed
+    const int64_t nx0 = seed_kind == KFC_SEED_WARM_GUARDS ? W : m;
     std::vector<cd> X0h;
     if (X0) {
-        read_host_V(op, X0, size_t(N) * m, X0h);
+        read_host_V(op, X0, size_t(N) * nx0, X0h);
         for (const cd& x : X0h)
             if (!std::isfinite(x.real()) || !std::isfinite(x.imag()))
                 return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: X0 has non-finite entries");
-        for (int64_t j = 0; j < m; ++j) {
+        for (int64_t j = 0; j < nx0; ++j) {
             bool z = true;
             for (int64_t i = 0; i < N; ++i) z &= X0h[i + j * N] == cd(0);
             if (z) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: column " + std::to_string(j + 1) + " of X0 is zero");
         }
     }
-    const int g = int(std::max<int64_t>(0, std::min<int64_t>(guard, (N - 5) / 3 - m)));   // src/lobpcg.jl:243
-    const int W = int(m) + g;
     std::vector<cd> P;
     if (seed_kind == KFC_SEED_PRODUCT) {
         int rc = op_product_seed(op, W, P);
@@ -1094,7 +1098,10 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
             random_block(X, N, W, op->cplx, s);
             return X;
         }
-        if (X0) {
+        if (X0 && nx0 == W) {   // the warm start carries its guards
+            X = X0h;
+            perturb(X, N, W, op->cplx, 0x4b46);
+        } else if (X0) {
             std::vector<cd> F = X0h;
             perturb(F, N, int(m), op->cplx, 0x4b46);
             std::copy(F.begin(), F.end(), X.begin());

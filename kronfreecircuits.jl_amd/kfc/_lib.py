@@ -16,7 +16,7 @@ KFC_PREC_F64, KFC_PREC_F32 = 0, 1
 KFC_MODE_F64, KFC_MODE_MIXED = 0, 1
 KFC_DIAG, KFC_BANDED, KFC_SHIFT, KFC_DENSE = 0, 1, 2, 3
 KFC_PRECOND = {"ritz": 0, "jacobi": 1, "none": 2}
-KFC_SEED = {"product": 0, "random": 1}
+KFC_SEED = {"product": 0, "random": 1, "warm_guards": 2}
 
 
 class ArgumentError(ValueError):

@@ -144,9 +144,13 @@ def solve_stats(s, H, m, X0=None):
                              C.byref(st))
     elif isinstance(s, LOBPCG):
         x0 = None
+        seed_kind = L.KFC_SEED[s.seed]
         if X0 is not None:
             X0 = np.asarray(X0)
-            if X0.shape != (H.N, m):
+            w = m + guard_count(s, H.N, m)
+            if X0.shape == (H.N, w) and w > m:
+                seed_kind = L.KFC_SEED["warm_guards"]   # the warm start carries the guard columns
+            elif X0.shape != (H.N, m):
                 raise L.DimensionMismatch(f"LOBPCG: X0 has size {X0.shape}, expected ({H.N}, {m})")
             if H.dtype == np.float64 and np.iscomplexobj(X0):
                 if np.any(np.imag(X0) != 0):
@@ -155,7 +159,7 @@ def solve_stats(s, H, m, X0=None):
                 X0 = np.real(X0)
             x0 = pack_block(X0, H.dtype)
         rc = lib.kfc_lobpcg(H.handle, int(m), int(s.guard), float(s.tol), int(s.maxiter),
-                            L.KFC_PRECOND[s.preconditioner], float(s.eta), L.KFC_SEED[s.seed],
+                            L.KFC_PRECOND[s.preconditioner], float(s.eta), seed_kind,
                             x0.ctypes.data_as(L._D) if x0 is not None else None, int(s.rng_seed), _mode(s.mode),
                             E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
     elif isinstance(s, ChebyshevFilter):
@@ -169,6 +173,11 @@ def solve_stats(s, H, m, X0=None):
         raise L.ArgumentError(f"no device method for solver {type(s).__name__}")
     L.check(rc, H.ctx.handle)
     return E, unpack_block(V, H.N, m, H.dtype), st.as_dict()
+
+
+def guard_count(s, N, m):
+    """Guard columns of a LOBPCG solve: min(guard, (N - 5) / 3 - m) (src/lobpcg.jl:243)."""
+    return max(0, min(int(s.guard), (int(N) - 5) // 3 - int(m)))
 
 
 def solve(s, H, m, X0=None):
@@ -200,18 +209,31 @@ def solve_batch(s, Hs, m, X0s=None):
     Vb = [np.empty(H.N * m * (2 if H.dtype == np.complex128 else 1)) for H in Hs]
     Vp = (L._D * n)(*[v.ctypes.data_as(L._D) for v in Vb])
     x0p, keep = None, []
+    seed_kind = L.KFC_SEED[s.seed]
     if X0s is not None:
         x0p = (L._D * n)()
+        widths = set()
         for i, (H, X0) in enumerate(zip(Hs, X0s)):
             if X0 is None:
                 x0p[i] = L._D()
             else:
-                b = pack_block(np.asarray(X0), H.dtype)
+                X0 = np.asarray(X0)
+                w = m + guard_count(s, H.N, m)
+                if X0.shape not in ((H.N, m), (H.N, w)):
+                    raise L.DimensionMismatch(f"LOBPCG: X0 of instance {i} has size {X0.shape}, expected ({H.N}, {m})")
+                widths.add(X0.shape[1] > m)
+                b = pack_block(X0, H.dtype)
                 keep.append(b)
                 x0p[i] = b.ctypes.data_as(L._D)
+        if len(widths) > 1:
+            raise L.ArgumentError("solve_batch: every X0 carries the guard columns or none does")
+        if widths == {True}:
+            if any(X0 is None for X0 in X0s):
+                raise L.ArgumentError("solve_batch: warm starts with guard columns need an X0 for every instance")
+            seed_kind = L.KFC_SEED["warm_guards"]
     st = (L.Stats * n)()
     L.check(L.lib().kfc_lobpcg_batch(n, ops, int(m), int(s.guard), float(s.tol), int(s.maxiter),
-                                     L.KFC_PRECOND[s.preconditioner], float(s.eta), L.KFC_SEED[s.seed], x0p,
+                                     L.KFC_PRECOND[s.preconditioner], float(s.eta), seed_kind, x0p,
                                      int(s.rng_seed), _mode(s.mode), E.ctypes.data_as(L._D), Vp, st),
             Hs[0].ctx.handle)
     out = [(E[i * m:(i + 1) * m].copy(), unpack_block(Vb[i], Hs[i].N, m, Hs[i].dtype)) for i in range(n)]

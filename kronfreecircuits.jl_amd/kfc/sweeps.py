@@ -47,8 +47,9 @@ def features(theta):
 
 
 class ZeroPiSweep:
-    def __init__(self, ncut=40, Nphi=256, Nzeta=30, keep=40, m=16, contexts=4):
+    def __init__(self, ncut=40, Nphi=256, Nzeta=30, keep=40, m=16, contexts=4, guard=4):
         self.ncut, self.Nphi, self.Nzeta, self.keep, self.m = ncut, Nphi, Nzeta, keep, m
+        self.guard = guard
         self.ctxs = [L.context(0)] + [L.Context(0) for _ in range(max(0, contexts - 1))]
         self._phi = None
 
@@ -108,8 +109,13 @@ class ZeroPiSweep:
             out.append((E, V, st, kids))
         return out, time.perf_counter() - t0
 
-    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, chunk=None):
-        """All points' parent solves in flight on the contexts (round-robin)."""
+    def width(self, N):
+        """Columns LOBPCG converges on a parent of N states: m plus the guards."""
+        return self.m + max(0, min(self.guard, (N - 5) // 3 - self.m))
+
+    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, nev=None):
+        """All points' parent solves in flight on the contexts (round-robin);
+        ``nev`` pairs per point (default m)."""
         t0 = time.perf_counter()
         Hs, kidss = [], []
         for i, th in enumerate(thetas):
@@ -119,25 +125,29 @@ class ZeroPiSweep:
         X0s = None
         if x0_fn is not None:
             X0s = x0_fn(thetas, Hs, kidss)
-        res, st = solve_batch(LOBPCG(tol=tol, mode=mode), Hs, self.m, X0s)
+        res, st = solve_batch(LOBPCG(tol=tol, mode=mode, guard=self.guard), Hs, nev or self.m, X0s)
         wall = time.perf_counter() - t0
         return [(E, V, s, k) for (E, V), s, k in zip(res, st, kidss)], wall
 
 
 class WarmStart:
     """The learned warm start: a fixed 64-vector basis of the canonical frame and
-    the network mapping parameters to the 64 x 16 coefficients on it."""
+    the network mapping parameters to the coefficients on it of the lowest m + g
+    parent eigenvectors (LOBPCG converges the g guard columns too, so a start
+    that leaves them to the product seed gains nothing: 64 x 20 outputs for the
+    zero-pi parent's 16 + 4)."""
 
-    def __init__(self, sweep, nbasis=64, hidden=256, seed=0):
+    def __init__(self, sweep, nbasis=64, hidden=256, seed=0, width=None):
         from .mlp import MLP
-        self.sw, self.nb, self.m = sweep, nbasis, sweep.m
-        self.net = MLP((6, hidden, hidden, nbasis * sweep.m), seed=seed)
+        self.sw, self.nb = sweep, nbasis
+        self.m = width or sweep.m + sweep.guard
+        self.net = MLP((6, hidden, hidden, nbasis * self.m), seed=seed)
         self.B = None
         self.signref = None
 
     def fit_basis(self, frames):
         """B: the leading principal directions of the stacked training eigenvectors."""
-        Z = np.concatenate(frames, axis=1)
+        Z = np.concatenate([f[:, :self.m] for f in frames], axis=1)
         G = Z.T @ Z
         w, U = np.linalg.eigh(G)
         top = np.argsort(w)[::-1][:self.nb]
@@ -146,7 +156,7 @@ class WarmStart:
         self.B = Q
 
     def coeffs(self, Vf):
-        C = self.B.T @ Vf                       # nb x m
+        C = self.B.T @ Vf[:, :self.m]           # nb x m
         if self.signref is None:
             self.signref = np.argmax(np.abs(C), axis=0)
         s = np.sign(C[self.signref, np.arange(C.shape[1])])
@@ -170,7 +180,7 @@ class WarmStart:
     def x0_fn(self, thetas, Hs, kidss):
         out = []
         for Xf, H, kids in zip(self.predict_frames(thetas), Hs, kidss):
-            X0 = self.sw.from_frame(Xf, self.sw.frame_index(kids), H.N)
+            X0 = self.sw.from_frame(Xf, self.sw.frame_index(kids), H.N)[:, :self.sw.width(H.N)]
             bad = np.linalg.norm(X0, axis=0) == 0
             if np.any(bad):
                 X0[:, bad] = np.random.default_rng(0).standard_normal((H.N, int(bad.sum())))

@@ -11,11 +11,16 @@ solve, and its energies are the ones it returns.
 
 Two ways to run a batch:
 
-* plain: instance after instance, every leaf by FP64 LOBPCG from the product
-  seed, the top Dense (what materialize does per point on the device);
-* batched: the leaves of all instances in flight together on several device
-  contexts (kfc_lobpcg_batch) in FP32 with FP64 refinement (mode="mixed"),
-  then the tops."""
+* plain: instance after instance, every leaf by FP64 Lanczos, the top Dense
+  (what materialize does per point on the device);
+* batched: the leaves of all instances in flight together, one host thread per
+  device context, then the tops.
+
+The leaves are solved by Lanczos: a periodic grid of N points with an order-8
+stencil has a kinetic spectrum reaching 4 E_C (N/2)^2 ~ 1e5-1e6 GHz, where
+LOBPCG with a diagonal preconditioner (nearly constant here) needs thousands of
+iterations while a Krylov method needs a few hundred matvecs.  The leaf
+tolerance is 1e-9, above the rounding floor 10 eps |H| of these spectra."""
 from __future__ import annotations
 
 import time
@@ -25,7 +30,9 @@ import numpy as np
 from . import _lib as L
 from .bases import PeriodicGrid
 from .operator import KronFreeOperator
-from .solvers import LOBPCG, Dense, solve_batch, solve_stats
+from concurrent.futures import ThreadPoolExecutor
+
+from .solvers import Dense, Lanczos, solve_stats
 from .tree import (MaterializedNode, Node, circuit_terms, constant_term, fix_phases, materialize, node_axisterms,
                    operator, positive_mask, route)
 from .configs import two_transmons_spec
@@ -64,19 +71,19 @@ class TransmonSweep:
         self.order, self.k_leaf, self.k_top = order, k_leaf, k_top
         self.ctxs = [L.context(0)] + [L.Context(0) for _ in range(max(0, contexts - 1))]
 
-    def run_plain(self, insts, tol=1e-8):
-        """Instance after instance: materialize with FP64 LOBPCG leaves and a Dense top."""
+    def run_plain(self, insts, tol=1e-9, mode="f64"):
+        """Instance after instance: materialize with Lanczos leaves and a Dense top."""
         out, t0 = [], time.perf_counter()
         for inst in insts:
-            root = tree_of(inst, self.order, self.k_leaf, self.k_top, LOBPCG(tol=tol, mode="f64"))
-            sysm = materialize(root, spec_of(inst))
-            out.append(sysm.root)
+            root = tree_of(inst, self.order, self.k_leaf, self.k_top, Lanczos(tol=tol, mode=mode))
+            out.append(materialize(root, spec_of(inst)).root)
         return out, time.perf_counter() - t0
 
-    def run_batched(self, insts, mode="mixed", tol=1e-8):
-        """All leaves of all instances solved together, then every top."""
+    def run_batched(self, insts, mode="f64", tol=1e-9):
+        """All leaves of all instances solved together (one host thread per device
+        context, each working through its share), then every top."""
         t0 = time.perf_counter()
-        roots, specs, routed, leaf_ops, leaf_nodes = [], [], [], [], []
+        roots, specs, leaf_ops, leaf_nodes = [], [], [], []
         c = 0
         for inst in insts:
             root = tree_of(inst, self.order, self.k_leaf, self.k_top)
@@ -84,26 +91,33 @@ class TransmonSweep:
             rt = route(root, circuit_terms(spec))
             roots.append(root)
             specs.append(spec)
-            routed.append(rt)
             for leaf in root.children:
                 dims = [leaf.children[0][1].dim]
                 ts = node_axisterms(leaf, [None], rt.get(id(leaf), []), leaf.real)
-                H = KronFreeOperator(dims, ts, constant=0.0, block_capacity=self.k_leaf + 4,
-                                     ctx=self.ctxs[c % len(self.ctxs)])
+                H = KronFreeOperator(dims, ts, constant=0.0, ctx=self.ctxs[c % len(self.ctxs)])
                 c += 1
                 leaf_ops.append(H)
                 leaf_nodes.append(leaf)
-        res, st = solve_batch(LOBPCG(tol=tol, mode=mode), leaf_ops, self.k_leaf)
+        nctx = len(self.ctxs)
+        solver = Lanczos(tol=tol, mode=mode)
+
+        def work(k):   # every nctx-th leaf: the ones bound to context k
+            return [(i, solve_stats(solver, leaf_ops[i], self.k_leaf)) for i in range(k, len(leaf_ops), nctx)]
+        results = [None] * len(leaf_ops)
+        with ThreadPoolExecutor(max_workers=nctx) as ex:
+            for part in ex.map(work, range(nctx)):
+                for i, r in part:
+                    results[i] = r
         t_leaves = time.perf_counter() - t0
         mats = []
-        for (E, V), s, H, leaf in zip(res, st, leaf_ops, leaf_nodes):
+        for (E, V, st), H, leaf in zip(results, leaf_ops, leaf_nodes):
             V = fix_phases(np.asarray(V), positive_mask(leaf, H.dims))
-            mats.append(MaterializedNode(leaf, list(H.dims), np.asarray(E), V, [None], H, None, s))
+            mats.append(MaterializedNode(leaf, list(H.dims), np.asarray(E), V, [None], H, solver, st))
         tops = []
         for i, (root, spec) in enumerate(zip(roots, specs)):
             kids = mats[2 * i:2 * i + 2]
             Ht, _ = operator(root, spec, kids=kids)
-            E, V, s = solve_stats(Dense(), Ht, self.k_top)
+            E, V, st = solve_stats(Dense(), Ht, self.k_top)
             tops.append(MaterializedNode(root, list(Ht.dims), np.asarray(E)[:self.k_top],
-                                         np.asarray(V)[:, :self.k_top], kids, Ht, Dense(), s))
+                                         np.asarray(V)[:, :self.k_top], kids, Ht, Dense(), st))
         return tops, time.perf_counter() - t0, t_leaves
