@@ -78,6 +78,20 @@ def cpu_baseline(theta, N, n_matvec_gpu, m):
             "seconds_per_apply": round(t, 4)}
 
 
+def calibrate_copy(gib=1, reps=5):
+    """y = x + 0 over `gib` GiB of FP64, `reps` times: each launch reads and writes
+    exactly gib GiB, the yardstick that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = gib * (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "elementwise copy", "bytes_read_per_launch": n * 8,
+                      "bytes_written_per_launch": n * 8, "launches": reps}))
+
+
 def sweep_bench(args):
     """Zero-pi parameter sweep (parent dims [40, 40, 30], LOBPCG lowest 16, tol 1e-8):
     plain FP64 point after point against batched solves with and without the
@@ -207,6 +221,8 @@ def main():
     ap.add_argument("--workload", default="three_fluxonia",
                     choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep"])
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only an elementwise copy of known size (the reference for HBM byte counters)")
     ap.add_argument("--contexts", type=int, default=8, help="device contexts (streams) of a batched sweep")
     ap.add_argument("--test-points", type=int, default=64)
     ap.add_argument("--plain-points", type=int, default=8)
@@ -218,6 +234,8 @@ def main():
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.workload == "zero_pi_sweep":
         return sweep_bench(args)
     if args.workload == "transmon_sweep":

@@ -247,7 +247,7 @@ void gram_mfma(int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const
 // be columns of X: a slab is read completely before any of its rows is written.
 constexpr int TS_LDR = 65;
 
-template <class S, int TQ, int GR, int TPZ>
+template <class S, int TQ, int GR, int TPZ, int NPF>
 __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t ldx, int p, int ppad,
                                               const double* __restrict__ M, int q, S* Y, int64_t ldy, int64_t N,
                                               int64_t rows_per, int pz, double* __restrict__ part) {
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
 #pragma unroll
     for (int i = 0; i < TPZ; ++i) g[i] = gd4{0, 0, 0, 0};
     // the next slab is fetched into registers while the current one is contracted
-    constexpr int NPF = 20;   // ppad <= 80 columns over 4 waves
+    // (NPF >= ppad / 4 columns per wave)
     double pf[NPF];
     auto fetch = [&](int64_t rb) {
         const int64_t row = rb + lane;
@@ -351,8 +351,19 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
 template <class S, int TQ, int GR, int TPZ>
 void launch_tsmm(int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p, int ppad, const double* M,
                  int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz, double* part) {
-    hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N,
-                       rows_per, pz, part);
+    // prefetch registers sized to the block width: fewer registers, more waves
+    if (ppad <= 16)
+        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 4>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
+                           ldy, N, rows_per, pz, part);
+    else if (ppad <= 32)
+        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 8>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
+                           ldy, N, rows_per, pz, part);
+    else if (ppad <= 48)
+        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 12>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
+                           ldy, N, rows_per, pz, part);
+    else
+        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 20>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
+                           ldy, N, rows_per, pz, part);
 }
 
 template <class S>

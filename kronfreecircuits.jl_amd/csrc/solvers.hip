@@ -29,8 +29,13 @@ template <class S>
 int upload_block(kfc_ctx* ctx, const std::vector<cd>& H, int64_t N, int p, S* D, int64_t ld) {
     std::vector<S> h(size_t(N) * p);
     for (size_t i = 0; i < h.size(); ++i) h[i] = s_from<S>(H[i]);
-    KFC_HIP(ctx, hipMemcpy2DAsync(D, ld * sizeof(S), h.data(), N * sizeof(S), N * sizeof(S), p, hipMemcpyHostToDevice,
-                                  ctx->stream));
+    if (ld == N) {
+        KFC_HIP(ctx, hipMemcpyAsync(D, h.data(), sizeof(S) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        for (int j = 0; j < p; ++j)
+            KFC_HIP(ctx, hipMemcpyAsync(D + size_t(j) * ld, h.data() + size_t(j) * N, sizeof(S) * N,
+                                        hipMemcpyHostToDevice, ctx->stream));
+    }
     KFC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return KFC_OK;
 }
@@ -38,8 +43,13 @@ int upload_block(kfc_ctx* ctx, const std::vector<cd>& H, int64_t N, int p, S* D,
 template <class S>
 int download_block(kfc_ctx* ctx, const S* D, int64_t ld, int64_t N, int p, std::vector<cd>& H) {
     std::vector<S> h(size_t(N) * p);
-    KFC_HIP(ctx, hipMemcpy2DAsync(h.data(), N * sizeof(S), D, ld * sizeof(S), N * sizeof(S), p, hipMemcpyDeviceToHost,
-                                  ctx->stream));
+    if (ld == N) {
+        KFC_HIP(ctx, hipMemcpyAsync(h.data(), D, sizeof(S) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        for (int j = 0; j < p; ++j)
+            KFC_HIP(ctx, hipMemcpyAsync(h.data() + size_t(j) * N, D + size_t(j) * ld, sizeof(S) * N,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+    }
     KFC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     H.resize(h.size());
     for (size_t i = 0; i < h.size(); ++i) {
@@ -142,6 +152,18 @@ int apply_timed(kfc_op* op, const S* X, S* Y, int64_t ncols, int64_t ld, Timer& 
     if (prec_of<S>() == 1) t.cols32 += ncols;
     t.calls += 1;
     return KFC_OK;
+}
+
+// Fresh FP64 true residuals of pairs whose vectors are on the device (N x m, ld N).
+template <class S>
+int true_residuals_dev(kfc_op* op, Work& w, const std::vector<double>& E, const S* X, int m, std::vector<double>& res,
+                       Timer& t) {
+    const int64_t N = op->N;
+    DevBuf by;
+    KFC_HIP(op->ctx, by.ensure(sizeof(S) * size_t(N) * m));
+    int rc = apply_timed<S>(op, X, by.as<S>(), m, N, t);
+    if (rc == KFC_OK) rc = blas_residual<S>(w, by.as<S>(), N, by.as<S>(), N, X, N, m, N, E, res);
+    return rc;
 }
 
 // Fresh FP64 true residuals of host pairs.
@@ -665,15 +687,14 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
                         "Lanczos: only " + std::to_string(std::min(out.converged, int(m))) + " of " + std::to_string(m) +
                             " eigenpairs converged (normres = " +
                             std::to_string(out.resid.empty() ? 0.0 : out.resid[std::min<int64_t>(m, n) - 1]) + ")");
-    std::vector<cd> Vh;
-    rc = download_block<S>(op->ctx, res64.as<S>(), N, N, int(m), Vh);
-    if (rc != KFC_OK) return rc;
+    // the FP64 (or complex FP64) result block has the caller's host layout: one copy
+    KFC_HIP(op->ctx, hipMemcpyAsync(Vout, res64.p, sizeof(S) * size_t(N) * m, hipMemcpyDeviceToHost, op->ctx->stream));
+    KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
     std::vector<double> E(out.theta.begin(), out.theta.begin() + m);
     for (int64_t j = 0; j < m; ++j) Eout[j] = E[j];
-    write_host_V(op, Vh, Vout);
     if (st) {
         std::vector<double> res;
-        rc = true_residuals<S>(op, w, E, Vh, int(m), res, tm);
+        rc = true_residuals_dev<S>(op, w, E, res64.as<S>(), int(m), res, tm);
         if (rc != KFC_OK) return rc;
         *st = kfc_stats{};
         st->max_residual = *std::max_element(res.begin(), res.end());
@@ -955,34 +976,51 @@ int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>
             if (rc != KFC_OK) return rc;
             GBu = g2;
         }
-        auto ip = [&](const cd* a, const cd* b) {
-            cd s = 0;
+        // GB-products of the vectors the P columns are orthogonalized against, so
+        // that each inner product a^H GB v costs O(nsu) (u = GB v is updated along)
+        auto gbmul = [&](const cd* x, std::vector<cd>& y) {
+            y.assign(nsu, cd(0));
             for (int j = 0; j < nsu; ++j) {
-                cd t = 0;
-                for (int i = 0; i < nsu; ++i) t += std::conj(a[i]) * GBu[i + size_t(j) * nsu];
-                s += t * b[j];
+                const cd xj = x[j];
+                if (xj == cd(0)) continue;
+                for (int i = 0; i < nsu; ++i) y[i] += GBu[i + size_t(j) * nsu] * xj;
             }
-            return s;
         };
+        auto dot = [&](const cd* a, const cd* b) {
+            cd t = 0;
+            for (int i = 0; i < nsu; ++i) t += std::conj(a[i]) * b[i];
+            return t;
+        };
+        std::vector<std::vector<cd>> GX(W), gpcols;
+        for (int q = 0; q < W; ++q) gbmul(&C[size_t(q) * nsu], GX[q]);
         int npn = 0;
         for (int c = nlock; c < W; ++c) {
-            std::vector<cd> v(nsu, cd(0));
+            std::vector<cd> v(nsu, cd(0)), u;
             for (int i = W; i < nsu; ++i) v[i] = C[i + size_t(c) * nsu];
+            gbmul(v.data(), u);
             for (int pass = 0; pass < 2; ++pass) {
                 for (int q = 0; q < W; ++q) {
                     const cd* x = &C[size_t(q) * nsu];
-                    const cd s = ip(x, v.data());
-                    for (int i = 0; i < nsu; ++i) v[i] -= s * x[i];
+                    const cd sx = dot(x, u.data());
+                    for (int i = 0; i < nsu; ++i) {
+                        v[i] -= sx * x[i];
+                        u[i] -= sx * GX[q][i];
+                    }
                 }
-                for (auto& pc : pcols) {
-                    const cd s = ip(pc.data(), v.data());
-                    for (int i = 0; i < nsu; ++i) v[i] -= s * pc[i];
+                for (size_t k = 0; k < pcols.size(); ++k) {
+                    const cd sx = dot(pcols[k].data(), u.data());
+                    for (int i = 0; i < nsu; ++i) {
+                        v[i] -= sx * pcols[k][i];
+                        u[i] -= sx * gpcols[k][i];
+                    }
                 }
             }
-            const double nv = std::sqrt(std::max(ip(v.data(), v.data()).real(), 0.0));
+            const double nv = std::sqrt(std::max(dot(v.data(), u.data()).real(), 0.0));
             if (nv < 1e-10) continue;
             for (auto& x : v) x /= nv;
+            for (auto& x : u) x /= nv;
             pcols.push_back(v);
+            gpcols.push_back(u);
             ++npn;
         }
         for (int c = 0; c < npn; ++c)
