@@ -8,7 +8,7 @@ tree materialization with Dense solves).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
 it, and only as the checker: no product code path calls it.
 
-Pinning (what makes each part trustworthy, see tests/test_oracle_*.py):
+Pinning (what makes each part trustworthy; the checks are in tests/test_oracle.py):
   * fd weights: exact rationals and moment identities (test/test_bases.jl:6, 47-54)
   * basis operators: closed forms (Fock ladders, charge shifts, Gaussian
     derivative accuracy, plane-wave symbols of the cyclic stencils)
