@@ -93,16 +93,5 @@ def zero_pi_tree(E_Cz, E_L=0.05, ncut=40, Nphi=256, Nzeta=30, keep=40, m=16, sol
     ze = Node([("zeta", FockBasis(Nzeta, (2 * E_Cz / E_L) ** 0.25))], keep=min(keep, Nzeta))
     return Node([th, ph, ze], keep=m, compute=m, solver=solver or LOBPCG())
 
-
-def periodic_transmons_theta(seed, n_inst):
-    """Batched two-transmon sweep: grid sizes from {256, 384, 512, 640, 768} and
-    E_J ~ U[20, 35], E_C ~ U[0.8, 1.4], n_g ~ U[0, 0.5], g ~ U[0.2, 0.4]."""
-    r = np.random.default_rng(seed)
-    sizes = np.array([256, 384, 512, 640, 768])
-    return dict(N1=r.choice(sizes, n_inst), N2=r.choice(sizes, n_inst), E_J1=r.uniform(20, 35, n_inst),
-                E_C1=r.uniform(0.8, 1.4, n_inst), n_g1=r.uniform(0, 0.5, n_inst), E_J2=r.uniform(20, 35, n_inst),
-                E_C2=r.uniform(0.8, 1.4, n_inst), n_g2=r.uniform(0, 0.5, n_inst), g=r.uniform(0.2, 0.4, n_inst))
-
-
 def periodic_transmon_leaf(N, order=8):
     return PeriodicGrid(N, order)
