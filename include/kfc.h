@@ -128,6 +128,12 @@ int kfc_op_fused(const kfc_op* op);
  * times the candidates on the first application of each column count. */
 int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march);
 int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march);
+/* Algorithmic work of one column of kfc_apply on the path the operator takes
+ * (no counterpart in the reference; the roofline denominators): *bytes = HBM
+ * bytes that must move (read x, write y, plus the stored diagonal when the path
+ * reads it), *flops = floating-point operations (2 per multiply-add; the dense
+ * kept-basis pairs count 2 N (d_p + d_q) per merged term). */
+int kfc_op_work(const kfc_op* op, double* bytes, double* flops);
 
 /* Replaces LinearAlgebra.mul!(y, H, x) (src/apply.jl:416-432):
  * Y = H X for ncols columns with leading dimension ld (>= prod(dims)).

@@ -165,6 +165,27 @@ int kfc_op_destroy(kfc_op* op) {
 int64_t kfc_op_size(const kfc_op* op) { return op ? op->N : 0; }
 int kfc_op_fused(const kfc_op* op) { return op && op->plan.ok ? 1 : 0; }
 
+int kfc_op_work(const kfc_op* op, double* bytes, double* flops) {
+    if (!op || !bytes || !flops) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_op_work: null argument");
+    const double N = double(op->N), s = op->cplx ? 16.0 : 8.0;
+    double f = 0;
+    if (op->plan.ok) {
+        int used = 0;
+        for (int p = 0; p < 27; ++p) used += op->plan.pos[p].used ? 1 : 0;
+        f = 2.0 * N * (used + 1);   // one multiply-add per stencil position and the diagonal
+        *bytes = 2.0 * N * s + (op->plan.need_full_diag ? N * s : 0.0);
+    } else if (op->pairs.ok) {
+        for (const PairGroup& g : op->pairs.groups) f += double(g.c.size()) * 2.0 * N * double(g.dp + g.dq);
+        f += 2.0 * N;   // the diagonal
+        *bytes = 3.0 * N * s;   // x, y and the stored diagonal
+    } else {
+        for (const Term& t : op->terms) f += 2.0 * N * double(t.factors.size());
+        *bytes = (2.0 + 2.0 * double(op->terms.size())) * N * s;
+    }
+    *flops = op->cplx ? 4.0 * f : f;
+    return KFC_OK;
+}
+
 int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march) {
     if (!op || !variant || !march) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_op_stencil_get: null argument");
     *variant = op->stencil_variant;

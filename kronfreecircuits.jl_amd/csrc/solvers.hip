@@ -394,6 +394,8 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             if (rc != KFC_OK) return rc;
         }
         const int nb0 = s + 1;
+        std::vector<cd> pend_ri;   // a pending K <- K pend_ri (pend_n columns), see the Cholesky QR
+        int pend_n = 0;
         // block classical Gram-Schmidt against Q, twice (K <- K - Q Cm), the second
         // and third reads of [Q K] fused with the next Gram: C1 = Q^H K;
         // K -= Q C1 with C2 = Q^H K; K -= Q C2 with G = K^H K
@@ -449,12 +451,19 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                 }
             }
             const int nb = s + 1;
-            // K[:, 0:nind] <- K[:, 0:nind] Rp^{-1}, in place (with the next pass's Gram)
+            // K[:, 0:nind] <- K[:, 0:nind] Rp^{-1}, in place (with the next pass's Gram).
+            // The second scaling is left pending when this block fills the basis:
+            // the thick restart that follows folds it into its own product.
             const std::vector<cd> Ri = inv_upper(nind, Rp);
-            std::vector<cd> Gn;
-            rc = blas_tsmm<S>(w, K, N, nind, Ri, nind, K, N, N, pass == 0 ? 2 : 0, 0, pass == 0 ? &Gn : nullptr);
-            if (rc != KFC_OK) return rc;
-            G.swap(Gn);
+            if (pass == 1 && !dep_last && k + s >= kd) {
+                pend_ri = Ri;
+                pend_n = nind;
+            } else {
+                std::vector<cd> Gn;
+                rc = blas_tsmm<S>(w, K, N, nind, Ri, nind, K, N, N, pass == 0 ? 2 : 0, 0, pass == 0 ? &Gn : nullptr);
+                if (rc != KFC_OK) return rc;
+                G.swap(Gn);
+            }
             ldg = nind;
             // Rt <- Rp Rt (the dependent column's coefficients come from the partial factor)
             std::vector<cd> Rn(size_t(nb0) * nb0, cd(0));
@@ -525,6 +534,11 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             s_cap = std::min(want, s_limit);
         }
         bool stop = conv >= m || dep_last || (n >= kd && restarts >= maxiter);
+        if (stop && pend_n > 0) {   // the explicit check needs the basis itself
+            rc = blas_tsmm<S>(w, K, N, pend_n, pend_ri, pend_n, K, N, N, 0, 0, nullptr);
+            if (rc != KFC_OK) return rc;
+            pend_n = 0;
+        }
         if (stop && !dep_last) {
             // confirm with an explicit Rayleigh-Ritz on the basis: W = H V gives the
             // exact projection and the true residuals of the wanted Ritz pairs
@@ -603,11 +617,26 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             int keep = (3 * kd + 2 * conv) / 5;
             keep = std::max(keep, std::min(m + 1, kd - 2));
             keep = std::min(keep, kd - 2);
-            // [V2 Ritz vectors, q] = [V q] [[Y_keep, 0]; [0, 1]] in one pass
+            // [V2 Ritz vectors, q] = [V q] [[Y_keep, 0]; [0, 1]] in one pass; with the
+            // block's last scaling pending, V[:, k:] holds Qn P^{-1} and the rows of the
+            // block (and the q column) are mapped through P^{-1} first
             std::vector<cd> M(size_t(n + 1) * (keep + 1), cd(0));
             for (int cI = 0; cI < keep; ++cI)
                 for (int i = 0; i < n; ++i) M[i + size_t(cI) * (n + 1)] = Y[i + size_t(cI) * n];
             M[n + size_t(keep) * (n + 1)] = 1.0;
+            if (pend_n > 0) {
+                const int k0 = n + 1 - pend_n;   // first block column (the block ends with q)
+                std::vector<cd> Mn = M;
+                for (int cI = 0; cI <= keep; ++cI)
+                    for (int l = 0; l < pend_n; ++l) {
+                        cd acc = 0;
+                        for (int j = l; j < pend_n; ++j)
+                            acc += pend_ri[l + size_t(j) * pend_n] * M[k0 + j + size_t(cI) * (n + 1)];
+                        Mn[k0 + l + size_t(cI) * (n + 1)] = acc;
+                    }
+                M.swap(Mn);
+                pend_n = 0;
+            }
             rc = blas_tsmm<S>(w, V, N, n + 1, M, keep + 1, V2, N, N, 0, 0, nullptr);
             if (rc != KFC_OK) return rc;
             std::swap(V, V2);

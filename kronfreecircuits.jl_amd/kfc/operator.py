@@ -138,6 +138,12 @@ class KronFreeOperator:
         L.check(L.lib().kfc_op_stencil_get(self.handle, C.byref(v), C.byref(m)), self.ctx.handle)
         return v.value, m.value
 
+    def work(self):
+        """``(bytes, flops)`` of one column of ``mul``: the roofline numerators."""
+        b, f = C.c_double(), C.c_double()
+        L.check(L.lib().kfc_op_work(self.handle, C.byref(b), C.byref(f)), self.ctx.handle)
+        return b.value, f.value
+
     def set_stencil(self, variant, march=0):
         """Pin the fused stencil's launch shape; ``variant=-1`` hands it back to the autotuner."""
         L.check(L.lib().kfc_op_stencil_set(self.handle, int(variant), int(march)), self.ctx.handle)
