@@ -191,18 +191,18 @@ def transmon_bench(args):
 
     insts = T.draw(args.instances, seed=args.seed)
     sw = T.TransmonSweep(contexts=args.contexts)
-    mode = args.leaf_mode
-    sw.run_batched(insts[:2], mode=mode)      # warm-up
+    mode, method = args.leaf_mode, args.leaf_method
+    sw.run_batched(insts[:2], mode=mode, method=method)      # warm-up
     sw.run_plain(insts[:1])
     plain, t_plain = sw.run_plain(insts[:args.plain_points])
     n_plain = sum(len(t.energies) for t in plain
                   if all(l.stats["max_residual"] <= 1e-8 for l in t.children))
     rate_plain = n_plain / t_plain
     log(f"[transmons] plain f64: {args.plain_points} instances {t_plain:.2f} s -> {rate_plain:.1f} pairs/s")
-    tops, wall, t_leaves = sw.run_batched(insts, mode=mode)
+    tops, wall, t_leaves = sw.run_batched(insts, mode=mode, method=method)
     ok = [t for t in tops if all(l.stats["max_residual"] <= 1e-8 for l in t.children)]
     value = sum(len(t.energies) for t in ok) / wall
-    log(f"[transmons] batched {mode}: {len(insts)} instances {wall:.2f} s (leaves {t_leaves:.2f} s) "
+    log(f"[transmons] batched {method} {mode}: {len(insts)} instances {wall:.2f} s (leaves {t_leaves:.2f} s) "
         f"-> {value:.1f} pairs/s")
     dE = max(float(np.max(np.abs(a.energies - b.energies) / np.abs(a.energies))) for a, b in zip(plain, tops))
     out = {"metric": "eigenpairs/s at resid<=1e-8 (512 coupled transmons on periodic grids, tops keep 20)",
@@ -212,7 +212,8 @@ def transmon_bench(args):
            "data": "synthetic (seeded circuit parameters)",
            "config": {"workload": "transmon_sweep", "instances": len(insts), "grid_sizes": list(T.GRID_SIZES),
                       "order": 8, "k_leaf": 8, "k_top": 20, "contexts": args.contexts, "mode": mode,
-                      "leaf_solver": "Lanczos tol 1e-9"},
+                      "leaf_solver": ("batched ChebyshevFilter degree 60, extra 8, tol 1e-9" if method == "chebyshev"
+                                      else "Lanczos krylovdim 48, tol 1e-9, one host thread per context")},
            "plain_fp64": {"instances": args.plain_points, "seconds": round(t_plain, 3),
                           "eigenpairs_per_s": round(rate_plain, 2)},
            "speedup_vs_plain": round(value / rate_plain, 2), "seconds_leaves": round(t_leaves, 3),
@@ -244,7 +245,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--nbasis", type=int, default=64, help="frame basis size of the warm start")
     ap.add_argument("--sweep-mode", default="mixed", choices=["f64", "mixed"])
-    ap.add_argument("--leaf-mode", default="f64", choices=["f64", "mixed"], help="transmon sweep leaf solves")
+    ap.add_argument("--leaf-mode", default="mixed", choices=["f64", "mixed"], help="transmon sweep leaf solves")
+    ap.add_argument("--leaf-method", default="chebyshev", choices=["chebyshev", "lanczos"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path", file=sys.stderr)

@@ -200,6 +200,20 @@ int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats);
  * _worst_true_residual per column (src/solvers.jl:65-69). */
 int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid);
 
+/* ---- batched sweeps of small one-axis nodes (no reference counterpart: a batch
+ * of the ChebyshevFilter solves of src/chebyshev.jl:310-406, which the
+ * reference runs one node at a time; closest caller bench/common.jl:135-163).
+ * n operators on one context, each with a single axis and banded, shift or
+ * diagonal terms (the leaves of a sweep of periodic-grid or charge-basis
+ * modes), all real or all complex.  Every filter step, Gram and rotation of
+ * the whole batch is one launch.  nb = m + extra <= 32 block columns per
+ * problem, a fixed filter degree, mode KFC_MODE_MIXED runs FP32 filter steps to
+ * the FP32 floor and then FP64 to tol.  E: n x m ascending; V[i]: host N_i x m
+ * (interleaved complex for complex operators); stats: n records. */
+int kfc_chebyshev_batch(int32_t n, kfc_op* const* ops, int64_t m, int64_t extra, int32_t degree, double tol,
+                        int64_t maxiter, uint64_t seed, int32_t mode, double* E, double* const* V,
+                        kfc_stats* stats);
+
 /* ---- learned warm start (no reference counterpart; feeds X0 of kfc_lobpcg).
  * A 6 -> 256 -> 256 -> 1024 GELU MLP with BF16 weights and FP32 accumulation. */
 typedef struct kfc_mlp kfc_mlp;

@@ -356,6 +356,7 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
     // storage) or ~1e-5 (FP32) of its norm counts as dependent: the Krylov space
     // is then invariant
     const double dep = single ? 1e-10 : 1e-20;
+    const double cond_floor = single ? 1e-8 : 1e-14;
     const double eps_s = single ? 5.96e-8 : 1.11e-16;
     const int64_t N = op->N;
     DevBuf bufA, bufB, bufR;
@@ -382,6 +383,9 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
     std::vector<cd> T, b;   // k x k and k
     int64_t restarts = 0;
     int s_limit = std::max(1, std::min(s_max, LZ_SMAX)), s_cap = s_limit, anchors = 0;
+    // H Q = Q T + q b^T holds to rounding (false after an explicit check found the
+    // basis polluted: then every first projection is computed from Q itself)
+    bool relation_exact = true;
     for (int64_t cyc = 0;; ++cyc) {
         int s = std::min(s_cap, kd - k);
         if (!ritz_known) s = std::min(s, 4);   // the first block runs on the loose interval
@@ -402,8 +406,34 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
         std::vector<cd> Cm(size_t(k) * nb0, cd(0)), Gk;
         if (k > 0) {
             std::vector<cd> C1, C2;
-            rc = blas_gram<S>(w, V, N, k, K, N, nb0, N, C1);
-            if (rc != KFC_OK) return rc;
+            if (relation_exact) {
+                // The first projection follows from the Krylov relation without a pass
+                // over Q: Q^H H v = T Q^H v + conj(b) (q^H v), so Q^H K obeys the
+                // block's Chebyshev recurrence given q^H K (one short Gram of the block).
+                // The explicit second pass then removes what rounding left.
+                std::vector<cd> g0;
+                rc = blas_gram<S>(w, K, N, 1, K, N, nb0, N, g0);
+                if (rc != KFC_OK) return rc;
+                C1.assign(size_t(k) * nb0, cd(0));
+                auto hcol = [&](int i, std::vector<cd>& out) {   // Q^H (H - c) K_i
+                    out.assign(k, cd(0));
+                    for (int r = 0; r < k; ++r) {
+                        cd acc = std::conj(b[r]) * g0[i] - cc * C1[r + size_t(i) * k];
+                        for (int j = 0; j < k; ++j) acc += T[r + size_t(j) * k] * C1[j + size_t(i) * k];
+                        out[r] = acc;
+                    }
+                };
+                std::vector<cd> hv;
+                for (int i = 0; i < s; ++i) {
+                    hcol(i, hv);
+                    for (int r = 0; r < k; ++r)
+                        C1[r + size_t(i + 1) * k] = i == 0 ? hv[r] / ee
+                                                           : 2.0 * hv[r] / ee - C1[r + size_t(i - 1) * k];
+                }
+            } else {
+                rc = blas_gram<S>(w, V, N, k, K, N, nb0, N, C1);
+                if (rc != KFC_OK) return rc;
+            }
             auto upd = [&](const std::vector<cd>& C) {   // [[-C]; [I]] on [Q K]
                 std::vector<cd> M(size_t(k + nb0) * nb0, cd(0));
                 for (int j = 0; j < nb0; ++j) {
@@ -435,7 +465,10 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             for (;;) {
                 double prel = 1;
                 const int n0 = nind;
-                const int fc = chol_upper(nind, G, ldg, Rp, dep, prel);
+                // a first-pass pivot below `cond_floor` (relative, squared) would make the
+                // block's condition number too large for the second pass to repair
+                // (eps cond^2 >= 1): such a column ends the block
+                const int fc = chol_upper(nind, G, ldg, Rp, pass == 0 ? cond_floor : dep, prel);
                 if (fc == nind) break;
                 if (pass == 0 && fc >= 1 && (std::fabs(prel) <= dep || fc == 1)) {
                     dcoef.assign(Rp.begin() + size_t(fc) * n0, Rp.begin() + size_t(fc) * n0 + fc);
@@ -447,7 +480,11 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                     nind = fc;
                     dep_last = false;
                 } else {
-                    return kfc_fail(op->ctx, KFC_EBREAKDOWN, "Lanczos: the Krylov block lost rank");
+                    return kfc_fail(op->ctx, KFC_EBREAKDOWN,
+                                    "Lanczos: the Krylov block lost rank (cycle " + std::to_string(cyc) + ", pass " +
+                                        std::to_string(pass) + ", column " + std::to_string(fc) + " of " +
+                                        std::to_string(nind) + ", pivot ratio " + std::to_string(prel) + ", basis " +
+                                        std::to_string(k) + " of " + std::to_string(kd) + ")");
                 }
             }
             const int nb = s + 1;
@@ -575,7 +612,10 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             while (conv < n && resid[conv] <= tol && (conv >= q || tr[conv] <= tbar)) ++conv;
             bool polluted = false;
             for (int i = 0; i < q; ++i) polluted |= resid[i] <= tol && tr[i] > tbar;
-            if (polluted) s_limit = 1;
+            if (polluted) {
+                s_limit = 1;
+                relation_exact = false;
+            }
             ++anchors;
             out.theta = theta;
             out.resid = resid;
@@ -670,6 +710,10 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
     int kd = int(std::max<int64_t>(2 * m + 10, krylovdim));   // src/solvers.jl:92
     kd = int(std::min<int64_t>(kd, N));
     if (kd > 512) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "Lanczos: krylovdim above 512 is not served");
+    // a Krylov basis that can fill (almost) the whole space: its last blocks are
+    // dependent by construction, and the Rayleigh-Ritz of the assembled operator is
+    // the exact answer the recurrence would reach (N <= 524 here)
+    if (int64_t(kd) + LZ_SMAX + 1 >= N) return dense_t<S>(op, m, Eout, Vout, st);
     Work w;
     w.ctx = op->ctx;
     Timer tm;

@@ -34,7 +34,7 @@ from .bases import PeriodicGrid
 from .operator import KronFreeOperator
 from concurrent.futures import ThreadPoolExecutor
 
-from .solvers import Dense, Lanczos, solve_stats
+from .solvers import ChebyshevFilter, Dense, Lanczos, solve_chebyshev_batch, solve_stats
 from .tree import (MaterializedNode, Node, circuit_terms, constant_term, fix_phases, materialize, node_axisterms,
                    operator, positive_mask, route)
 from .configs import two_transmons_spec
@@ -81,9 +81,12 @@ class TransmonSweep:
             out.append(materialize(root, spec_of(inst)).root)
         return out, time.perf_counter() - t0
 
-    def run_batched(self, insts, mode="f64", tol=1e-9):
-        """All leaves of all instances solved together (one host thread per device
-        context, each working through its share), then every top."""
+    def run_batched(self, insts, mode="f64", tol=1e-9, method="lanczos", degree=60, extra=8):
+        """All leaves of all instances solved together, then every top.  method
+        "lanczos": one host thread per device context, each working through its
+        share of the leaves; "chebyshev": one batched Chebyshev-filtered subspace
+        iteration over every leaf on one context (kfc_chebyshev_batch), each
+        filter step one launch for the whole batch."""
         t0 = time.perf_counter()
         roots, specs, leaf_ops, leaf_nodes = [], [], [], []
         c = 0
@@ -96,20 +99,26 @@ class TransmonSweep:
             for leaf in root.children:
                 dims = [leaf.children[0][1].dim]
                 ts = node_axisterms(leaf, [None], rt.get(id(leaf), []), leaf.real)
-                H = KronFreeOperator(dims, ts, constant=0.0, ctx=self.ctxs[c % len(self.ctxs)])
+                ctx = self.ctxs[0] if method == "chebyshev" else self.ctxs[c % len(self.ctxs)]
+                H = KronFreeOperator(dims, ts, constant=0.0, ctx=ctx)
                 c += 1
                 leaf_ops.append(H)
                 leaf_nodes.append(leaf)
         nctx = len(self.ctxs)
-        solver = Lanczos(tol=tol, krylovdim=48, mode=mode)
+        if method == "chebyshev":
+            solver = ChebyshevFilter(tol=tol, degree=degree, extra=extra, mode=mode)
+            res, st = solve_chebyshev_batch(leaf_ops, self.k_leaf, extra=extra, degree=degree, tol=tol, mode=mode)
+            results = [(E, V, s_) for (E, V), s_ in zip(res, st)]
+        else:
+            solver = Lanczos(tol=tol, krylovdim=48, mode=mode)
 
-        def work(k):   # every nctx-th leaf: the ones bound to context k
-            return [(i, solve_stats(solver, leaf_ops[i], self.k_leaf)) for i in range(k, len(leaf_ops), nctx)]
-        results = [None] * len(leaf_ops)
-        with ThreadPoolExecutor(max_workers=nctx) as ex:
-            for part in ex.map(work, range(nctx)):
-                for i, r in part:
-                    results[i] = r
+            def work(k):   # every nctx-th leaf: the ones bound to context k
+                return [(i, solve_stats(solver, leaf_ops[i], self.k_leaf)) for i in range(k, len(leaf_ops), nctx)]
+            results = [None] * len(leaf_ops)
+            with ThreadPoolExecutor(max_workers=nctx) as ex:
+                for part in ex.map(work, range(nctx)):
+                    for i, r in part:
+                        results[i] = r
         t_leaves = time.perf_counter() - t0
         mats = []
         for (E, V, st), H, leaf in zip(results, leaf_ops, leaf_nodes):

@@ -76,3 +76,77 @@ def test_transmon_batch_vs_oracle():
             assert leaf.stats["max_residual"] <= 1e-8
     for p, b in zip(plain, tops):
         assert np.max(np.abs(p.energies - b.energies) / np.abs(p.energies)) < 1e-9
+
+
+def _periodic_transmon(N, EJ, EC, ng, ctx=None):
+    import kfc
+    b = kfc.PeriodicGrid(N, 8)
+    return kfc.KronFreeOperator([N], [kfc.AxisTerm(4 * EC, [(0, kfc.localop(b, ("n2", 0)))]),
+                                      kfc.AxisTerm(-8 * EC * ng, [(0, kfc.localop(b, ("n", 0)))]),
+                                      kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", 1)))]),
+                                      kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", -1)))])],
+                                constant=4 * EC * ng ** 2, ctx=ctx)
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+def test_chebyshev_batch_complex_vs_dense(mode):
+    """Uneven complex periodic grids in one batch: every problem's lowest pairs
+    match Dense, with orthonormal vectors and residuals within tolerance."""
+    import kfc
+    params = [(96, 25.0, 1.0, 0.31), (160, 30.0, 1.2, 0.05), (64, 20.0, 0.9, 0.77), (200, 33.0, 1.3, 0.5)]
+    Hs = [_periodic_transmon(*p) for p in params]
+    res, st = kfc.solve_chebyshev_batch(Hs, 6, extra=6, degree=40, tol=1e-9, mode=mode)
+    for H, (E, V), s in zip(Hs, res, st):
+        Ed, _ = kfc.solve(kfc.Dense(), H, 6)
+        assert np.max(np.abs(E - Ed)) < 1e-9 * max(1.0, np.max(np.abs(Ed)))
+        assert np.allclose(V.conj().T @ V, np.eye(6), atol=1e-9)
+        r = kfc.residuals(H, E, V)
+        assert np.max(r) < 1e-8 and s["max_residual"] <= 1e-9
+
+
+def test_chebyshev_batch_real_charge_basis():
+    """Real one-axis operators (charge-basis transmons: shifts and a diagonal)."""
+    import kfc
+    Hs = []
+    for EJ, EC, ng in ((30.0, 1.2, 0.0), (12.0, 0.5, 0.0), (40.0, 2.0, 0.0)):
+        b = kfc.ChargeBasis(30)
+        Hs.append(kfc.KronFreeOperator([b.dim], [kfc.AxisTerm(4 * EC, [(0, kfc.localop(b, ("n2", 0)))]),
+                                                 kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", 1)))]),
+                                                 kfc.AxisTerm(-EJ / 2, [(0, kfc.localop(b, ("expiphi", -1)))])]))
+    assert all(H.dtype == np.float64 for H in Hs)
+    res, st = kfc.solve_chebyshev_batch(Hs, 5, extra=5, degree=30, tol=1e-10, mode="f64")
+    for H, (E, V), s in zip(Hs, res, st):
+        Ed, _ = kfc.solve(kfc.Dense(), H, 5)
+        assert np.max(np.abs(E - Ed)) < 1e-10 * max(1.0, np.max(np.abs(Ed)))
+        assert np.max(kfc.residuals(H, E, V)) < 1e-9
+
+
+def test_chebyshev_batch_rejects():
+    import kfc
+    from kfc import configs
+    H1 = _periodic_transmon(64, 25.0, 1.0, 0.3)
+    H3 = configs.three_fluxonia_operator(configs.three_fluxonia_theta(1), N=8)
+    with pytest.raises(kfc.UnsupportedError):
+        kfc.solve_chebyshev_batch([H3], 2, extra=2)
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve_chebyshev_batch([H1], 40, extra=40)     # m + extra above N
+    with pytest.raises(kfc.UnsupportedError):
+        kfc.solve_chebyshev_batch([H1], 20, extra=20)     # more than 32 block columns
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve_chebyshev_batch([H1], 4, extra=4, degree=0)
+
+
+def test_transmon_batch_chebyshev_vs_oracle():
+    from oracle import kfc_oracle as O
+    from kfc import transmon_sweep as T
+    insts = T.draw(3, seed=12)
+    for i, N in enumerate((64, 112, 80)):
+        insts[i]["N1"], insts[i]["N2"] = N, 176 - N
+    sw = T.TransmonSweep(contexts=1, k_leaf=8, k_top=12)
+    tops, _, _ = sw.run_batched(insts, mode="mixed", method="chebyshev", degree=40)
+    for inst, top in zip(insts, tops):
+        spec = O.two_transmons_spec(inst["E_J1"], inst["E_C1"], inst["n_g1"], inst["E_J2"], inst["E_C2"],
+                                    inst["n_g2"], inst["g"])
+        leaves = [O.Node([(f"varphi_{i}", O.PeriodicGrid(inst[f"N{i}"], 8))], keep=8, real=False) for i in (1, 2)]
+        ref = O.materialize(O.Node(leaves, keep=12, real=False), spec)
+        assert np.max(np.abs(top.energies - ref.energies) / np.abs(ref.energies)) < 1e-9
