@@ -185,6 +185,15 @@ int kfc_lobpcg_batch(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard, do
                      int32_t precond, double eta, int32_t seed_kind, const double* const* X0s, uint64_t seed,
                      int32_t mode, double* E, double* const* Vs, kfc_stats* stats);
 
+/* The same sweep in lockstep: n real operators of one size on ONE context (the
+ * sweep points of one node: same dims and term structure), every LOBPCG step one
+ * launch for all points, the contract of lobpcg_solve (src/lobpcg.jl:217-301) per
+ * point.  m + guard <= 21.  Arguments and outputs as kfc_lobpcg_batch; a point
+ * whose block collapses is re-solved by kfc_lobpcg. */
+int kfc_lobpcg_lockstep(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard, double tol, int64_t maxiter,
+                        int32_t precond, double eta, int32_t seed_kind, const double* const* X0s, uint64_t seed,
+                        int32_t mode, double* E, double* const* Vs, kfc_stats* stats);
+
 /* Replaces solve(::ChebyshevFilter, H, m) (src/chebyshev.jl:310-406).
  * ortho: 0 = :svqb, 1 = :cholqr; bound: a finite number, or NaN for :gershgorin. */
 int kfc_chebyshev(kfc_op* op, int64_t m, int32_t degree, int64_t extra, int64_t maxiter,

@@ -52,9 +52,9 @@ struct PairArgs {
     const T* B[MAXT];     // dq x dq column-major
 };
 
-// P: padded extent (multiple of 16) bounding dp and dq.
+// P: padded extent (multiple of 16) bounding dp and dq.  Slabs s0, s0 + step, ...
 template <class T, int P>
-__global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
+__device__ inline void dense_pair_slabs(const PairArgs<T>& a, int64_t s0, int64_t step) {
     using v4 = typename V4<T>::type;
     constexpr int NS = ((P / 16) * (P / 16) + 3) / 4;   // output tiles per wave
     __shared__ T Xs[P * P];   // column-major, leading dimension P
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
     const int tm = a.Mp / 16, tn = a.Nq / 16, ntile = tm * tn;
     const int r16 = lane & 15, k4 = lane >> 4;
     const int64_t perc = a.L * a.M * a.R;
-    for (int64_t s = blockIdx.x; s < a.nslab; s += gridDim.x) {
+    for (int64_t s = s0; s < a.nslab; s += step) {
         const int64_t col = s / perc;
         int64_t rem = s - col * perc;
         const int64_t l = rem % a.L;
@@ -134,6 +134,19 @@ __global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
             }
         }
     }
+}
+
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
+    dense_pair_slabs<T, P>(a, blockIdx.x, gridDim.x);
+}
+
+// One group of many operators of one structure: problem blockIdx.y takes its
+// arguments from the table.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_dense_pair_batch(const PairArgs<T>* __restrict__ tab) {
+    const PairArgs<T> a = tab[blockIdx.y];
+    dense_pair_slabs<T, P>(a, blockIdx.x, gridDim.x);
 }
 
 bool same(const std::vector<cd>& x, const std::vector<cd>& y) { return x == y; }
@@ -243,37 +256,87 @@ int pairs_upload(kfc_op* op, DeviceData& dd) {
 template int pairs_upload<double>(kfc_op*, DeviceData&);
 template int pairs_upload<float>(kfc_op*, DeviceData&);
 
+// The launch arguments of group gi of op on ncols columns of X -> Y.
 template <class T>
-int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st) {
+static PairArgs<T> pair_args(const kfc_op* op, size_t gi, const void* X, void* Y, int64_t ncols, int64_t ld) {
     const int prec = std::is_same<T, double>::value ? 0 : 1;
     const DeviceData& dd = op->dd[prec];
     std::vector<int64_t> stride(op->ndims + 1, 1);
     for (int a = 1; a <= op->ndims; ++a) stride[a] = stride[a - 1] * op->dims[a - 1];
+    const PairGroup& g = op->pairs.groups[gi];
+    PairArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.X = static_cast<const T*>(X);
+    a.Y = static_cast<T*>(Y);
+    a.D = gi == 0 ? dd.diag.as<T>() : nullptr;
+    a.ld = ld;
+    a.dp = int(g.dp);
+    a.dq = int(g.dq);
+    a.Mp = (a.dp + 15) / 16 * 16;
+    a.Nq = (a.dq + 15) / 16 * 16;
+    a.L = stride[g.p];
+    a.M = stride[g.q] / (stride[g.p] * g.dp);
+    a.R = op->N / stride[g.q + 1];
+    a.sp = stride[g.p];
+    a.sq = stride[g.q];
+    a.sr = stride[g.q + 1];
+    a.nslab = a.L * a.M * a.R * ncols;
+    a.nterm = int(g.A.size());
+    for (int t = 0; t < a.nterm; ++t) {
+        a.c[t] = T(g.c[t].real());
+        a.A[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t]);
+        a.B[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t + 1]);
+    }
+    return a;
+}
+
+bool pairs_same_structure(const kfc_op* a, const kfc_op* b) {
+    if (!a->pairs.ok || !b->pairs.ok || a->N != b->N || a->dims != b->dims) return false;
+    if (a->pairs.groups.size() != b->pairs.groups.size()) return false;
+    for (size_t g = 0; g < a->pairs.groups.size(); ++g) {
+        const PairGroup &x = a->pairs.groups[g], &y = b->pairs.groups[g];
+        if (x.p != y.p || x.q != y.q || x.dp != y.dp || x.dq != y.dq) return false;
+    }
+    return true;
+}
+
+template <class T>
+int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
+                       hipStream_t st, void* host_tab, void* dev_tab) {
+    if (n == 0) return KFC_OK;
+    const kfc_op* op0 = ops[0];
+    const size_t ng = op0->pairs.groups.size();
+    PairArgs<T>* h = static_cast<PairArgs<T>*>(host_tab);
+    PairArgs<T>* d = static_cast<PairArgs<T>*>(dev_tab);
+    for (size_t gi = 0; gi < ng; ++gi)
+        for (int q = 0; q < n; ++q) h[gi * n + q] = pair_args<T>(ops[q], gi, X[q], Y[q], ncols, ld);
+    KFC_HIP(op0->ctx, hipMemcpyAsync(d, h, sizeof(PairArgs<T>) * ng * n, hipMemcpyHostToDevice, st));
+    for (size_t gi = 0; gi < ng; ++gi) {
+        const PairArgs<T>& a = h[gi * n];
+        const int Pm = std::max(a.Mp, a.Nq);
+        const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>(a.nslab, std::max(1, 4096 / n))));
+        const dim3 grid(gx, unsigned(n));
+        const PairArgs<T>* tab = d + gi * n;
+        if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair_batch<T, 16>), grid, dim3(256), 0, st, tab);
+        else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair_batch<T, 32>), grid, dim3(256), 0, st, tab);
+        else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair_batch<T, 48>), grid, dim3(256), 0, st, tab);
+        else hipLaunchKernelGGL((k_dense_pair_batch<T, 64>), grid, dim3(256), 0, st, tab);
+        KFC_HIP(op0->ctx, hipGetLastError());
+    }
+    return KFC_OK;
+}
+
+template int launch_pairs_batch<double>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t,
+                                        hipStream_t, void*, void*);
+template int launch_pairs_batch<float>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t,
+                                       hipStream_t, void*, void*);
+
+size_t pair_args_bytes(int prec) { return prec == 0 ? sizeof(PairArgs<double>) : sizeof(PairArgs<float>); }
+
+template <class T>
+int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st) {
     for (size_t gi = 0; gi < op->pairs.groups.size(); ++gi) {
-        const PairGroup& g = op->pairs.groups[gi];
-        PairArgs<T> a;
-        std::memset(&a, 0, sizeof(a));
-        a.X = static_cast<const T*>(X);
-        a.Y = static_cast<T*>(Y);
-        a.D = gi == 0 ? dd.diag.as<T>() : nullptr;
-        a.ld = ld;
-        a.dp = int(g.dp);
-        a.dq = int(g.dq);
-        a.Mp = (a.dp + 15) / 16 * 16;
-        a.Nq = (a.dq + 15) / 16 * 16;
-        a.L = stride[g.p];
-        a.M = stride[g.q] / (stride[g.p] * g.dp);
-        a.R = op->N / stride[g.q + 1];
-        a.sp = stride[g.p];
-        a.sq = stride[g.q];
-        a.sr = stride[g.q + 1];
-        a.nslab = a.L * a.M * a.R * ncols;
-        a.nterm = int(g.A.size());
-        for (int t = 0; t < a.nterm; ++t) {
-            a.c[t] = T(g.c[t].real());
-            a.A[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t]);
-            a.B[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t + 1]);
-        }
+        const PairArgs<T> a = pair_args<T>(op, gi, X, Y, ncols, ld);
         const int Pm = std::max(a.Mp, a.Nq);
         const unsigned grid = unsigned(std::min<int64_t>(a.nslab, 4096));
         if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair<T, 16>), dim3(grid), dim3(256), 0, st, a);

@@ -216,6 +216,14 @@ int op_build_plan(kfc_op* op);
 int op_build_pairs(kfc_op* op);
 template <class T> int pairs_upload(kfc_op* op, DeviceData& dd);
 template <class T> int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st);
+// Many operators of one pair structure in one launch per group (dense_pair.hip):
+// problem q maps X[q] -> Y[q]; host_tab (pinned) and dev_tab hold n x groups
+// launch records of pair_args_bytes(prec) bytes each.
+bool pairs_same_structure(const kfc_op* a, const kfc_op* b);
+size_t pair_args_bytes(int prec);
+template <class T>
+int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
+                       hipStream_t st, void* host_tab, void* dev_tab);
 int op_host_diag(kfc_op* op);
 int op_ensure_device(kfc_op* op, int prec);
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st);
@@ -227,3 +235,6 @@ int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t nco
 // A is n x n column-major (overwritten); w[n]; Z[n x n] eigenvectors.
 void herm_eig(int n, std::vector<cd>& A, std::vector<double>& w, std::vector<cd>& Z);
 void sym_eig(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);
+// The lowest k eigenpairs (w: all n eigenvalues ascending; Z: n x k), by inverse
+// iteration on the tridiagonal form: several times faster than sym_eig for k << n.
+void sym_eig_lowest(int n, int k, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);

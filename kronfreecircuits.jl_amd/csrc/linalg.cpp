@@ -92,6 +92,13 @@ void tridiagonalize(int n, std::vector<T>& A, std::vector<double>& d, std::vecto
     }
 }
 
+// sqrt(a^2 + b^2) without the cost of std::hypot where no scaling is needed
+inline double fast_hypot(double a, double b) {
+    const double m = std::max(std::fabs(a), std::fabs(b));
+    if (m < 1e150 && m > 1e-150) return std::sqrt(a * a + b * b);
+    return std::hypot(a, b);
+}
+
 // Implicit QL with Wilkinson shifts on the symmetric tridiagonal (d, e[i] = T[i+1, i]),
 // rotations accumulated into the columns of Z (n_rows x n).
 template <class T>
@@ -110,14 +117,14 @@ void tql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<T>& 
             if (m == l) break;
             if (++iter > 60) break;   // no convergence (never seen for symmetric input)
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = std::hypot(g, 1.0);
+            double r = fast_hypot(g, 1.0);
             g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
             double s = 1.0, c = 1.0, p = 0.0;
             int i = m - 1;
             bool deflated = false;
             for (; i >= l; --i) {
                 double f = s * e[i], b = c * e[i];
-                r = std::hypot(f, g);
+                r = fast_hypot(f, g);
                 e[i + 1] = r;
                 if (r == 0.0) {
                     d[i + 1] -= p;
@@ -165,7 +172,173 @@ void eig_impl(int n, std::vector<T>& A, std::vector<double>& w, std::vector<T>& 
     Z.swap(Zs);
 }
 
+// Solve (T - sigma I) x = b for the symmetric tridiagonal T (diagonal d, off-diagonal
+// e) by Gaussian elimination with partial pivoting (the LU of a tridiagonal has two
+// superdiagonals); a zero pivot is replaced by a tiny one, as inverse iteration wants.
+struct TriLU {
+    int n = 0;
+    std::vector<double> dl, dg, du, du2;
+    std::vector<char> swp;
+    void factor(int n_, const std::vector<double>& d, const std::vector<double>& e, double sigma, double tiny) {
+        n = n_;
+        dg.resize(n);
+        dl.assign(std::max(n - 1, 0), 0.0);
+        du.assign(std::max(n - 1, 0), 0.0);
+        du2.assign(std::max(n - 2, 0), 0.0);
+        swp.assign(std::max(n - 1, 0), 0);
+        for (int i = 0; i < n; ++i) dg[i] = d[i] - sigma;
+        for (int i = 0; i + 1 < n; ++i) dl[i] = du[i] = e[i];
+        for (int i = 0; i + 1 < n; ++i) {
+            if (std::fabs(dg[i]) >= std::fabs(dl[i])) {
+                if (dg[i] == 0.0) dg[i] = tiny;
+                const double f = dl[i] / dg[i];
+                dl[i] = f;
+                dg[i + 1] -= f * du[i];
+            } else {   // rows i and i + 1 trade places
+                const double f = dg[i] / dl[i];
+                dg[i] = dl[i];
+                dl[i] = f;
+                const double t = du[i];
+                du[i] = dg[i + 1];
+                dg[i + 1] = t - f * dg[i + 1];
+                if (i + 2 < n) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -f * du[i + 1];
+                }
+                swp[i] = 1;
+            }
+        }
+        if (n > 0 && dg[n - 1] == 0.0) dg[n - 1] = tiny;
+    }
+    void solve(std::vector<double>& x) const {
+        for (int i = 0; i + 1 < n; ++i) {
+            if (swp[i]) std::swap(x[i], x[i + 1]);
+            x[i + 1] -= dl[i] * x[i];
+        }
+        x[n - 1] /= dg[n - 1];
+        if (n >= 2) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dg[n - 2];
+        for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dg[i];
+    }
+};
+
 }  // namespace
+
+// Lowest k eigenpairs of a real symmetric matrix: Householder tridiagonalization
+// (Golub & Van Loan, 4th ed., alg. 8.3.1), all eigenvalues by implicit QL without
+// vectors, the k eigenvectors of the tridiagonal by inverse iteration with the
+// members of a cluster (eigenvalues closer than 1e-3 of the norm) orthogonalized
+// against each other (Wilkinson; the scheme of Peters & Wilkinson, as LAPACK's
+// stein uses it), then back-transformed through the reflectors.  O(4/3 n^3 + 2 k n^2)
+// against the O(9 n^3) of QL with vectors.
+void sym_eig_lowest(int n, int k, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z) {
+    auto a = [&](int i, int j) -> double& { return A[i + size_t(j) * n]; };
+    k = std::max(0, std::min(k, n));
+    std::vector<double> d(n), e(n, 0.0), beta(n, 0.0), V(size_t(n) * n, 0.0), p(n), q(n);
+    double anorm = 0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += std::fabs(a(i, j));
+        anorm = std::max(anorm, s);
+    }
+    for (int j = 0; j + 2 < n; ++j) {
+        const int m = n - j - 1;
+        double* x = &A[(j + 1) + size_t(j) * n];
+        double sigma = 0;
+        for (int i = 1; i < m; ++i) sigma += x[i] * x[i];
+        const double x0 = x[0];
+        if (sigma == 0.0) {
+            e[j] = x0;
+            continue;
+        }
+        const double mu = std::sqrt(x0 * x0 + sigma);
+        const double v0 = x0 <= 0 ? x0 - mu : -sigma / (x0 + mu);
+        const double b = 2 * v0 * v0 / (sigma + v0 * v0);
+        double* v = &V[(j + 1) + size_t(j) * n];
+        v[0] = 1.0;
+        for (int i = 1; i < m; ++i) v[i] = x[i] / v0;
+        beta[j] = b;
+        e[j] = mu;
+        // A22 <- P A22 P with P = I - b v v^T:  p = b A22 v, w = p - (b/2)(p.v) v,
+        // A22 -= v w^T + w v^T
+        std::fill(p.begin(), p.begin() + m, 0.0);
+        for (int l = 0; l < m; ++l) {
+            const double vl = b * v[l];
+            const double* col = &A[(j + 1) + size_t(j + 1 + l) * n];
+            for (int i = 0; i < m; ++i) p[i] += col[i] * vl;
+        }
+        double pv = 0;
+        for (int i = 0; i < m; ++i) pv += p[i] * v[i];
+        const double K = 0.5 * b * pv;
+        for (int i = 0; i < m; ++i) q[i] = p[i] - K * v[i];
+        for (int l = 0; l < m; ++l) {
+            double* col = &A[(j + 1) + size_t(j + 1 + l) * n];
+            const double vl = v[l], ql = q[l];
+            for (int i = 0; i < m; ++i) col[i] -= v[i] * ql + q[i] * vl;
+        }
+    }
+    for (int i = 0; i < n; ++i) d[i] = a(i, i);
+    if (n >= 2) e[n - 2] = a(n - 1, n - 2);
+    e[n - 1] = 0.0;
+    // eigenvalues
+    std::vector<double> dq = d, eq = e, none;
+    tql<double>(n, dq, eq, none, 0);
+    std::sort(dq.begin(), dq.end());
+    w = dq;
+    // eigenvectors of the tridiagonal for the lowest k
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tnorm = std::max(anorm, 1e-300);
+    const double ortol = 1e-3 * tnorm, tiny = eps * tnorm;
+    std::vector<double> Y(size_t(n) * k), x(n);
+    TriLU lu;
+    int c0 = 0;   // first member of the current cluster
+    uint64_t rs = 0x9e3779b97f4a7c15ull;
+    for (int j = 0; j < k; ++j) {
+        if (j > 0 && w[j] - w[j - 1] > ortol) c0 = j;
+        // members of a cluster get slightly separated shifts
+        double sig = w[j];
+        if (j > c0 && w[j] - w[j - 1] < 10 * eps * tnorm) sig = w[j - 1] + 10 * eps * tnorm;
+        lu.factor(n, d, e, sig, tiny);
+        for (int i = 0; i < n; ++i) {   // a fixed pseudo-random start
+            rs ^= rs << 13;
+            rs ^= rs >> 7;
+            rs ^= rs << 17;
+            x[i] = double(int64_t(rs % 2001) - 1000) / 1000.0;
+        }
+        for (int it = 0; it < 4; ++it) {
+            lu.solve(x);
+            for (int pass = 0; pass < (j > c0 ? 2 : 1); ++pass)
+                for (int l = c0; l < j; ++l) {
+                    const double* y = &Y[size_t(l) * n];
+                    double s = 0;
+                    for (int i = 0; i < n; ++i) s += y[i] * x[i];
+                    for (int i = 0; i < n; ++i) x[i] -= s * y[i];
+                }
+            double nx = 0;
+            for (int i = 0; i < n; ++i) nx += x[i] * x[i];
+            nx = std::sqrt(nx);
+            if (!(nx > 0)) {
+                for (int i = 0; i < n; ++i) x[i] = i == j ? 1.0 : 0.0;
+                continue;
+            }
+            for (int i = 0; i < n; ++i) x[i] /= nx;
+        }
+        std::copy(x.begin(), x.end(), Y.begin() + size_t(j) * n);
+    }
+    // back-transform z = H_0 H_1 ... H_{n-3} y
+    Z = Y;
+    for (int c = 0; c < k; ++c) {
+        double* z = &Z[size_t(c) * n];
+        for (int j = n - 3; j >= 0; --j) {
+            if (beta[j] == 0.0) continue;
+            const double* v = &V[(j + 1) + size_t(j) * n];
+            const int m = n - j - 1;
+            double s = 0;
+            for (int i = 0; i < m; ++i) s += v[i] * z[j + 1 + i];
+            s *= beta[j];
+            for (int i = 0; i < m; ++i) z[j + 1 + i] -= s * v[i];
+        }
+    }
+}
 
 void herm_eig(int n, std::vector<cd>& A, std::vector<double>& w, std::vector<cd>& Z) { eig_impl<cd>(n, A, w, Z); }
 void sym_eig(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z) {

@@ -12,6 +12,9 @@
 
 int op_full_diagonal(kfc_op* op, std::vector<double>& D);
 int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
+int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& X0h);
+void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
+                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X);
 
 namespace {
 
@@ -1166,15 +1169,8 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
     const int64_t nx0 = seed_kind == KFC_SEED_WARM_GUARDS ? W : m;
     std::vector<cd> X0h;
     if (X0) {
-        read_host_V(op, X0, size_t(N) * nx0, X0h);
-        for (const cd& x : X0h)
-            if (!std::isfinite(x.real()) || !std::isfinite(x.imag()))
-                return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: X0 has non-finite entries");
-        for (int64_t j = 0; j < nx0; ++j) {
-            bool z = true;
-            for (int64_t i = 0; i < N; ++i) z &= X0h[i + j * N] == cd(0);
-            if (z) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: column " + std::to_string(j + 1) + " of X0 is zero");
-        }
+        int rc = lobpcg_check_x0(op, X0, nx0, X0h);
+        if (rc != KFC_OK) return rc;
     }
     std::vector<cd> P;
     if (seed_kind == KFC_SEED_PRODUCT) {
@@ -1202,35 +1198,9 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
         KFC_HIP(op->ctx, dD.ensure(sizeof(double) * D.size()));
         KFC_HIP(op->ctx, hipMemcpy(dD.p, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
     }
-    // initial block (_seed_columns, src/lobpcg.jl:182-190)
     auto seed_block = [&](bool random_only, uint64_t s) {
-        std::vector<cd> X(size_t(N) * W);
-        if (random_only) {
-            random_block(X, N, W, op->cplx, s);
-            return X;
-        }
-        if (X0 && nx0 == W) {   // the warm start carries its guards
-            X = X0h;
-            perturb(X, N, W, op->cplx, 0x4b46);
-        } else if (X0) {
-            std::vector<cd> F = X0h;
-            perturb(F, N, int(m), op->cplx, 0x4b46);
-            std::copy(F.begin(), F.end(), X.begin());
-            if (!P.empty() && g > 0) {
-                std::vector<cd> Gd(P.begin() + size_t(m) * N, P.end());
-                perturb(Gd, N, g, op->cplx, 0x4b46);
-                std::copy(Gd.begin(), Gd.end(), X.begin() + size_t(m) * N);
-            } else if (g > 0) {
-                std::vector<cd> R;
-                random_block(R, N, g, op->cplx, s);
-                std::copy(R.begin(), R.end(), X.begin() + size_t(m) * N);
-            }
-        } else if (!P.empty()) {
-            X = P;
-            perturb(X, N, W, op->cplx, 0x4b46);
-        } else {
-            random_block(X, N, W, op->cplx, s);
-        }
+        std::vector<cd> X;
+        lobpcg_seed_block(op, int(m), W, X0h, X0 ? nx0 : 0, P, random_only, s, X);
         return X;
     };
     Work w;
@@ -1327,6 +1297,59 @@ int residuals_t(kfc_op* op, int64_t m, const double* E, const double* V, double*
 }
 
 }  // namespace
+
+// X0 (prod(dims) x nx0, the caller's host layout) read and checked as
+// lobpcg_solve checks it (src/lobpcg.jl:226-233).
+int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& X0h) {
+    const int64_t N = op->N;
+    read_host_V(op, X0, size_t(N) * nx0, X0h);
+    for (const cd& x : X0h)
+        if (!std::isfinite(x.real()) || !std::isfinite(x.imag()))
+            return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: X0 has non-finite entries");
+    for (int64_t j = 0; j < nx0; ++j) {
+        bool z = true;
+        for (int64_t i = 0; i < N; ++i) z &= X0h[i + j * N] == cd(0);
+        if (z) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG: column " + std::to_string(j + 1) + " of X0 is zero");
+    }
+    return KFC_OK;
+}
+
+// The initial block, N x W (_seed_columns, src/lobpcg.jl:182-190): the warm start
+// (nx0 = m, or W when it carries its guards) or the product seed, perturbed; the
+// guards of an m-column warm start from the product seed (else random); a random
+// block when neither exists or random_only (the retry).
+void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
+                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X) {
+    const int64_t N = op->N;
+    const int g = W - m;
+    X.assign(size_t(N) * W, cd(0));
+    if (random_only) {
+        random_block(X, N, W, op->cplx, s);
+        return;
+    }
+    if (nx0 == W) {   // the warm start carries its guards
+        X = X0h;
+        perturb(X, N, W, op->cplx, 0x4b46);
+    } else if (nx0 > 0) {
+        std::vector<cd> F = X0h;
+        perturb(F, N, m, op->cplx, 0x4b46);
+        std::copy(F.begin(), F.end(), X.begin());
+        if (!P.empty() && g > 0) {
+            std::vector<cd> Gd(P.begin() + size_t(m) * N, P.end());
+            perturb(Gd, N, g, op->cplx, 0x4b46);
+            std::copy(Gd.begin(), Gd.end(), X.begin() + size_t(m) * N);
+        } else if (g > 0) {
+            std::vector<cd> R;
+            random_block(R, N, g, op->cplx, s);
+            std::copy(R.begin(), R.end(), X.begin() + size_t(m) * N);
+        }
+    } else if (!P.empty()) {
+        X = P;
+        perturb(X, N, W, op->cplx, 0x4b46);
+    } else {
+        random_block(X, N, W, op->cplx, s);
+    }
+}
 
 extern "C" {
 

@@ -60,12 +60,15 @@ def three_fluxonia_node(N=512, order=2, span=6 * math.pi, m=10, solver=None):
                 solver=solver or Lanczos())
 
 
-def three_fluxonia_operator(theta, N=512, order=2, span=6 * math.pi, device=0):
-    """The flat three-fluxonia node's real operator (one KronFreeOperator)."""
+def three_fluxonia_operator(theta, N=512, order=2, span=6 * math.pi, device=0, ctx=None):
+    """The flat three-fluxonia node's real operator (one KronFreeOperator), on
+    device `device` or on the context `ctx`."""
     spec = three_fluxonia_spec(theta["E_C"], theta["E_L"], theta["E_J"], theta["phi_ext"])
     node = three_fluxonia_node(N, order, span)
     named = circuit_terms(spec)
     ts = node_axisterms(node, [None] * 3, named, real=True)
+    if ctx is not None:
+        return KronFreeOperator([N] * 3, ts, constant=constant_term(spec), ctx=ctx)
     return KronFreeOperator([N] * 3, ts, constant=constant_term(spec), device=device)
 
 

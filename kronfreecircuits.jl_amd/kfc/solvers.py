@@ -197,10 +197,13 @@ def residuals(H, E, V):
     return out
 
 
-def solve_batch(s, Hs, m, X0s=None):
-    """Independent LOBPCG solves of a parameter sweep (``kfc_lobpcg_batch``):
-    operators built on different contexts (``kfc._lib.Context``) run their solves
-    concurrently on the device.  Returns lists of (E, V) and the stats."""
+def solve_batch(s, Hs, m, X0s=None, lockstep=False):
+    """Independent LOBPCG solves of a parameter sweep.  lockstep=False
+    (``kfc_lobpcg_batch``): operators built on different contexts
+    (``kfc._lib.Context``) run their solves concurrently on the device.
+    lockstep=True (``kfc_lobpcg_lockstep``): real operators of one size on one
+    context advance together, each LOBPCG step one launch for the whole sweep.
+    Returns lists of (E, V) and the stats."""
     if not isinstance(s, LOBPCG):
         raise L.ArgumentError("solve_batch serves LOBPCG sweeps")
     n = len(Hs)
@@ -232,9 +235,9 @@ def solve_batch(s, Hs, m, X0s=None):
                 raise L.ArgumentError("solve_batch: warm starts with guard columns need an X0 for every instance")
             seed_kind = L.KFC_SEED["warm_guards"]
     st = (L.Stats * n)()
-    L.check(L.lib().kfc_lobpcg_batch(n, ops, int(m), int(s.guard), float(s.tol), int(s.maxiter),
-                                     L.KFC_PRECOND[s.preconditioner], float(s.eta), seed_kind, x0p,
-                                     int(s.rng_seed), _mode(s.mode), E.ctypes.data_as(L._D), Vp, st),
+    fn = L.lib().kfc_lobpcg_lockstep if lockstep else L.lib().kfc_lobpcg_batch
+    L.check(fn(n, ops, int(m), int(s.guard), float(s.tol), int(s.maxiter), L.KFC_PRECOND[s.preconditioner],
+               float(s.eta), seed_kind, x0p, int(s.rng_seed), _mode(s.mode), E.ctypes.data_as(L._D), Vp, st),
             Hs[0].ctx.handle)
     out = [(E[i * m:(i + 1) * m].copy(), unpack_block(Vb[i], Hs[i].N, m, Hs[i].dtype)) for i in range(n)]
     return out, [x.as_dict() for x in st]

@@ -113,19 +113,20 @@ class ZeroPiSweep:
         """Columns LOBPCG converges on a parent of N states: m plus the guards."""
         return self.m + max(0, min(self.guard, (N - 5) // 3 - self.m))
 
-    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, nev=None):
-        """All points' parent solves in flight on the contexts (round-robin);
-        ``nev`` pairs per point (default m)."""
+    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, nev=None, lockstep=False):
+        """All points' parent solves in flight: on the contexts round-robin, or
+        (lockstep) on one context advancing together; ``nev`` pairs per point
+        (default m)."""
         t0 = time.perf_counter()
         Hs, kidss = [], []
         for i, th in enumerate(thetas):
-            H, kids = self.parent(th, ctx=self.ctxs[i % len(self.ctxs)])
+            H, kids = self.parent(th, ctx=self.ctxs[0 if lockstep else i % len(self.ctxs)])
             Hs.append(H)
             kidss.append(kids)
         X0s = None
         if x0_fn is not None:
             X0s = x0_fn(thetas, Hs, kidss)
-        res, st = solve_batch(LOBPCG(tol=tol, mode=mode, guard=self.guard), Hs, nev or self.m, X0s)
+        res, st = solve_batch(LOBPCG(tol=tol, mode=mode, guard=self.guard), Hs, nev or self.m, X0s, lockstep=lockstep)
         wall = time.perf_counter() - t0
         return [(E, V, s, k) for (E, V), s, k in zip(res, st, kidss)], wall
 
