@@ -109,9 +109,10 @@ def sweep_bench(args):
     def counted(out):
         return sum(int(np.sum(np.asarray(o[2]["max_residual"]) <= 1e-8)) * m for o in out)
 
+    lock = not args.contexts_only
     # warm-up: one point through every path (compiles, allocates, tunes)
     sw.run_plain(test[:1])
-    sw.run_batched(test[:2], mode=args.sweep_mode)
+    sw.run_batched(test[:2], mode=args.sweep_mode, lockstep=lock)
     plain, t_plain = sw.run_plain(test[:args.plain_points])
     rate_plain = counted(plain) / t_plain
     log(f"[sweep] plain f64: {args.plain_points} points {t_plain:.2f} s -> {rate_plain:.1f} pairs/s, "
@@ -119,7 +120,7 @@ def sweep_bench(args):
     t0 = time.perf_counter()
     # training data: the lowest m + g pairs (the guards too) from the plain solver
     nev = sw.width(40 * 40 * 30)
-    gen, t_gen = sw.run_batched(train, mode="f64", nev=nev)
+    gen, t_gen = sw.run_batched(train, mode="f64", nev=nev, lockstep=lock)
     frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
     log(f"[sweep] training data: {len(train)} points ({len(train) * nev} eigenpairs) in {t_gen:.1f} s")
     ws = WarmStart(sw, nbasis=args.nbasis, seed=1)
@@ -130,7 +131,7 @@ def sweep_bench(args):
     res = {}
     for name, mode, x0 in (("batched_f64", "f64", None), ("batched_mixed", "mixed", None),
                            ("batched_f64_learned", "f64", ws.x0_fn), ("batched_mixed_learned", "mixed", ws.x0_fn)):
-        out, wall = sw.run_batched(test, mode=mode, x0_fn=x0)
+        out, wall = sw.run_batched(test, mode=mode, x0_fn=x0, lockstep=lock)
         res[name] = {"seconds": round(wall, 3), "eigenpairs_per_s": round(counted(out) / wall, 2),
                      "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in out])), 1),
                      "max_residual": max(o[2]["max_residual"] for o in out),
@@ -140,6 +141,12 @@ def sweep_bench(args):
             f"mean iters {res[name]['mean_iterations']}")
         if name == "batched_f64":
             ref = out
+    # the same sweep through independent solves on several contexts, for comparison
+    out_ctx, wall_ctx = sw.run_batched(test, mode="f64")
+    res["contexts_f64"] = {"seconds": round(wall_ctx, 3), "eigenpairs_per_s": round(counted(out_ctx) / wall_ctx, 2),
+                           "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in out_ctx])), 1),
+                           "max_residual": max(o[2]["max_residual"] for o in out_ctx)}
+    log(f"[sweep] contexts_f64 ({args.contexts} contexts): {wall_ctx:.2f} s, {res['contexts_f64']['eigenpairs_per_s']} pairs/s")
     # agreement with the plain path and what the warm start captures
     dE = max(float(np.max(np.abs(a[0] - b[0]) / np.abs(b[0]))) for a, b in zip(ref[:args.plain_points], plain))
     Xf = ws.predict_frames(test[:8])
@@ -151,11 +158,13 @@ def sweep_bench(args):
     # the parent apply's algorithmic work (dense kept-basis pairs on MFMA)
     H0, _ = sw.parent(test[0])
     bytes_col, flops_col = H0.work()
-    # per-solve HIP-event apply time summed over concurrent solves overlaps; the
-    # per-column rate is taken from the plain (one solve at a time) run
+    # the batched FP64 sweep's apply: HIP-event time of its apply launches (lockstep:
+    # one launch per axis-pair group for all points) against the columns applied
+    bcols, bsec = res["batched_f64"]["matvec_columns"], res["batched_f64"]["seconds_apply_gpu"]
+    apply_rate = flops_col * bcols / bsec / 1e12 if bsec > 0 else None
     plain_cols = sum(o[2]["n_matvec"] for o in plain)
     plain_apply = sum(o[2]["seconds_apply_gpu"] for o in plain)
-    apply_rate = flops_col * plain_cols / plain_apply / 1e12 if plain_apply > 0 else None
+    plain_rate = flops_col * plain_cols / plain_apply / 1e12 if plain_apply > 0 else None
     out = {"metric": "eigenpairs/s at resid<=1e-8 (zero-pi sweep, parent [40,40,30], LOBPCG lowest 16)",
            "value": res["batched_mixed_learned"]["eigenpairs_per_s"], "unit": "eigenpairs/s", "n_gpus": 1,
            "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * res["batched_mixed_learned"]["seconds"], 1),
@@ -163,17 +172,21 @@ def sweep_bench(args):
            "data": "synthetic (seeded circuit parameters)",
            "config": {"workload": "zero_pi_sweep", "dims": [40, 40, 30], "m": m, "tol": 1e-8,
                       "test_points": args.test_points, "train_points": args.train_points, "nbasis": args.nbasis,
-                      "contexts": args.contexts},
+                      "contexts": args.contexts,
+                      "batched": "lockstep LOBPCG (kfc_lobpcg_lockstep)" if lock else "independent solves on contexts"},
            "plain_fp64": {"points": args.plain_points, "seconds": round(t_plain, 3),
                           "eigenpairs_per_s": round(rate_plain, 2),
                           "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in plain])), 1)},
            **res,
            "speedup_vs_plain": {k: round(v["eigenpairs_per_s"] / rate_plain, 2) for k, v in res.items()},
            "best": best, "max_rel_dE_batched_vs_plain": dE,
-           "roofline": {"bound": "mfma", "kernel": "k_dense_pair", "flops_per_column": flops_col,
+           "roofline": {"bound": "mfma", "kernel": "k_dense_pair_reg_batch" if lock else "k_dense_pair_reg",
+                        "flops_per_column": flops_col,
                         "achieved": round(apply_rate, 4) if apply_rate else None, "peak": 78.6, "unit": "TFLOP/s",
                         "frac": round(apply_rate / 78.6, 5) if apply_rate else None,
-                        "note": "FP64 MFMA; plain solves' HIP-event apply time (launch-bound at 48 000 states)"},
+                        "plain_solve_rate": round(plain_rate, 4) if plain_rate else None,
+                        "note": "FP64 MFMA; HIP-event time of the batched FP64 sweep's applies (plain_solve_rate: "
+                                "one solve at a time, launch-bound at 48 000 states)"},
            "warm_start": {"train_seconds": round(t_train, 2), "data_seconds": round(t_gen, 2),
                           "initial_loss": float(loss[0]), "final_loss": float(loss[-1]),
                           "outputs": f"{ws.nb} x {ws.m}", "training_eigenpairs": len(train) * nev,
@@ -239,6 +252,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise copy of known size (the reference for HBM byte counters)")
     ap.add_argument("--contexts", type=int, default=16, help="device contexts (streams) of a batched sweep")
+    ap.add_argument("--contexts-only", action="store_true",
+                    help="zero-pi sweep: batched solves as independent solves on contexts instead of lockstep")
     ap.add_argument("--test-points", type=int, default=64)
     ap.add_argument("--plain-points", type=int, default=8)
     ap.add_argument("--train-points", type=int, default=256)

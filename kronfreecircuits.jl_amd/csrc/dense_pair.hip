@@ -136,17 +136,257 @@ __device__ inline void dense_pair_slabs(const PairArgs<T>& a, int64_t s0, int64_
     }
 }
 
+// The same slabs with the port matrices staged in LDS (P <= 48): each term's B, then
+// its A, is copied once per slab into a shared buffer with coalesced loads, so the
+// matrix-core loops read both operands from LDS instead of issuing a global load per
+// step.  Xs, Ts and the matrix buffer take 3 P^2 scalars (55 KB at P = 48, FP64).
+template <class T, int P>
+__device__ inline void dense_pair_slabs_lds(const PairArgs<T>& a, int64_t s0, int64_t step) {
+    using v4 = typename V4<T>::type;
+    constexpr int NS = ((P / 16) * (P / 16) + 3) / 4;   // output tiles per wave
+    __shared__ T Xs[P * P];   // column-major, leading dimension P
+    __shared__ T Ts[P * P];
+    __shared__ T Ms[P * P];   // B_t, then A_t (zero padded)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tm = a.Mp / 16, tn = a.Nq / 16, ntile = tm * tn;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int64_t perc = a.L * a.M * a.R;
+    auto stage = [&](const T* __restrict__ Mg, int d) {   // Ms[i + k P] = M[i + k d], zero outside d x d
+        for (int e = tid; e < P * P; e += 256) {
+            const int i = e % P, k = e / P;
+            Ms[e] = (i < d && k < d) ? Mg[i + (int64_t)k * d] : T(0);
+        }
+    };
+    for (int64_t s = s0; s < a.nslab; s += step) {
+        const int64_t col = s / perc;
+        int64_t rem = s - col * perc;
+        const int64_t l = rem % a.L;
+        rem /= a.L;
+        const int64_t m = rem % a.M;
+        const int64_t r = rem / a.M;
+        const int64_t base = col * a.ld + l + m * a.L * a.dp + r * a.sr;
+        __syncthreads();   // the previous slab's readers of Xs, Ts and Ms are done
+        for (int e = tid; e < P * P; e += 256) {
+            const int i = e % P, j = e / P;
+            Xs[e] = (i < a.dp && j < a.dq) ? a.X[base + i * a.sp + j * a.sq] : T(0);
+        }
+        v4 acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = v4{0, 0, 0, 0};
+        for (int t = 0; t < a.nterm; ++t) {
+            if (t > 0) __syncthreads();   // the previous term's readers of Ms and Ts are done
+            stage(a.B[t], a.dq);
+            __syncthreads();
+            // Ts = Xs B^T : (dp x dq)(dq x dq)
+            for (int tile = wave; tile < ntile; tile += 4) {
+                const int ti = tile % tm, tj = tile / tm;
+                v4 c = v4{0, 0, 0, 0};
+                const int row = ti * 16 + r16, bc = tj * 16 + r16;
+#pragma unroll 4
+                for (int kk = 0; kk < a.Nq; kk += 4) {
+                    const int k = kk + k4;
+                    c = mfma(Xs[row + k * P], Ms[bc + k * P], c);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) Ts[(ti * 16 + out_row(T(0), k4, i)) + (tj * 16 + r16) * P] = c[i];
+            }
+            __syncthreads();   // Ts complete, B no longer read
+            stage(a.A[t], a.dp);
+            __syncthreads();
+            // acc += c_t A Ts : (dp x dp)(dp x dq)
+#pragma unroll
+            for (int slot = 0; slot < NS; ++slot) {
+                const int tile = wave + 4 * slot;
+                if (tile >= ntile) break;
+                const int ti = tile % tm, tj = tile / tm;
+                v4 c = v4{0, 0, 0, 0};
+                const int row = ti * 16 + r16, cc = tj * 16 + r16;
+#pragma unroll 4
+                for (int kk = 0; kk < a.Mp; kk += 4) {
+                    const int k = kk + k4;
+                    c = mfma(Ms[row + k * P], Ts[k + cc * P], c);
+                }
+                acc[slot] += a.c[t] * c;
+            }
+        }
+        // epilogue: Y_s = D .* X_s + acc   (or Y_s += acc)
+#pragma unroll
+        for (int slot = 0; slot < NS; ++slot) {
+            const int tile = wave + 4 * slot;
+            if (tile >= ntile) break;
+            const int ti = tile % tm, tj = tile / tm;
+            const int cc = tj * 16 + r16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = ti * 16 + out_row(T(0), k4, i);
+                if (row < a.dp && cc < a.dq) {
+                    const int64_t idx = base + row * a.sp + cc * a.sq;
+                    const int64_t didx = idx - col * a.ld;
+                    const T y0 = a.D ? a.D[didx] * Xs[row + cc * P] : a.Y[idx];
+                    a.Y[idx] = y0 + acc[slot][i];
+                }
+            }
+        }
+    }
+}
+
+// Register-fragment variant for groups of at most NT terms (P <= 48): a block has
+// P / 16 waves; in Ts = Xs B^T wave w forms tile column w, in acc += A Ts tile row
+// w, so each wave needs only its own 16 rows of every B_t and A_t, held in registers
+// for the whole block (loaded once, not per slab).  The next slab is fetched into
+// registers while the current one is contracted, the tiles' accumulation chains are
+// interleaved, and the result goes through LDS so that Y is read and written along
+// the slab's memory order.
+template <class T, int P, int NT>
+__device__ inline void dense_pair_slabs_reg(const PairArgs<T>& a, int64_t s0, int64_t step) {
+    using v4 = typename V4<T>::type;
+    constexpr int NW = P / 16, NTH = 64 * NW, KS = P / 4;
+    // the next slab is prefetched into registers only where they are not all taken
+    // by the fragments
+    constexpr bool PREF = NT == 1 || sizeof(T) == 4;
+    constexpr int NPF = PREF ? (P * P + NTH - 1) / NTH : 1;
+    __shared__ T Xs[P * P];   // column-major, leading dimension P
+    __shared__ T Ts[P * P];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int tm = a.Mp / 16, tn = a.Nq / 16, ksq = a.Nq / 4, ksp = a.Mp / 4;
+    const int nterm = a.nterm;
+    T Bf[NT][KS], Af[NT][KS];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + k4, j = w * 16 + r16;
+            Bf[t][ks] = (t < nterm && j < a.dq && k < a.dq) ? a.B[t][j + (int64_t)k * a.dq] : T(0);
+            Af[t][ks] = (t < nterm && j < a.dp && k < a.dp) ? a.A[t][j + (int64_t)k * a.dp] : T(0);
+        }
+    const int64_t perc = a.L * a.M * a.R;
+    auto base_of = [&](int64_t s, int64_t& col) {
+        col = s / perc;
+        int64_t rem = s - col * perc;
+        const int64_t l = rem % a.L;
+        rem /= a.L;
+        const int64_t m = rem % a.M;
+        const int64_t r = rem / a.M;
+        return col * a.ld + l + m * a.L * a.dp + r * a.sr;
+    };
+    T pf[NPF];
+    auto fetch = [&](int64_t s) {
+        int64_t col;
+        const int64_t base = base_of(s, col);
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH, i = e % P, j = e / P;
+            pf[u] = (e < P * P && i < a.dp && j < a.dq) ? a.X[base + i * a.sp + j * a.sq] : T(0);
+        }
+    };
+    if (PREF && s0 < a.nslab) fetch(s0);
+    for (int64_t s = s0; s < a.nslab; s += step) {
+        int64_t col;
+        const int64_t base = base_of(s, col);
+        __syncthreads();   // the previous slab's readers of Xs and Ts are done
+        if constexpr (PREF) {
+#pragma unroll
+            for (int u = 0; u < NPF; ++u) {
+                const int e = tid + u * NTH;
+                if (e < P * P) Xs[e] = pf[u];
+            }
+        } else {
+            for (int e = tid; e < P * P; e += NTH) {
+                const int i = e % P, j = e / P;
+                Xs[e] = (i < a.dp && j < a.dq) ? a.X[base + i * a.sp + j * a.sq] : T(0);
+            }
+        }
+        __syncthreads();
+        if (PREF && s + step < a.nslab) fetch(s + step);
+        v4 acc[NW];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc[j] = v4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            if (t < nterm) {   // uniform across the block
+                // Ts[:, tile column w] = Xs B_t^T
+                if (w < tn) {
+                    v4 c[NW];
+#pragma unroll
+                    for (int ti = 0; ti < NW; ++ti) c[ti] = v4{0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+                        if (ks < ksq)
+#pragma unroll
+                            for (int ti = 0; ti < NW; ++ti)
+                                if (ti < tm) c[ti] = mfma(Xs[(ti * 16 + r16) + (4 * ks + k4) * P], Bf[t][ks], c[ti]);
+#pragma unroll
+                    for (int ti = 0; ti < NW; ++ti)
+                        if (ti < tm)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                Ts[(ti * 16 + out_row(T(0), k4, i)) + (w * 16 + r16) * P] = c[ti][i];
+                }
+                __syncthreads();
+                // acc[tile row w] += c_t A_t Ts
+                if (w < tm) {
+                    v4 c[NW];
+#pragma unroll
+                    for (int tj = 0; tj < NW; ++tj) c[tj] = v4{0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+                        if (ks < ksp)
+#pragma unroll
+                            for (int tj = 0; tj < NW; ++tj)
+                                if (tj < tn) c[tj] = mfma(Af[t][ks], Ts[(4 * ks + k4) + (tj * 16 + r16) * P], c[tj]);
+#pragma unroll
+                    for (int tj = 0; tj < NW; ++tj) acc[tj] += a.c[t] * c[tj];
+                }
+                __syncthreads();   // Ts is free again
+            }
+        }
+        // the result through Ts, then Y along the slab's memory order
+        if (w < tm)
+#pragma unroll
+            for (int tj = 0; tj < NW; ++tj)
+                if (tj < tn)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) Ts[(w * 16 + out_row(T(0), k4, i)) + (tj * 16 + r16) * P] = acc[tj][i];
+        __syncthreads();
+        for (int e = tid; e < P * P; e += NTH) {
+            const int i = e % P, j = e / P;
+            if (i < a.dp && j < a.dq) {
+                const int64_t idx = base + i * a.sp + j * a.sq;
+                const T y0 = a.D ? a.D[idx - col * a.ld] * Xs[e] : a.Y[idx];
+                a.Y[idx] = y0 + Ts[e];
+            }
+        }
+    }
+}
+
+template <class T, int P, int NT>
+__global__ __launch_bounds__(64 * (P / 16)) void k_dense_pair_reg(const PairArgs<T> a) {
+    dense_pair_slabs_reg<T, P, NT>(a, blockIdx.x, gridDim.x);
+}
+
+template <class T, int P, int NT>
+__global__ __launch_bounds__(64 * (P / 16)) void k_dense_pair_reg_batch(const PairArgs<T>* __restrict__ tab) {
+    // a private copy: every term index in this kernel is a compile-time constant, so
+    // the copy is scalarized (no scratch)
+    const PairArgs<T> a = tab[blockIdx.y];
+    dense_pair_slabs_reg<T, P, NT>(a, blockIdx.x, gridDim.x);
+}
+
 template <class T, int P>
 __global__ __launch_bounds__(256) void k_dense_pair(const PairArgs<T> a) {
-    dense_pair_slabs<T, P>(a, blockIdx.x, gridDim.x);
+    if constexpr (P <= 48) dense_pair_slabs_lds<T, P>(a, blockIdx.x, gridDim.x);
+    else dense_pair_slabs<T, P>(a, blockIdx.x, gridDim.x);
 }
 
 // One group of many operators of one structure: problem blockIdx.y takes its
 // arguments from the table.
 template <class T, int P>
 __global__ __launch_bounds__(256) void k_dense_pair_batch(const PairArgs<T>* __restrict__ tab) {
-    const PairArgs<T> a = tab[blockIdx.y];
-    dense_pair_slabs<T, P>(a, blockIdx.x, gridDim.x);
+    // the record is read in place (a private copy indexed by term would live in scratch)
+    const PairArgs<T>& a = tab[blockIdx.y];
+    if constexpr (P <= 48) dense_pair_slabs_lds<T, P>(a, blockIdx.x, gridDim.x);
+    else dense_pair_slabs<T, P>(a, blockIdx.x, gridDim.x);
 }
 
 bool same(const std::vector<cd>& x, const std::vector<cd>& y) { return x == y; }
@@ -314,10 +554,23 @@ int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* co
     for (size_t gi = 0; gi < ng; ++gi) {
         const PairArgs<T>& a = h[gi * n];
         const int Pm = std::max(a.Mp, a.Nq);
+        int ntmax = 0;
+        for (int q = 0; q < n; ++q) ntmax = std::max(ntmax, h[gi * n + q].nterm);
         const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>(a.nslab, std::max(1, 4096 / n))));
         const dim3 grid(gx, unsigned(n));
         const PairArgs<T>* tab = d + gi * n;
-        if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair_batch<T, 16>), grid, dim3(256), 0, st, tab);
+        if (Pm <= 48 && ntmax <= 2) {
+            if (Pm <= 16) {
+                if (ntmax <= 1) hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 16, 1>), grid, dim3(64), 0, st, tab);
+                else hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 16, 2>), grid, dim3(64), 0, st, tab);
+            } else if (Pm <= 32) {
+                if (ntmax <= 1) hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 32, 1>), grid, dim3(128), 0, st, tab);
+                else hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 32, 2>), grid, dim3(128), 0, st, tab);
+            } else {
+                if (ntmax <= 1) hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 48, 1>), grid, dim3(192), 0, st, tab);
+                else hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 48, 2>), grid, dim3(192), 0, st, tab);
+            }
+        } else if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair_batch<T, 16>), grid, dim3(256), 0, st, tab);
         else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair_batch<T, 32>), grid, dim3(256), 0, st, tab);
         else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair_batch<T, 48>), grid, dim3(256), 0, st, tab);
         else hipLaunchKernelGGL((k_dense_pair_batch<T, 64>), grid, dim3(256), 0, st, tab);
@@ -339,7 +592,18 @@ int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, 
         const PairArgs<T> a = pair_args<T>(op, gi, X, Y, ncols, ld);
         const int Pm = std::max(a.Mp, a.Nq);
         const unsigned grid = unsigned(std::min<int64_t>(a.nslab, 4096));
-        if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair<T, 16>), dim3(grid), dim3(256), 0, st, a);
+        if (Pm <= 48 && a.nterm <= 2) {
+            if (Pm <= 16) {
+                if (a.nterm <= 1) hipLaunchKernelGGL((k_dense_pair_reg<T, 16, 1>), dim3(grid), dim3(64), 0, st, a);
+                else hipLaunchKernelGGL((k_dense_pair_reg<T, 16, 2>), dim3(grid), dim3(64), 0, st, a);
+            } else if (Pm <= 32) {
+                if (a.nterm <= 1) hipLaunchKernelGGL((k_dense_pair_reg<T, 32, 1>), dim3(grid), dim3(128), 0, st, a);
+                else hipLaunchKernelGGL((k_dense_pair_reg<T, 32, 2>), dim3(grid), dim3(128), 0, st, a);
+            } else {
+                if (a.nterm <= 1) hipLaunchKernelGGL((k_dense_pair_reg<T, 48, 1>), dim3(grid), dim3(192), 0, st, a);
+                else hipLaunchKernelGGL((k_dense_pair_reg<T, 48, 2>), dim3(grid), dim3(192), 0, st, a);
+            }
+        } else if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair<T, 16>), dim3(grid), dim3(256), 0, st, a);
         else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair<T, 32>), dim3(grid), dim3(256), 0, st, a);
         else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair<T, 48>), dim3(grid), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_dense_pair<T, 64>), dim3(grid), dim3(256), 0, st, a);

@@ -116,7 +116,11 @@ class ZeroPiSweep:
     def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, nev=None, lockstep=False):
         """All points' parent solves in flight: on the contexts round-robin, or
         (lockstep) on one context advancing together; ``nev`` pairs per point
-        (default m)."""
+        (default m).  Lockstep serves blocks of up to 21 columns (m + guards); a
+        wider request runs on the contexts."""
+        n_ev = nev or self.m
+        if lockstep and 3 * (n_ev + self.guard) > 64:
+            lockstep = False
         t0 = time.perf_counter()
         Hs, kidss = [], []
         for i, th in enumerate(thetas):
