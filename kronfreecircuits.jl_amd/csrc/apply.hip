@@ -654,9 +654,26 @@ __device__ inline void store_stream(Cx<R>* p, Cx<R> v) {
 constexpr int SP_TX = 64, SP_TY = 16, SP_R = SP_TY / 4, SP_W = SP_TX + 2, SP_H = SP_TY + 2,
               SP_TILE = SP_W * SP_H, SP_NL = (SP_TILE + 255) / 256;
 
-template <class S, bool DP>
-__global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
-    __shared__ S tile[2][SP_TILE];
+// Zeros that the LDS-DMA variant loads for points outside the domain.
+__device__ uint32_t g_lds_zero[64] = {0};
+
+// Planes per batch of the LDS-DMA variant.
+constexpr int SA_D = 2;
+
+// ASYNC = false: the plane sweep above.  ASYNC = true (variant 3): the planes go
+// straight from HBM into LDS with global_load_lds (no registers held for the
+// prefetch), SA_D planes per batch into one of two buffers; a batch is computed
+// without barriers while the next one streams in, with one barrier per batch.
+template <class S, bool DP, bool ASYNC>
+__device__ inline void stencil_sweep(const StArgs<S>& a) {
+    constexpr int DW = sizeof(S) / 4;                 // dwords per point
+    constexpr int TDW = SP_TILE * DW;                 // dwords per plane tile
+    constexpr int NCH = (TDW + 255) / 256;            // dword chunks per thread
+    constexpr int PDW = NCH * 256;                    // padded plane (dwords)
+    constexpr int NT_SYNC = ASYNC ? 1 : 2;
+    __shared__ S tile[NT_SYNC][ASYNC ? 1 : SP_TILE];
+    __shared__ __attribute__((aligned(16))) uint32_t bufA[ASYNC ? SA_D * PDW : 1];
+    __shared__ __attribute__((aligned(16))) uint32_t bufB[ASYNC ? SA_D * PDW : 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nblk = gridDim.x;
     const int per = nblk >> 3;
@@ -687,6 +704,35 @@ __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
         goff[j] = gok[j] ? gx + gy * int(a.s1) : 0;
     }
     const int z0 = bz * a.L, z1 = min(z0 + a.L, a.d2);
+    // LDS-DMA: dword e = (j * 4 + wave) * 64 + lane of a plane tile is chunk j of
+    // this thread; its offset in the plane (dwords), or -1 outside the domain
+    int aoff[ASYNC ? NCH : 1];
+    if constexpr (ASYNC) {
+#pragma unroll
+        for (int j = 0; j < NCH; ++j) {
+            const int e = (j * 4 + wv) * 64 + lane;
+            const int el = e / DW, part = e - el * DW;
+            const int r = el / SP_W, c = el - r * SP_W;
+            const int gx = x0 - 1 + c, gy = y0 - 1 + r;
+            const bool ok = e < TDW && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1;
+            aoff[j] = ok ? (gx + gy * int(a.s1)) * DW + part : -1;
+        }
+    }
+    const uint32_t* __restrict__ Xd = reinterpret_cast<const uint32_t*>(Xc);
+    // planes zf .. zf + SA_D - 1 (planes outside [0, d2) or after z1 load zeros)
+    auto issue = [&](uint32_t* buf, int zf) {
+#pragma unroll
+        for (int p = 0; p < SA_D; ++p) {
+            const int z = zf + p;
+            const bool zin = z >= 0 && z < a.d2 && z <= z1;
+            const uint32_t* Xz = Xd + (int64_t)(zin ? z : 0) * a.s2 * DW;
+#pragma unroll
+            for (int j = 0; j < NCH; ++j) {
+                const uint32_t* src = (zin && aoff[j] >= 0) ? Xz + aoff[j] : g_lds_zero + lane;
+                __builtin_amdgcn_global_load_lds(src, buf + p * PDW + (j * 4 + wv) * 64, 4, 0, 0);
+            }
+        }
+    };
     S pre[SP_NL];
     auto fetch = [&](int z) {
         const bool zin = z >= 0 && z < a.d2;
@@ -726,13 +772,7 @@ __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
     S accM[SP_R], acc0[SP_R], accP[SP_R], ctrM[SP_R], ctr0[SP_R];
 #pragma unroll
     for (int r = 0; r < SP_R; ++r) accM[r] = acc0[r] = accP[r] = ctrM[r] = ctr0[r] = zero;
-    int buf = 0;
-    fetch(z0 - 1);
-    stash(tile[0]);
-    __syncthreads();
-    for (int zp = z0 - 1; zp <= z1; ++zp) {
-        if (zp + 1 <= z1) fetch(zp + 1);
-        const S* t = tile[buf];
+    auto plane = [&](const S* __restrict__ t, int zp) {
         // rows r0 .. r0 + SP_R + 1 of the patch, columns lane .. lane + 2
         S v[SP_R + 2][3];
 #pragma unroll
@@ -798,10 +838,52 @@ __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
             accP[r] = zero;
             ctrM[r] = ctr0[r];
         }
-        if (zp + 1 <= z1) stash(tile[buf ^ 1]);
+    };
+    if constexpr (!ASYNC) {
+        int buf = 0;
+        fetch(z0 - 1);
+        stash(tile[0]);
         __syncthreads();
-        buf ^= 1;
+        for (int zp = z0 - 1; zp <= z1; ++zp) {
+            if (zp + 1 <= z1) fetch(zp + 1);
+            plane(tile[buf], zp);
+            if (zp + 1 <= z1) stash(tile[buf ^ 1]);
+            __syncthreads();
+            buf ^= 1;
+        }
+    } else {
+        // planes z0 - 1 .. z1; batch k in bufA (k even) or bufB (k odd).  Every
+        // barrier waits for this thread's LDS-DMA loads (vmcnt 0) first.
+        const int zs = z0 - 1;
+        issue(bufA, zs);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __syncthreads();
+        for (int zb = zs; zb <= z1; zb += 2 * SA_D) {
+            if (zb + SA_D <= z1) issue(bufB, zb + SA_D);
+#pragma unroll 1
+            for (int p = 0; p < SA_D; ++p)
+                if (zb + p <= z1) plane(reinterpret_cast<const S*>(bufA + p * PDW), zb + p);
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            __syncthreads();
+            if (zb + SA_D > z1) break;
+            if (zb + 2 * SA_D <= z1) issue(bufA, zb + 2 * SA_D);
+#pragma unroll 1
+            for (int p = 0; p < SA_D; ++p)
+                if (zb + SA_D + p <= z1) plane(reinterpret_cast<const S*>(bufB + p * PDW), zb + SA_D + p);
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            __syncthreads();
+        }
     }
+}
+
+template <class S, bool DP>
+__global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
+    stencil_sweep<S, DP, false>(a);
+}
+
+template <class S, bool DP>
+__global__ __launch_bounds__(256) void k_stencil_a(const StArgs<S> a) {
+    stencil_sweep<S, DP, true>(a);
 }
 
 template <class S>
@@ -845,13 +927,13 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     a.gamma = s_from<S>(gamma);
     bool cw = true;
     for (int p = 0; p < 27; ++p) cw &= !(a.u[p] || a.v[p] || a.w[p]);
-    const int variant = (op->stencil_variant == 2 && !cw) ? 1 : op->stencil_variant;
-    const int ty = variant == 2 ? SP_TY : ST_TY;
+    const int variant = (op->stencil_variant >= 2 && !cw) ? 1 : op->stencil_variant;
+    const int ty = variant >= 2 ? SP_TY : ST_TY;
     a.nb0 = int((a.d0 + ST_TX - 1) / ST_TX);
     a.nb1 = int((a.d1 + ty - 1) / ty);
     a.ncols = int(ncols);
     int L = op->stencil_march > 0 ? op->stencil_march : 64;
-    const int64_t min_blocks = variant == 2 ? 2048 : 8192;   // the plane sweep wants long marches
+    const int64_t min_blocks = variant >= 2 ? 2048 : 8192;   // the plane sweep wants long marches
     while (L > 8 && (int64_t)a.nb0 * a.nb1 * ((a.d2 + L - 1) / L) * ncols < min_blocks) L >>= 1;
     L = std::min(L, a.d2);
     a.L = std::max(L, 1);
@@ -863,6 +945,9 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     if (variant == 2) {
         if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_stencil_p<S, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else if (variant == 3) {
+        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_a<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_a<S, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (variant == 0) {
         hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (cw) {
@@ -1007,7 +1092,8 @@ static int tune_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64
         op->stencil_variant = atoi(env);
         return KFC_OK;
     }
-    const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}, {2, 16}, {2, 32}, {2, 64}, {2, 128}};
+    const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}, {2, 16}, {2, 32},
+                                         {2, 64}, {2, 128}, {3, 16}, {3, 32}, {3, 64}, {3, 128}};
     hipEvent_t e0, e1;
     KFC_HIP(op->ctx, hipEventCreate(&e0));
     KFC_HIP(op->ctx, hipEventCreate(&e1));

@@ -71,7 +71,7 @@ def test_fused_path_random(dims, cplx):
     assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
 
 
-STENCIL_SHAPES = [(0, 64), (1, 16), (1, 64), (2, 16), (2, 64), (2, 128)]
+STENCIL_SHAPES = [(0, 64), (1, 16), (1, 64), (2, 16), (2, 64), (2, 128), (3, 16), (3, 64), (3, 128)]
 
 
 @pytest.mark.parametrize("variant,march", STENCIL_SHAPES)
