@@ -654,6 +654,19 @@ __device__ inline void store_stream(Cx<R>* p, Cx<R> v) {
 constexpr int SP_TX = 64, SP_TY = 16, SP_R = SP_TY / 4, SP_W = SP_TX + 2, SP_H = SP_TY + 2,
               SP_TILE = SP_W * SP_H, SP_NL = (SP_TILE + 255) / 256;
 
+// A value at a wave-uniform address, read through the scalar unit (constant
+// address space; the data are never written during the launch).
+template <class S>
+__device__ inline S uniform_load(const S* p, int i) {
+    if constexpr (IsCx<S>::value) {
+        using R = typename RealOf<S>::type;
+        const __attribute__((address_space(4))) R* q = (const __attribute__((address_space(4))) R*)p;
+        return S{q[2 * i], q[2 * i + 1]};
+    } else {
+        return ((const __attribute__((address_space(4))) S*)p)[i];
+    }
+}
+
 // Zeros that the LDS-DMA variant loads for points outside the domain.
 __device__ uint32_t g_lds_zero[64] = {0};
 
@@ -801,10 +814,15 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
         }
         // the diagonal of output zp
         if (zp >= z0 && zp < z1) {
-            const S d2 = a.D2 ? a.D2[zp] : zero;
+            // uniform values through the scalar unit (constant address space): their
+            // wait is on the scalar counter and leaves the vector prefetch in flight
+            const S d2 = a.D2 ? uniform_load(a.D2, zp) : zero;
             S dp2[NDP];
 #pragma unroll
-            for (int k = 0; k < NDP; ++k) dp2[k] = (DP && k < a.ndp) ? (a.dp2[k] ? a.dp2[k][zp] : one) : zero;
+            for (int k = 0; k < NDP; ++k)
+                dp2[k] = (DP && k < a.ndp)
+                             ? (a.dp2[k] ? uniform_load(a.dp2[k], zp) : one)
+                             : zero;
 #pragma unroll
             for (int r = 0; r < SP_R; ++r) {
                 S dv = s_add(d01[r], d2);
