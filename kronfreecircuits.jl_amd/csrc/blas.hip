@@ -764,6 +764,58 @@ __global__ void k_randn(S* X, int64_t ldx, int p, int64_t N, uint64_t seed) {
     }
 }
 
+// The seed perturbation of LOBPCG (_perturbed_seed, src/lobpcg.jl:167-177): column j
+// of block b plus a random vector r scaled to sqrt(eps) times the column's norm,
+// r from a counter-based normal stream (seed, j, row); complex entries have
+// independent parts scaled by 1/sqrt(2).  One block per column: the norms in a
+// first pass, the update in a second (r regenerated).  Columns j >= ncols[b] are
+// left as they are.
+template <class S>
+__global__ __launch_bounds__(256) void k_perturb(S* X, int64_t ldx, int64_t N, int64_t bstride,
+                                                 const int* __restrict__ ncols, uint64_t seed) {
+    using R = typename RealOf<S>::type;
+    const int j = blockIdx.x, bi = blockIdx.y;
+    if (ncols && j >= ncols[bi]) return;   // the whole block leaves before any barrier
+    S* x = X + bi * bstride + (int64_t)j * ldx;
+    const uint64_t ks = mix64(seed ^ mix64(uint64_t(j) * 0x100000001B3ull));
+    constexpr double sc = IsCx<S>::value ? 0.7071067811865476 : 1.0;
+    double nx = 0, nr = 0;
+    for (int64_t i = threadIdx.x; i < N; i += 256) {
+        double a, b;
+        normal2(ks + uint64_t(i), a, b);
+        nx += s_abs2(x[i]);
+        nr += IsCx<S>::value ? 0.5 * (a * a + b * b) : a * a;
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = nx;
+    red[1][threadIdx.x] = nr;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    const double f = red[1][0] > 0 ? 1.4901161193847656e-08 * sqrt(red[0][0]) / sqrt(red[1][0]) : 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += 256) {
+        double a, b;
+        normal2(ks + uint64_t(i), a, b);
+        if constexpr (IsCx<S>::value) x[i] = S{R(x[i].re + f * sc * a), R(x[i].im + f * sc * b)};
+        else x[i] = R(x[i] + f * a);
+    }
+}
+
+template <class S>
+int blas_perturb(Work& w, S* X, int64_t ldx, int p, int64_t N, int batch, int64_t bstride, const int* ncols,
+                 uint64_t seed) {
+    if (p <= 0 || N == 0 || batch <= 0) return KFC_OK;
+    hipLaunchKernelGGL(k_perturb<S>, dim3(unsigned(p), unsigned(batch)), dim3(256), 0, w.ctx->stream, X, ldx, N,
+                       bstride, ncols, seed);
+    KFC_HIP(w.ctx, hipGetLastError());
+    return KFC_OK;
+}
+
 template <class S>
 int blas_randn(Work& w, S* X, int64_t ldx, int p, int64_t N, uint64_t seed) {
     if (p == 0 || N == 0) return KFC_OK;
@@ -954,6 +1006,7 @@ int lz_update(Work& w, const S* V, int64_t ldv, int k, const std::vector<cd>& hs
     template int blas_diag_expect<S>(Work&, const S*, int64_t, int, int64_t, const double*, std::vector<double>&);  \
     template int blas_axpby<S>(Work&, const S*, int64_t, S*, int64_t, int, int64_t, double, double);                \
     template int blas_randn<S>(Work&, S*, int64_t, int, int64_t, uint64_t);                                         \
+    template int blas_perturb<S>(Work&, S*, int64_t, int, int64_t, int, int64_t, const int*, uint64_t);           \
     template int lz_multidot<S>(Work&, const S*, int64_t, int, const double*, const S*, int64_t, std::vector<cd>&,   \
                                 double*);                                                                           \
     template int lz_update<S>(Work&, const S*, int64_t, int, const std::vector<cd>&, S*, int64_t, double*);

@@ -46,7 +46,7 @@ int op_full_diagonal(kfc_op* op, std::vector<double>& D);
 int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
 int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& X0h);
 void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
-                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X);
+                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X, int& nperturb);
 
 namespace {
 
@@ -957,7 +957,7 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
     DevBuf dD;
     if (pmode) KFC_HIP(ctx, dD.ensure(sizeof(double) * size_t(n) * N * 2));
     {
-        std::vector<int> rcs(n, KFC_OK);
+        std::vector<int> rcs(n, KFC_OK), npert(n, 0);
         std::vector<std::vector<double>> Xh(n), Dh(n);
         Team team0(std::max(1, std::min(16, int(std::thread::hardware_concurrency()))));
         team0.run(n, [&](int i) {
@@ -972,7 +972,7 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                 if (rcs[i] != KFC_OK) return;
             }
             lobpcg_seed_block(op, int(m), W, X0h, (X0s && X0s[i]) ? nx0 : 0, P,
-                              seed_kind == KFC_SEED_RANDOM && !(X0s && X0s[i]), seed + i, X);
+                              seed_kind == KFC_SEED_RANDOM && !(X0s && X0s[i]), seed + i, X, npert[i]);
             Xh[i].resize(X.size());
             for (size_t k = 0; k < X.size(); ++k) Xh[i][k] = X[k].real();
             if (pmode) {
@@ -986,6 +986,10 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
         });
         for (int i = 0; i < n; ++i)
             if (rcs[i] != KFC_OK) return kfc_fail(ctx, rcs[i], "instance " + std::to_string(i) + ": " + ops[i]->ctx->err);
+        // the seeded columns of every point perturbed on the device, one launch
+        DevBuf dnp;
+        KFC_HIP(ctx, dnp.ensure(sizeof(int) * n));
+        KFC_HIP(ctx, hipMemcpy(dnp.p, npert.data(), sizeof(int) * n, hipMemcpyHostToDevice));
         for (int i = 0; i < n; ++i) {
             pts[i].op = ops[i];
             KFC_HIP(ctx, hipMemcpy(S64[0].as<double>() + size_t(i) * 3 * W * N, Xh[i].data(), sizeof(double) * N * W,
@@ -998,6 +1002,11 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                                        hipMemcpyHostToDevice));
             }
         }
+        Work wp;
+        wp.ctx = ctx;
+        rc = blas_perturb<double>(wp, S64[0].as<double>(), N, W, N, n, int64_t(3) * W * N, dnp.as<int>(), 0x4b46);
+        if (rc != KFC_OK) return rc;
+        KFC_HIP(ctx, hipStreamSynchronize(st));
     }
     bool pair_batch = true;
     for (int i = 0; i < n; ++i) {

@@ -14,7 +14,7 @@ int op_full_diagonal(kfc_op* op, std::vector<double>& D);
 int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
 int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& X0h);
 void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
-                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X);
+                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X, int& nperturb);
 
 namespace {
 
@@ -902,7 +902,7 @@ bool gen_eig(int n, const std::vector<cd>& GA, const std::vector<cd>& GB, std::v
 
 template <class S>
 int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>& X0h, const double* Ddev,
-                Timer& tm, LobpcgRes& out, double switch_tol) {
+                Timer& tm, LobpcgRes& out, double switch_tol, int nperturb = 0) {
     const int64_t N = op->N;
     const int W = cfg.w;
     DevBuf sA, sB, aA, aB, tmpb;
@@ -919,6 +919,7 @@ int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>
     S* tmp = tmpb.as<S>();
     auto col = [&](S* b, int c) { return b + size_t(c) * N; };
     int rc = upload_block<S>(op->ctx, X0h, N, W, Sb, N);
+    if (rc == KFC_OK) rc = blas_perturb<S>(w, Sb, N, nperturb, N, 1, 0, nullptr, 0x4b46);
     if (rc != KFC_OK) return rc;
     int kx = 0;
     rc = ortho_block<S>(w, Sb, N, 0, W, tmp, kx, 1e-14);
@@ -1120,26 +1121,6 @@ int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>
     return download_block<S>(op->ctx, Sb, N, N, W, out.X);
 }
 
-// MersenneTwister-free seeded perturbation of a host block: each column plus a
-// random vector of norm sqrt(eps) times the column's norm (_perturbed_seed, src/lobpcg.jl:167-177).
-void perturb(std::vector<cd>& X, int64_t N, int p, bool cplx, uint64_t seed) {
-    std::mt19937_64 g(seed);
-    std::normal_distribution<double> nd;
-    const double delta = std::sqrt(2.220446049250313e-16);
-    for (int j = 0; j < p; ++j) {
-        cd* c = &X[size_t(j) * N];
-        std::vector<cd> r(N);
-        double nc = 0, nr = 0;
-        for (int64_t i = 0; i < N; ++i) {
-            r[i] = cplx ? cd(nd(g), nd(g)) / std::sqrt(2.0) : cd(nd(g), 0);
-            nc += std::norm(c[i]);
-            nr += std::norm(r[i]);
-        }
-        const double f = delta * std::sqrt(nc) / std::sqrt(nr);
-        for (int64_t i = 0; i < N; ++i) c[i] += f * r[i];
-    }
-}
-
 void random_block(std::vector<cd>& X, int64_t N, int p, bool cplx, uint64_t seed) {
     std::mt19937_64 g(seed);
     std::normal_distribution<double> nd;
@@ -1198,9 +1179,10 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
         KFC_HIP(op->ctx, dD.ensure(sizeof(double) * D.size()));
         KFC_HIP(op->ctx, hipMemcpy(dD.p, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
     }
+    int nperturb = 0;
     auto seed_block = [&](bool random_only, uint64_t s) {
         std::vector<cd> X;
-        lobpcg_seed_block(op, int(m), W, X0h, X0 ? nx0 : 0, P, random_only, s, X);
+        lobpcg_seed_block(op, int(m), W, X0h, X0 ? nx0 : 0, P, random_only, s, X, nperturb);
         return X;
     };
     Work w;
@@ -1218,7 +1200,7 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
             kfc_op_gershgorin(op, &hb);
             const double sw = std::max(tol, 256.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
             LobpcgRes r32;
-            int rc = lobpcg_core<L>(op, w, cfg, Xs, Dp, tm, r32, sw);
+            int rc = lobpcg_core<L>(op, w, cfg, Xs, Dp, tm, r32, sw, nperturb);
             if (rc != KFC_OK) return rc;
             if (r32.breakdown) {
                 res = r32;
@@ -1230,7 +1212,7 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
             res.iterations += r32.iterations;
             return rc;
         }
-        return lobpcg_core<S>(op, w, cfg, Xs, Dp, tm, res, 0.0);
+        return lobpcg_core<S>(op, w, cfg, Xs, Dp, tm, res, 0.0, nperturb);
     };
     int rc = run(seed_block(seed_kind == KFC_SEED_RANDOM && !X0, seed));
     if (rc != KFC_OK) return rc;
@@ -1315,29 +1297,29 @@ int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& 
 }
 
 // The initial block, N x W (_seed_columns, src/lobpcg.jl:182-190): the warm start
-// (nx0 = m, or W when it carries its guards) or the product seed, perturbed; the
-// guards of an m-column warm start from the product seed (else random); a random
-// block when neither exists or random_only (the retry).
+// (nx0 = m, or W when it carries its guards) or the product seed; the guards of an
+// m-column warm start from the product seed (else random); a random block when
+// neither exists or random_only (the retry).  The seeded columns (the first
+// nperturb) are perturbed on the device after upload (blas_perturb, seed 0x4b46).
 void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
-                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X) {
+                       const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X, int& nperturb) {
     const int64_t N = op->N;
     const int g = W - m;
     X.assign(size_t(N) * W, cd(0));
+    nperturb = 0;
     if (random_only) {
         random_block(X, N, W, op->cplx, s);
         return;
     }
     if (nx0 == W) {   // the warm start carries its guards
         X = X0h;
-        perturb(X, N, W, op->cplx, 0x4b46);
+        nperturb = W;
     } else if (nx0 > 0) {
-        std::vector<cd> F = X0h;
-        perturb(F, N, m, op->cplx, 0x4b46);
-        std::copy(F.begin(), F.end(), X.begin());
+        std::copy(X0h.begin(), X0h.begin() + size_t(N) * m, X.begin());
+        nperturb = m;
         if (!P.empty() && g > 0) {
-            std::vector<cd> Gd(P.begin() + size_t(m) * N, P.end());
-            perturb(Gd, N, g, op->cplx, 0x4b46);
-            std::copy(Gd.begin(), Gd.end(), X.begin() + size_t(m) * N);
+            std::copy(P.begin() + size_t(m) * N, P.end(), X.begin() + size_t(m) * N);
+            nperturb = W;
         } else if (g > 0) {
             std::vector<cd> R;
             random_block(R, N, g, op->cplx, s);
@@ -1345,7 +1327,7 @@ void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0
         }
     } else if (!P.empty()) {
         X = P;
-        perturb(X, N, W, op->cplx, 0x4b46);
+        nperturb = W;
     } else {
         random_block(X, N, W, op->cplx, s);
     }
