@@ -194,6 +194,44 @@ def sweep_bench(args):
     print(json.dumps(out))
 
 
+def ports_bench(args):
+    """Port formation V' O V on a materialized zero-pi parent (48 000 states, 16
+    kept): the host formula (numpy, what _port_matrix does on the CPU) against the
+    device path (kfc_sandwich), per port, for single and joint ports."""
+    import numpy as np
+
+    from kfc import configs, materialize
+    from kfc.tree import joint_port, port_matrix
+
+    th = configs.zero_pi_theta(8800)
+    spec, E_Cz = configs.zero_pi_spec(th["E_CS"], th["n_g"])
+    mn = materialize(configs.zero_pi_tree(E_Cz), spec).root
+    singles = [("theta", ("n", 0)), ("theta", ("expiphi", 1)), ("phi", ("phi", 1)), ("phi", ("n", 0)),
+               ("zeta", ("n", 0)), ("zeta", ("phi2", 0))]
+    joints = [[("theta", ("expiphi", 1)), ("phi", ("expiphi", 1))], [("theta", ("n", 0)), ("zeta", ("n", 0))]]
+    out = {}
+    for dev in (False, True):
+        port_matrix(mn, *singles[0], device=dev)       # warm-up (library load, tuner)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for mode, port in singles:
+                port_matrix(mn, mode, port, device=dev)
+            for f in joints:
+                joint_port(mn, f, device=dev)
+        out["device" if dev else "host"] = (time.perf_counter() - t0) / (args.steps * (len(singles) + len(joints)))
+    err = max(float(np.max(np.abs(port_matrix(mn, m, p, device=True) - port_matrix(mn, m, p, device=False))))
+              for m, p in singles)
+    res = {"metric": "port matrices/s (zero-pi parent, 48 000 states x 16 kept)", "value": round(1.0 / out["device"], 2),
+           "unit": "ports/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": round(1e3 * out["device"] * (len(singles) + len(joints)), 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "ports", "dims": list(mn.dims), "kept": int(mn.vectors.shape[1]),
+                      "ports": len(singles), "joint_ports": len(joints)},
+           "host_seconds_per_port": round(out["host"], 5), "device_seconds_per_port": round(out["device"], 5),
+           "speedup": round(out["host"] / out["device"], 2), "max_abs_diff_device_vs_host": err}
+    print(json.dumps(res))
+
+
 def transmon_bench(args):
     """512 coupled-transmon instances on periodic grids ({256..768} points per
     mode, complex leaves keeping 8, tops keeping 20 = 10 240 eigenpairs): plain
@@ -247,7 +285,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many cycles")
     ap.add_argument("--workload", default="three_fluxonia",
-                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep"])
+                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports"])
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise copy of known size (the reference for HBM byte counters)")
@@ -270,6 +308,8 @@ def main():
         return calibrate_copy()
     if args.workload == "zero_pi_sweep":
         return sweep_bench(args)
+    if args.workload == "ports":
+        return ports_bench(args)
     if args.workload == "transmon_sweep":
         return transmon_bench(args)
     import numpy as np

@@ -210,6 +210,13 @@ int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats);
  * _worst_true_residual per column (src/solvers.jl:65-69). */
 int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid);
 
+/* G = V^H (op V) for a DEVICE block V (k <= 1024 columns of prod(dims), leading
+ * dimension ld, FP64 or complex FP64 as op), G[k x k] on the host (complex:
+ * interleaved).  With op a single product term — the node's port operators on
+ * their axes — this is the port formation _port_matrix / _joint_port
+ * (src/materialize.jl:441-444, 490-493) on the device. */
+int kfc_sandwich(kfc_op* op, const void* V, int64_t k, int64_t ld, double* G);
+
 /* ---- batched sweeps of small one-axis nodes (no reference counterpart: a batch
  * of the ChebyshevFilter solves of src/chebyshev.jl:310-406, which the
  * reference runs one node at a time; closest caller bench/common.jl:135-163).

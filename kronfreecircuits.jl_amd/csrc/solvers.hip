@@ -1363,6 +1363,46 @@ int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, doubl
 
 }  // extern "C"
 
+namespace {
+
+// G = V^H (op V) for a device block V (k columns, leading dimension ld): the
+// operator applied into a scratch block, then one Gram on the matrix cores.
+template <class S>
+int sandwich_t(kfc_op* op, const void* V, int64_t k, int64_t ld, double* G) {
+    const int64_t N = op->N;
+    Work w;
+    w.ctx = op->ctx;
+    DevBuf dW;
+    KFC_HIP(op->ctx, dW.ensure(sizeof(S) * size_t(ld) * k));
+    int rc = op_apply(op, V, dW.p, k, ld, KFC_PREC_F64, op->ctx->stream);
+    if (rc != KFC_OK) return rc;
+    std::vector<cd> Gh;
+    rc = blas_gram<S>(w, static_cast<const S*>(V), ld, int(k), dW.as<S>(), ld, int(k), N, Gh);
+    if (rc != KFC_OK) return rc;
+    KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
+    for (size_t i = 0; i < Gh.size(); ++i) {
+        if (IsCx<S>::value) {
+            G[2 * i] = Gh[i].real();
+            G[2 * i + 1] = Gh[i].imag();
+        } else {
+            G[i] = Gh[i].real();
+        }
+    }
+    return KFC_OK;
+}
+
+}  // namespace
+
+// The projected operator V^H H V of a device block; with H a single product term
+// (the node's port operator on its axes) this is _port_matrix / _joint_port
+// (src/materialize.jl:441-444, 490-493).
+extern "C" int kfc_sandwich(kfc_op* op, const void* V, int64_t k, int64_t ld, double* G) {
+    if (!op || !V || !G) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_sandwich: null argument");
+    if (k < 1 || k > 1024) return kfc_fail(op->ctx, KFC_EARG, "kfc_sandwich: k must lie in 1:1024, got " + std::to_string(k));
+    if (ld < op->N) return kfc_fail(op->ctx, KFC_EARG, "kfc_sandwich: ld below prod(dims)");
+    return op->cplx ? sandwich_t<Cx<double>>(op, V, k, ld, G) : sandwich_t<double>(op, V, k, ld, G);
+}
+
 // Independent LOBPCG solves of several operators, one host thread per distinct
 // context: each context's stream carries its operators' solves in order, and the
 // contexts' streams run concurrently on the device (a parameter sweep).

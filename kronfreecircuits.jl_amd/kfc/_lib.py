@@ -103,6 +103,7 @@ SIGNATURES = {
     "kfc_lobpcg_batch": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int32,
                                    C.c_double, C.c_int32, C.POINTER(_D), C.c_uint64, C.c_int32, _D, C.POINTER(_D),
                                    C.POINTER(Stats)]),
+    "kfc_sandwich": (C.c_int, [_P, C.c_void_p, C.c_int64, C.c_int64, _D]),
     "kfc_lobpcg_lockstep": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int64, C.c_double, C.c_int64,
                                       C.c_int32, C.c_double, C.c_int32, C.POINTER(_D), C.c_uint64, C.c_int32, _D,
                                       C.POINTER(_D), C.POINTER(Stats)]),
@@ -132,6 +133,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libkfc.so is not built at {LIB_PATH}; run __graft_entry__.build()")
+        # torch carries its own HIP runtime, which finds the GPU only when it starts
+        # before this library's (the device blocks of the Python API are torch
+        # tensors): start it first
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             if hasattr(L, name):

@@ -123,24 +123,67 @@ def _axis_apply(op, V, dims, a):
     return T.reshape(V.shape, order="F")
 
 
-def port_matrix(mn, mode, port):
+# Port formation moves to the device (kfc_sandwich) from this many entries of V on
+# (the zero-pi parent's 48 000 x 16 block is above it, the leaves are below).
+DEVICE_PORT_ENTRIES = 1 << 19
+
+
+def sandwich(V, dims, ops, ctx=None):
+    """V' (x_a O_a) V on the device (``kfc_sandwich``): ``ops`` is a list of
+    (axis, LocalOp); the operator is built as one AxisTerm and applied to V
+    uploaded once, then contracted with V on the matrix cores.  A complex
+    operator on real V is taken in complex arithmetic (the reference forms its
+    real and imaginary components separately, src/materialize.jl:456-461; the
+    result is the same matrix)."""
+    import ctypes as C
+    import torch
+    V = np.asarray(V)
+    N, k = V.shape
+    cplx = np.iscomplexobj(V) or any(np.iscomplexobj(dense(op)) for _, op in ops)
+    term = AxisTerm(1.0 + 0j if cplx else 1.0, sorted(ops, key=lambda t: t[0]))
+    H = KronFreeOperator(list(dims), [term], constant=0.0, ctx=ctx)
+    if (H.dtype == np.complex128) != cplx:
+        raise L.ArgumentError("sandwich: operator and block types disagree")
+    Vt = torch.from_numpy(np.ascontiguousarray(V.T.astype(np.complex128 if cplx else np.float64))).to("cuda")
+    G = np.empty(2 * k * k if cplx else k * k)
+    L.check(L.lib().kfc_sandwich(H.handle, C.c_void_p(Vt.data_ptr()), int(k), int(N), G.ctypes.data_as(L._D)),
+            H.ctx.handle)
+    if cplx:
+        G = G.view(np.complex128)
+    return G.reshape((k, k), order="F")
+
+
+def port_matrix(mn, mode, port, device=None):
     """``_port_matrix`` (src/materialize.jl:443-459): V' O V with O the mode's port
-    on its axis (a basis child's localop or a node child's port)."""
+    on its axis (a basis child's localop or a node child's port).  ``device``:
+    form it with ``sandwich`` (default: when V has at least DEVICE_PORT_ENTRIES
+    entries)."""
     a = mn.node.child_of(mode)
     ch = mn.node.children[a]
     op = localop(ch[1], port) if isinstance(ch, tuple) else DenseOp(port_matrix(mn.children[a], mode, port))
     V = mn.vectors
+    if device is None:
+        device = V.size >= DEVICE_PORT_ENTRIES
+    if device:
+        return sandwich(V, mn.dims, [(a, op)])
     return V.conj().T @ _axis_apply(op, V, mn.dims, a)
 
 
-def joint_port(mn, factors):
+def joint_port(mn, factors, device=None):
     """``_joint_port`` (src/materialize.jl:490-518): V' (x_a O_a) V."""
     V = mn.vectors
-    W = V
+    ops = []
     for mode, port in factors:
         a = mn.node.child_of(mode)
         ch = mn.node.children[a]
         op = localop(ch[1], port) if isinstance(ch, tuple) else DenseOp(port_matrix(mn.children[a], mode, port))
+        ops.append((a, op))
+    if device is None:
+        device = V.size >= DEVICE_PORT_ENTRIES
+    if device:
+        return sandwich(V, mn.dims, ops)
+    W = V
+    for a, op in ops:
         W = _axis_apply(op, W, mn.dims, a)
     return V.conj().T @ W
 
