@@ -27,7 +27,8 @@ int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M,
               double beta);
 // Y[:, 0:q] = X[:, 0:p] M (M p x q host, column-major), Y may be columns of X (each row
 // of X is read before that row of Y is written).  gram = 1 also returns G = X[:, 0:pz]^H Y
-// (pz x q), gram = 2 returns G = Y^H Y (q x q), over all N rows of the result.
+// (pz x q), gram = 2 returns G = Y^H Y (q x q), gram = 3 returns G = [X[:, 0:pz] Y]^H Y
+// ((pz + q) x q, the new Y in its lower block), over all N rows of the result.
 template <class S>
 int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M, int q, S* Y, int64_t ldy, int64_t N,
               int gram, int pz, std::vector<cd>* G);

@@ -242,9 +242,12 @@ void gram_mfma(int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const
 // Y[:, 0:q] = X[:, 0:p] M for real blocks, p <= 80, q <= 64.  A block stages a
 // 64-row slab of X in LDS (as FP64), each wave forms the 16 x q rows of Y of its
 // quarter as Y^T = M^T X^T with v_mfma_f64_16x16x4f64 (so each store writes 16
-// consecutive rows of a column), and with GR = 1 or 2 also contracts the slab's
-// rows into G = X[:, 0:pz]^T Y or Y^T Y (q <= 16) while the slab is in LDS.  Y may
-// be columns of X: a slab is read completely before any of its rows is written.
+// consecutive rows of a column), and with GR = 1, 2 or 3 also contracts the slab's
+// rows into G = X[:, 0:pz]^T Y, Y^T Y or [X[:, 0:pz] Y]^T Y (q <= 16) while the slab
+// is in LDS.  Y may be columns of X: a slab is read completely before any of its
+// rows is written.  GR = 3 places the slab's Y rows (as stored) in columns
+// pz..pz+q-1 of the staged slab (pz + q <= p): each wave reads and then overwrites
+// only its own 16 rows there, so no other wave's product sees them.
 constexpr int TS_LDR = 65;
 
 template <class S, int TQ, int GR, int TPZ, int NPF>
@@ -305,7 +308,11 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
             for (int r = 0; r < 4; ++r) {
                 const int col = 16 * j + hi + 4 * r;
                 if (yrow < r1 && col < q) Y[yrow + col * ldy] = S(acc[j][r]);
-                if (GR != 0 && j == 0) sY[col * TS_LDR + 16 * wv + lo] = (yrow < r1 && col < q) ? acc[0][r] : 0.0;
+                if (GR != 0 && j == 0) {
+                    const double yv = (yrow < r1 && col < q) ? (GR == 3 ? double(S(acc[0][r])) : acc[0][r]) : 0.0;
+                    sY[col * TS_LDR + 16 * wv + lo] = yv;
+                    if (GR == 3 && col < q) sX[(pz + col) * TS_LDR + 16 * wv + lo] = yv;
+                }
             }
         if (GR != 0) {
             __syncthreads();
@@ -313,7 +320,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
             for (int u = 0; u < 4; ++u) {
                 const int rr = 4 * (wv + 4 * u) + hi;
                 const double b = sY[lo * TS_LDR + rr];
-                if (GR == 1) {
+                if (GR == 1 || GR == 3) {
 #pragma unroll
                     for (int z = 0; z < TPZ; ++z)
                         g[z] = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[(16 * z + lo) * TS_LDR + rr], b, g[z], 0, 0, 0);
@@ -326,7 +333,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
     if (GR == 0) return;
     // sum the waves' Gram tiles in LDS
     double* red = sm;
-    const int gp = GR == 1 ? pz : q;
+    const int gp = GR == 1 ? pz : (GR == 3 ? pz + q : q);
     __syncthreads();
     for (int w = 0; w < 4; ++w) {
         if (wv == w) {
@@ -381,6 +388,13 @@ void tsmm_dispatch(int gr, int tq, int tpz, int nchunk, size_t shmem, hipStream_
         }
     } else if (gr == 2) {
         KFC_TS(1, 2, 1);
+    } else if (gr == 3) {
+        switch (tpz) {
+            case 1: KFC_TS(1, 3, 1); break;
+            case 2: KFC_TS(1, 3, 2); break;
+            case 3: KFC_TS(1, 3, 3); break;
+            default: KFC_TS(1, 3, 4); break;
+        }
     } else {
         switch (tpz) {
             case 1: KFC_TS(1, 1, 1); break;
@@ -484,11 +498,13 @@ int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M,
 template <class S>
 int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M, int q, S* Y, int64_t ldy, int64_t N,
               int gram, int pz, std::vector<cd>* G) {
+    const int gp = gram == 1 ? pz : (gram == 3 ? pz + q : q);   // rows of G
     if (q == 0 || N == 0) {
-        if (G) G->assign(size_t(gram == 1 ? pz : q) * q, cd(0));
+        if (G) G->assign(size_t(gp) * q, cd(0));
         return KFC_OK;
     }
-    bool mfma = !IsCx<S>::value && p <= 80 && q <= 64 && (gram == 0 || (q <= 16 && pz <= 64 && pz <= p));
+    const int gz = gram == 1 || gram == 3 ? gp : 0;   // columns the Gram takes from the slab
+    bool mfma = !IsCx<S>::value && p <= 80 && q <= 64 && (gram == 0 || (q <= 16 && gz <= 64 && gz <= p));
     if (!mfma) {
         // general path: product into scratch, copy, then the Gram
         KFC_HIP(w.ctx, w.tmp.ensure(sizeof(S) * size_t(N) * q));
@@ -496,11 +512,22 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
         if (rc == KFC_OK) rc = blas_axpby<S>(w, w.tmp.as<S>(), N, Y, ldy, q, N, 1.0, 0.0);
         if (rc == KFC_OK && gram == 1) rc = blas_gram<S>(w, X, ldx, pz, Y, ldy, q, N, *G);
         if (rc == KFC_OK && gram == 2) rc = blas_gram<S>(w, Y, ldy, q, Y, ldy, q, N, *G);
+        if (rc == KFC_OK && gram == 3) {
+            std::vector<cd> G1, G2;
+            rc = blas_gram<S>(w, X, ldx, pz, Y, ldy, q, N, G1);
+            if (rc == KFC_OK) rc = blas_gram<S>(w, Y, ldy, q, Y, ldy, q, N, G2);
+            if (rc == KFC_OK) {
+                G->assign(size_t(gp) * q, cd(0));
+                for (int j = 0; j < q; ++j) {
+                    for (int i = 0; i < pz; ++i) (*G)[i + size_t(j) * gp] = G1[i + size_t(j) * pz];
+                    for (int i = 0; i < q; ++i) (*G)[pz + i + size_t(j) * gp] = G2[i + size_t(j) * q];
+                }
+            }
+        }
         return rc;
     }
     if constexpr (!IsCx<S>::value) {
-        const int tq = (q + 15) / 16, ppad = (p + 3) / 4 * 4, tpz = (pz + 15) / 16;
-        const int gp = gram == 1 ? pz : q;
+        const int tq = (q + 15) / 16, ppad = (p + 3) / 4 * 4, tpz = (gz + 15) / 16;
         const int nchunk = nchunks_for(N, 4096, 2048);
         const int64_t rows_per = (N + nchunk - 1) / nchunk;
         std::vector<double> h(size_t(p) * q);
@@ -513,7 +540,7 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
             KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(gp) * q));
         }
         size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 * tq + (gram ? 16 * TS_LDR : 0);
-        words = std::max<size_t>(words, size_t(gram == 1 ? tpz : 1) * 256);
+        words = std::max<size_t>(words, size_t(gz > 0 ? tpz : 1) * 256);
         tsmm_dispatch<S>(gram, tq, tpz, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
                          w.small2.as<double>(), q, Y, ldy, N, rows_per, pz, gram ? w.part.as<double>() : nullptr);
         KFC_HIP(w.ctx, hipGetLastError());

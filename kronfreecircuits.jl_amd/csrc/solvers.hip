@@ -403,12 +403,14 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
         const int nb0 = s + 1;
         std::vector<cd> pend_ri;   // a pending K <- K pend_ri (pend_n columns), see the Cholesky QR
         int pend_n = 0;
-        // block classical Gram-Schmidt against Q, twice (K <- K - Q Cm), the second
-        // and third reads of [Q K] fused with the next Gram: C1 = Q^H K;
-        // K -= Q C1 with C2 = Q^H K; K -= Q C2 with G = K^H K
-        std::vector<cd> Cm(size_t(k) * nb0, cd(0)), Gk;
+        // block classical Gram-Schmidt against Q, twice (K <- K - Q Cm): C1 = Q^H K;
+        // K -= Q C1 with [C2; Gk] = [Q K]^H K in the same pass.  The second
+        // subtraction K -= Q C2 is left to the first Cholesky QR scaling, which
+        // reads [Q K] anyway; its Gram follows from this pass's: with Q^H Q = I,
+        // (K - Q C2)^H (K - Q C2) = K^H K - C2^H C2.
+        std::vector<cd> Cm(size_t(k) * nb0, cd(0)), Gk, C2;
         if (k > 0) {
-            std::vector<cd> C1, C2;
+            std::vector<cd> C1;
             if (relation_exact) {
                 // The first projection follows from the Krylov relation without a pass
                 // over Q: Q^H H v = T Q^H v + conj(b) (q^H v), so Q^H K obeys the
@@ -445,10 +447,21 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                 }
                 return M;
             };
-            rc = blas_tsmm<S>(w, V, N, k + nb0, upd(C1), nb0, K, N, N, 1, k, &C2);
+            std::vector<cd> G1;
+            rc = blas_tsmm<S>(w, V, N, k + nb0, upd(C1), nb0, K, N, N, 3, k, &G1);
             if (rc != KFC_OK) return rc;
-            rc = blas_tsmm<S>(w, V, N, k + nb0, upd(C2), nb0, K, N, N, 2, 0, &Gk);
-            if (rc != KFC_OK) return rc;
+            const int gp = k + nb0;
+            C2.assign(size_t(k) * nb0, cd(0));
+            Gk.assign(size_t(nb0) * nb0, cd(0));
+            for (int j = 0; j < nb0; ++j)
+                for (int i = 0; i < k; ++i) C2[i + size_t(j) * k] = G1[i + size_t(j) * gp];
+            for (int j = 0; j < nb0; ++j)
+                for (int i = 0; i <= j; ++i) {
+                    cd acc = 0.5 * (G1[k + i + size_t(j) * gp] + std::conj(G1[k + j + size_t(i) * gp]));
+                    for (int l = 0; l < k; ++l) acc -= std::conj(C2[l + size_t(i) * k]) * C2[l + size_t(j) * k];
+                    Gk[i + size_t(j) * nb0] = acc;
+                    Gk[j + size_t(i) * nb0] = std::conj(acc);
+                }
             for (size_t i = 0; i < Cm.size(); ++i) Cm[i] = C1[i] + C2[i];
         } else {
             rc = blas_gram<S>(w, K, N, nb0, K, N, nb0, N, Gk);
@@ -498,6 +511,22 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             if (pass == 1 && !dep_last && k + s >= kd) {
                 pend_ri = Ri;
                 pend_n = nind;
+            } else if (pass == 0 && k > 0) {
+                // K[:, 0:nind] <- (K - Q C2)[:, 0:nind] Ri: [[-C2 Ri]; [Ri]] on [Q K]
+                const int pr = k + nind;
+                std::vector<cd> Mq(size_t(pr) * nind, cd(0));
+                for (int j = 0; j < nind; ++j) {
+                    for (int i = 0; i < k; ++i) {
+                        cd acc = 0;
+                        for (int l = 0; l <= j; ++l) acc -= C2[i + size_t(l) * k] * Ri[l + size_t(j) * nind];
+                        Mq[i + size_t(j) * pr] = acc;
+                    }
+                    for (int l = 0; l <= j; ++l) Mq[k + l + size_t(j) * pr] = Ri[l + size_t(j) * nind];
+                }
+                std::vector<cd> Gn;
+                rc = blas_tsmm<S>(w, V, N, pr, Mq, nind, K, N, N, 2, 0, &Gn);
+                if (rc != KFC_OK) return rc;
+                G.swap(Gn);
             } else {
                 std::vector<cd> Gn;
                 rc = blas_tsmm<S>(w, K, N, nind, Ri, nind, K, N, N, pass == 0 ? 2 : 0, 0, pass == 0 ? &Gn : nullptr);
