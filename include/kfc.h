@@ -153,6 +153,19 @@ int kfc_op_gershgorin(kfc_op* op, double* out);
 /* Replaces product_seed(H, k) (src/apply.jl:655-685): k orthonormal product
  * states into host X[prod(dims) x k] (complex interleaved for KFC_C128). */
 int kfc_op_product_seed(kfc_op* op, int64_t k, double* X);
+/* Replaces _device_diagonal / _full_diagonal(H) on the device (src/apply.jl:455-478):
+ * the real diagonal of H into device D[prod(dims)], formed on the device by the
+ * same term-by-term arithmetic as kfc_op_diagonal (equal bit for bit).  Returns
+ * after the kernel finished. */
+int kfc_op_diagonal_device(kfc_op* op, double* D);
+/* Replaces product_seed(H, k) (src/apply.jl:655-685) with the block built on the
+ * device: device X[prod(dims) x k], leading dimension ldx >= prod(dims), FP64 (or
+ * interleaved complex128 for KFC_C128); 1 <= k <= min(prod(dims), 65535).  The k
+ * smallest summed energies are found without forming the prod(dims) sums (a
+ * best-first walk of the per-axis orders); when every axis is already an eigenbasis,
+ * the k smallest diagonal entries are selected on the device (radix select).  Equal
+ * to kfc_op_product_seed bit for bit.  Returns after the kernels finished. */
+int kfc_op_product_seed_device(kfc_op* op, int64_t k, void* X, int64_t ldx);
 
 /* ---- solvers (the Solver plugin contract, src/solvers.jl:3-31) ------------------- */
 /* Each writes E[m] ascending and V[prod(dims) x m] (host, column-major,

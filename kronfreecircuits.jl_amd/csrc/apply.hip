@@ -24,7 +24,7 @@
 // ------------------------------------------------------------------------------------------
 // Host: the fused diagonal (src/apply.jl:232-246, _add_term_diagonal! 438-447)
 
-static std::vector<cd> factor_diagonal(const Factor& f) {
+std::vector<cd> factor_diagonal(const Factor& f) {
     // diagonal(op) (src/localops.jl:102-108)
     std::vector<cd> d(f.d, cd(0));
     if (f.kind == KFC_DIAG) return f.diag;
@@ -304,12 +304,11 @@ static int build_device(kfc_op* op, DeviceData& dd) {
         }
     }
     if (!P.ok || P.need_full_diag) {
-        op_host_diag(op);
-        std::vector<S> h(op->N);
-        for (int64_t i = 0; i < op->N; ++i) h[i] = s_from<S>(op->hdiag[i]);
-        hipError_t e = dd.diag.ensure(h.size() * sizeof(S));
-        if (e == hipSuccess) e = hipMemcpy(dd.diag.p, h.data(), h.size() * sizeof(S), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return kfc_fail(op->ctx, KFC_ENOMEM, std::string("diagonal upload: ") + hipGetErrorString(e));
+        // the fused diagonal, formed on the device
+        hipError_t e = dd.diag.ensure(size_t(op->N) * sizeof(S));
+        if (e != hipSuccess) return kfc_fail(op->ctx, KFC_ENOMEM, std::string("diagonal: ") + hipGetErrorString(e));
+        rc = op_fused_diag_dev<S>(op, dd.diag.as<S>());
+        if (rc != KFC_OK) return rc;
     }
     if (!P.ok && !op->pairs.ok) {
         dd.fdev.assign(op->terms.size(), {});

@@ -58,10 +58,10 @@ template <class S, class T>
 int blas_convert(Work& w, const S* X, int64_t ldx, T* Y, int64_t ldy, int p, int64_t N);
 // The LOBPCG seed perturbation on the device: the first ncols[b] columns (all p when
 // ncols is null) of each of `batch` blocks (stride bstride) get a random vector of
-// sqrt(eps) times their norm added, deterministic in (seed, column, row).
+// rel (default sqrt(eps)) times their norm added, deterministic in (seed, column, row).
 template <class S>
 int blas_perturb(Work& w, S* X, int64_t ldx, int p, int64_t N, int batch, int64_t bstride, const int* ncols,
-                 uint64_t seed);
+                 uint64_t seed, double rel = 1.4901161193847656e-08);
 // N x p standard normal (complex: both parts), deterministic in (seed, column offset)
 template <class S>
 int blas_randn(Work& w, S* X, int64_t ldx, int p, int64_t N, uint64_t seed);

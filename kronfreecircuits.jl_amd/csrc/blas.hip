@@ -799,7 +799,7 @@ __global__ void k_randn(S* X, int64_t ldx, int p, int64_t N, uint64_t seed) {
 // left as they are.
 template <class S>
 __global__ __launch_bounds__(256) void k_perturb(S* X, int64_t ldx, int64_t N, int64_t bstride,
-                                                 const int* __restrict__ ncols, uint64_t seed) {
+                                                 const int* __restrict__ ncols, uint64_t seed, double rel) {
     using R = typename RealOf<S>::type;
     const int j = blockIdx.x, bi = blockIdx.y;
     if (ncols && j >= ncols[bi]) return;   // the whole block leaves before any barrier
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(256) void k_perturb(S* X, int64_t ldx, int64_t N, i
         }
         __syncthreads();
     }
-    const double f = red[1][0] > 0 ? 1.4901161193847656e-08 * sqrt(red[0][0]) / sqrt(red[1][0]) : 0.0;
+    const double f = red[1][0] > 0 ? rel * sqrt(red[0][0]) / sqrt(red[1][0]) : 0.0;
     for (int64_t i = threadIdx.x; i < N; i += 256) {
         double a, b;
         normal2(ks + uint64_t(i), a, b);
@@ -835,10 +835,10 @@ __global__ __launch_bounds__(256) void k_perturb(S* X, int64_t ldx, int64_t N, i
 
 template <class S>
 int blas_perturb(Work& w, S* X, int64_t ldx, int p, int64_t N, int batch, int64_t bstride, const int* ncols,
-                 uint64_t seed) {
+                 uint64_t seed, double rel) {
     if (p <= 0 || N == 0 || batch <= 0) return KFC_OK;
     hipLaunchKernelGGL(k_perturb<S>, dim3(unsigned(p), unsigned(batch)), dim3(256), 0, w.ctx->stream, X, ldx, N,
-                       bstride, ncols, seed);
+                       bstride, ncols, seed, rel);
     KFC_HIP(w.ctx, hipGetLastError());
     return KFC_OK;
 }
@@ -1033,7 +1033,7 @@ int lz_update(Work& w, const S* V, int64_t ldv, int k, const std::vector<cd>& hs
     template int blas_diag_expect<S>(Work&, const S*, int64_t, int, int64_t, const double*, std::vector<double>&);  \
     template int blas_axpby<S>(Work&, const S*, int64_t, S*, int64_t, int, int64_t, double, double);                \
     template int blas_randn<S>(Work&, S*, int64_t, int, int64_t, uint64_t);                                         \
-    template int blas_perturb<S>(Work&, S*, int64_t, int, int64_t, int, int64_t, const int*, uint64_t);           \
+    template int blas_perturb<S>(Work&, S*, int64_t, int, int64_t, int, int64_t, const int*, uint64_t, double);   \
     template int lz_multidot<S>(Work&, const S*, int64_t, int, const double*, const S*, int64_t, std::vector<cd>&,   \
                                 double*);                                                                           \
     template int lz_update<S>(Work&, const S*, int64_t, int, const std::vector<cd>&, S*, int64_t, double*);

@@ -9,7 +9,6 @@
 
 #include "blas.h"
 
-int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
 
 namespace {
 
@@ -27,6 +26,9 @@ template <class S> constexpr int prec_of() { return std::is_same<typename RealOf
 template <class S> struct Lower { using type = S; };
 template <> struct Lower<double> { using type = float; };
 template <> struct Lower<Cx<double>> { using type = Cx<float>; };
+template <class S> struct Upper { using type = S; };
+template <> struct Upper<float> { using type = double; };
+template <> struct Upper<Cx<float>> { using type = Cx<double>; };
 
 struct CTimer {
     double apply_gpu = 0, apply = 0;
@@ -260,12 +262,13 @@ struct ChebCfg {
     double bound, tol, hbound, lmax;
 };
 
-// One ChFSI run on scalar S from the host seed X0h (N x nb); returns theta, X (host) and
-// the locked count.  switch_tol > 0 stops once every unlocked residual is below it.
+// One ChFSI run on scalar S from the seed (N x nb): the FP64 device block X0d, else the
+// host block X0h; returns theta, X (host) and the locked count.  switch_tol > 0 stops
+// once every unlocked residual is below it.
 template <class S>
 int cheb_run(kfc_op* op, const ChebCfg& c, const std::vector<cd>& X0h, uint64_t seed, CTimer& tm,
              std::vector<double>& theta_out, std::vector<cd>& Xout, int& nlock_out, int64_t& it_out, int& deg_out,
-             double switch_tol, bool seed_locked_ok) {
+             double switch_tol, bool seed_locked_ok, const typename Upper<S>::type* X0d = nullptr) {
     const int64_t N = op->N;
     const int nb = int(std::min<int64_t>(c.m + c.extra, N));
     const int m = c.m;
@@ -279,7 +282,10 @@ int cheb_run(kfc_op* op, const ChebCfg& c, const std::vector<cd>& X0h, uint64_t 
     S* bufs[3] = {b0.as<S>(), b1.as<S>(), b2.as<S>()};
     S* X = bX.as<S>();
     S* HX = bHX.as<S>();
-    {
+    if (X0d) {
+        const int rc0 = blas_convert<typename Upper<S>::type, S>(ch.w, X0d, N, bufs[0], N, nb, N);
+        if (rc0 != KFC_OK) return rc0;
+    } else {
         std::vector<S> h(size_t(N) * nb);
         for (size_t i = 0; i < h.size(); ++i) h[i] = s_from<S>(X0h[i]);
         KFC_HIP(op->ctx, hipMemcpy(bufs[0], h.data(), h.size() * sizeof(S), hipMemcpyHostToDevice));
@@ -415,29 +421,23 @@ int chebyshev_t(kfc_op* op, int64_t m, int degree, int64_t extra, int64_t maxite
     int rc = kfc_op_gershgorin(op, &hb);
     if (rc != KFC_OK) return rc;
     ChebCfg c{int(m), degree, int(extra), rescale_every, ortho, maxiter, bound, tol, hb, std::isnan(bound) ? hb : bound};
-    // seed: the product seed plus a 1e-2 random admixture (_admixed_seed, src/chebyshev.jl:252-261), or random
-    std::vector<cd> X0;
-    std::mt19937_64 g(0x4346 ^ seed);
-    std::normal_distribution<double> nd;
-    if (seed_kind == KFC_SEED_PRODUCT) {
-        rc = op_product_seed(op, nb, X0);
-        if (rc != KFC_OK) return rc;
-        for (int j = 0; j < nb; ++j) {
-            cd* col = &X0[size_t(j) * N];
-            std::vector<cd> r(N);
-            double ncn = 0, nr = 0;
-            for (int64_t i = 0; i < N; ++i) {
-                r[i] = op->cplx ? cd(nd(g), nd(g)) / std::sqrt(2.0) : cd(nd(g), 0.0);
-                ncn += std::norm(col[i]);
-                nr += std::norm(r[i]);
-            }
-            const double f = SEED_ADMIXTURE * std::sqrt(ncn) / std::sqrt(nr);
-            for (int64_t i = 0; i < N; ++i) col[i] += f * r[i];
+    // seed, formed on the device: the product seed plus a 1e-2 random admixture
+    // (_admixed_seed, src/chebyshev.jl:252-261), or a random block
+    DevBuf dX0;
+    KFC_HIP(op->ctx, dX0.ensure(sizeof(S) * size_t(N) * nb));
+    {
+        Work w0;
+        w0.ctx = op->ctx;
+        if (seed_kind == KFC_SEED_PRODUCT) {
+            rc = op_product_seed_dev<S>(op, nb, dX0.as<S>(), N);
+            if (rc == KFC_OK) rc = blas_perturb<S>(w0, dX0.as<S>(), N, nb, N, 1, 0, nullptr, 0x4346 ^ seed, SEED_ADMIXTURE);
+        } else {
+            rc = blas_randn<S>(w0, dX0.as<S>(), N, nb, N, 0x4346 ^ seed);
         }
-    } else {
-        X0.resize(size_t(N) * nb);
-        for (auto& x : X0) x = op->cplx ? cd(nd(g), nd(g)) : cd(nd(g), 0.0);
+        if (rc != KFC_OK) return rc;
+        KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
     }
+    const std::vector<cd> X0;
     CTimer tm;
     std::vector<double> th;
     std::vector<cd> X;
@@ -447,7 +447,7 @@ int chebyshev_t(kfc_op* op, int64_t m, int degree, int64_t extra, int64_t maxite
         using L = typename Lower<S>::type;
         const double sw = std::max(tol, 256.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
         int64_t it32 = 0;
-        rc = cheb_run<L>(op, c, X0, seed, tm, th, X, nlock, it32, deg, sw, false);
+        rc = cheb_run<L>(op, c, X0, seed, tm, th, X, nlock, it32, deg, sw, false, dX0.as<S>());
         if (rc != KFC_OK) return rc;
         ChebCfg c2 = c;
         c2.maxiter = std::max<int64_t>(1, maxiter - it32);
@@ -455,7 +455,7 @@ int chebyshev_t(kfc_op* op, int64_t m, int degree, int64_t extra, int64_t maxite
         rc = cheb_run<S>(op, c2, Xs, seed + 1, tm, th, X, nlock, it, deg, 0.0, false);
         it += it32;
     } else {
-        rc = cheb_run<S>(op, c, X0, seed, tm, th, X, nlock, it, deg, 0.0, false);
+        rc = cheb_run<S>(op, c, X0, seed, tm, th, X, nlock, it, deg, 0.0, false, dX0.as<S>());
     }
     if (rc != KFC_OK) return rc;
     // the lowest m locked pairs, checked with a fresh FP64 application (src/chebyshev.jl:395-405)

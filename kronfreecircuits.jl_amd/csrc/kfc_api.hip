@@ -309,14 +309,9 @@ extern "C" int kfc_op_gershgorin(kfc_op* op, double* out) {
                 dmin += *std::min_element(v.begin(), v.end());
             }
         } else {
-            op_host_diag(op);
-            dmax = -INFINITY;
-            dmin = INFINITY;
-            for (const cd& x : op->hdiag) {
-                dmax = std::max(dmax, x.real());
-                dmin = std::min(dmin, x.real());
-                imax = std::max(imax, std::fabs(x.imag()));
-            }
+            // the fused diagonal's range, reduced on the device
+            const int rc = op_fused_diag_range_dev(op, &dmax, &dmin, &imax);
+            if (rc != KFC_OK) return rc;
         }
         op->gershgorin_lo = dmin - imax - off;
         op->gershgorin = dmax + imax + off;
@@ -333,6 +328,18 @@ int op_spectral_interval(kfc_op* op, double* lo, double* hi) {
     return rc;
 }
 
+std::vector<cd> kept_factor_diagonal(const Factor& f) {
+    std::vector<cd> M, dd(f.d);
+    if (f.kind == KFC_DIAG) return f.diag;
+    if (f.kind == KFC_DENSE) {
+        for (int64_t i = 0; i < f.d; ++i) dd[i] = f.matrix[i + i * f.d];
+        return dd;
+    }
+    dense_of(f, M);
+    for (int64_t i = 0; i < f.d; ++i) dd[i] = M[i + i * f.d];
+    return dd;
+}
+
 int op_full_diagonal(kfc_op* op, std::vector<double>& D) {
     // _full_diagonal(H) (src/apply.jl:459-478): fused diagonal plus the kept terms' diagonals
     op_host_diag(op);
@@ -344,13 +351,7 @@ int op_full_diagonal(kfc_op* op, std::vector<double>& D) {
         std::vector<std::vector<cd>> dv;
         bool zero = false;
         for (const Factor& f : t.factors) {
-            std::vector<cd> M, dd(f.d);
-            if (f.kind == KFC_DIAG) dd = f.diag;
-            else if (f.kind == KFC_DENSE) for (int64_t i = 0; i < f.d; ++i) dd[i] = f.matrix[i + i * f.d];
-            else {
-                dense_of(f, M);
-                for (int64_t i = 0; i < f.d; ++i) dd[i] = M[i + i * f.d];
-            }
+            std::vector<cd> dd = kept_factor_diagonal(f);
             bool allz = true;
             for (const cd& x : dd) allz &= x == cd(0);
             zero |= allz;
@@ -441,18 +442,19 @@ int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X) {
     if (rc != KFC_OK) return rc;
     X.assign(size_t(N) * k, cd(0));
     bool all_ident = std::all_of(ident.begin(), ident.end(), [](bool b) { return b; });
-    // the k smallest values with the smallest index first among ties (a bounded max-heap)
+    // the k smallest values with the smallest index first among ties (a bounded max-heap),
+    // ordered as partialsortperm's isless orders them
     auto k_smallest = [&](auto value_of) {
-        std::priority_queue<std::pair<double, int64_t>> heap;
+        std::priority_queue<std::pair<uint64_t, int64_t>> heap;
         for (int64_t i = 0; i < N; ++i) {
-            const double v = value_of(i);
-            if ((int64_t)heap.size() < k) heap.push({v, i});
-            else if (std::make_pair(v, i) < heap.top()) {
+            const std::pair<uint64_t, int64_t> p{seed_order_key(value_of(i)), i};
+            if ((int64_t)heap.size() < k) heap.push(p);
+            else if (p < heap.top()) {
                 heap.pop();
-                heap.push({v, i});
+                heap.push(p);
             }
         }
-        std::vector<std::pair<double, int64_t>> out;
+        std::vector<std::pair<uint64_t, int64_t>> out;
         while (!heap.empty()) { out.push_back(heap.top()); heap.pop(); }
         std::sort(out.begin(), out.end());
         return out;

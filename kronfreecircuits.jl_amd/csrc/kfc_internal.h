@@ -225,6 +225,39 @@ template <class T>
 int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
                        hipStream_t st, void* host_tab, void* dev_tab);
 int op_host_diag(kfc_op* op);
+
+// ---- derived quantities (host: kfc_api.hip, apply.hip; device: seeds.hip) ----------------
+// The order in which Julia's isless sorts doubles (-0.0 before 0.0, NaN last) as an
+// unsigned key: the order of the product seed's selection.
+inline uint64_t seed_order_key(double d) {
+    if (d != d) return ~0ull;
+    uint64_t u;
+    std::memcpy(&u, &d, sizeof u);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+std::vector<cd> factor_diagonal(const Factor& f);        // a fused (diagonal) factor's entries
+std::vector<cd> kept_factor_diagonal(const Factor& f);   // the diagonal of a kept factor, dense(op)[i, i]
+int op_separable(kfc_op* op, std::vector<std::vector<cd>>& U, std::vector<std::vector<double>>& lam,
+                 std::vector<bool>& ident);
+int op_full_diagonal(kfc_op* op, std::vector<double>& D);
+int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
+// The k smallest of the Kronecker sum of the per-axis values (indices per axis), or
+// false when it cannot decide without scanning every sum.
+bool kron_sum_smallest(const std::vector<std::vector<double>>& lam, int64_t k, size_t cap,
+                       std::vector<std::vector<int64_t>>& out);
+// Device: the full real diagonal into D (N doubles) with its minimum and the count of
+// non-finite entries; the fused diagonal as S; the fused diagonal's max/min real part and
+// max |imaginary part|; the product seed into X (N x k, leading dimension ldx), S the
+// operator's FP64 scalar.  Each synchronizes the context's stream before returning.
+int op_full_diag_dev(kfc_op* op, double* D, double* dmin, int64_t* nonfinite);
+template <class S> int op_fused_diag_dev(kfc_op* op, S* out);
+int op_fused_diag_range_dev(kfc_op* op, double* dmax, double* dmin, double* imax);
+template <class S> int op_product_seed_dev(kfc_op* op, int64_t k, S* X, int64_t ldx);
+// The LOBPCG preconditioner's diagonal D (and 1 / (D - minimum(D) + 1) after it with
+// jacobi) in dD, with the :jacobi shift's checks.
+int op_precond_diag_dev(kfc_op* op, bool jacobi, DevBuf& dD);
+// out = 1 / (D - sigma) over N device entries
+int op_shift_inverse_dev(kfc_op* op, const double* D, double sigma, double* out);
 int op_ensure_device(kfc_op* op, int prec);
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st);
 // Chebyshev step fused into the apply: C = a*(H B) + b*B + g*A  (A may be null)

@@ -42,9 +42,10 @@
 
 #include "blas.h"
 
-int op_full_diagonal(kfc_op* op, std::vector<double>& D);
-int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
 int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& X0h);
+template <class S>
+int lobpcg_seed_dev(kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0, bool product, bool random_only,
+                    uint64_t s, S* Xd, int& nperturb);
 void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
                        const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X, int& nperturb);
 
@@ -958,50 +959,53 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
     if (pmode) KFC_HIP(ctx, dD.ensure(sizeof(double) * size_t(n) * N * 2));
     {
         std::vector<int> rcs(n, KFC_OK), npert(n, 0);
-        std::vector<std::vector<double>> Xh(n), Dh(n);
+        std::vector<std::vector<cd>> X0hs(n);
+        std::vector<std::vector<double>> Xh(n);
+        std::vector<char> devseed(n, 0);
         Team team0(std::max(1, std::min(16, int(std::thread::hardware_concurrency()))));
+        // warm starts checked and host-made blocks (random, or a full warm start) formed on
+        // host threads; product seeds are formed on the device below
         team0.run(n, [&](int i) {
             kfc_op* op = ops[i];
-            std::vector<cd> X0h, P, X;
-            if (X0s && X0s[i]) {
-                rcs[i] = lobpcg_check_x0(op, X0s[i], nx0, X0h);
+            const bool warm = X0s && X0s[i];
+            if (warm) {
+                rcs[i] = lobpcg_check_x0(op, X0s[i], nx0, X0hs[i]);
                 if (rcs[i] != KFC_OK) return;
             }
-            if (seed_kind == KFC_SEED_PRODUCT) {
-                rcs[i] = op_product_seed(op, W, P);
-                if (rcs[i] != KFC_OK) return;
-            }
-            lobpcg_seed_block(op, int(m), W, X0h, (X0s && X0s[i]) ? nx0 : 0, P,
-                              seed_kind == KFC_SEED_RANDOM && !(X0s && X0s[i]), seed + i, X, npert[i]);
+            devseed[i] = seed_kind == KFC_SEED_PRODUCT && !(warm && nx0 == W);
+            if (devseed[i]) return;
+            std::vector<cd> X;
+            lobpcg_seed_block(op, int(m), W, X0hs[i], warm ? nx0 : 0, std::vector<cd>(),
+                              seed_kind == KFC_SEED_RANDOM && !warm, seed + i, X, npert[i]);
             Xh[i].resize(X.size());
             for (size_t k = 0; k < X.size(); ++k) Xh[i][k] = X[k].real();
-            if (pmode) {
-                op_full_diagonal(op, Dh[i]);
-                Dh[i].resize(2 * N);
-                double dmin = INFINITY;
-                for (int64_t k = 0; k < N; ++k) dmin = std::min(dmin, Dh[i][k]);
-                const double sigma = dmin - 1;
-                for (int64_t k = 0; k < N; ++k) Dh[i][N + k] = 1.0 / (Dh[i][k] - sigma);
-            }
         });
         for (int i = 0; i < n; ++i)
             if (rcs[i] != KFC_OK) return kfc_fail(ctx, rcs[i], "instance " + std::to_string(i) + ": " + ops[i]->ctx->err);
+        for (int i = 0; i < n; ++i) {
+            kfc_op* op = ops[i];
+            pts[i].op = op;
+            double* Xd = S64[0].as<double>() + size_t(i) * 3 * W * N;
+            if (devseed[i]) {
+                rc = lobpcg_seed_dev<double>(op, int(m), W, X0hs[i], (X0s && X0s[i]) ? nx0 : 0, true, false, seed + i, Xd,
+                                             npert[i]);
+                if (rc != KFC_OK) return kfc_fail(ctx, rc, "instance " + std::to_string(i) + ": " + op->ctx->err);
+            } else {
+                KFC_HIP(ctx, hipMemcpy(Xd, Xh[i].data(), sizeof(double) * N * W, hipMemcpyHostToDevice));
+            }
+            if (pmode) {
+                // D of every point first, then the FixedJacobi inverses 1 / (D - minimum(D) + 1)
+                double* Di = dD.as<double>() + size_t(i) * N;
+                double dmin = INFINITY;
+                rc = op_full_diag_dev(op, Di, &dmin, nullptr);
+                if (rc == KFC_OK) rc = op_shift_inverse_dev(op, Di, dmin - 1, dD.as<double>() + size_t(n + i) * N);
+                if (rc != KFC_OK) return kfc_fail(ctx, rc, "instance " + std::to_string(i) + ": " + op->ctx->err);
+            }
+        }
         // the seeded columns of every point perturbed on the device, one launch
         DevBuf dnp;
         KFC_HIP(ctx, dnp.ensure(sizeof(int) * n));
         KFC_HIP(ctx, hipMemcpy(dnp.p, npert.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-        for (int i = 0; i < n; ++i) {
-            pts[i].op = ops[i];
-            KFC_HIP(ctx, hipMemcpy(S64[0].as<double>() + size_t(i) * 3 * W * N, Xh[i].data(), sizeof(double) * N * W,
-                                   hipMemcpyHostToDevice));
-            if (pmode) {
-                // D of every point first, then the FixedJacobi inverses
-                KFC_HIP(ctx, hipMemcpy(dD.as<double>() + size_t(i) * N, Dh[i].data(), sizeof(double) * N,
-                                       hipMemcpyHostToDevice));
-                KFC_HIP(ctx, hipMemcpy(dD.as<double>() + size_t(n + i) * N, Dh[i].data() + N, sizeof(double) * N,
-                                       hipMemcpyHostToDevice));
-            }
-        }
         Work wp;
         wp.ctx = ctx;
         rc = blas_perturb<double>(wp, S64[0].as<double>(), N, W, N, n, int64_t(3) * W * N, dnp.as<int>(), 0x4b46);

@@ -11,10 +11,12 @@
 #include "blas.h"
 
 int op_full_diagonal(kfc_op* op, std::vector<double>& D);
-int op_product_seed(kfc_op* op, int64_t k, std::vector<cd>& X);
 int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& X0h);
 void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
                        const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X, int& nperturb);
+template <class S>
+int lobpcg_seed_dev(kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0, bool product, bool random_only,
+                    uint64_t s, S* Xd, int& nperturb);
 
 namespace {
 
@@ -24,6 +26,10 @@ inline double secs(clk::time_point a) { return std::chrono::duration<double>(clk
 template <class S> struct Lower { using type = S; };
 template <> struct Lower<double> { using type = float; };
 template <> struct Lower<Cx<double>> { using type = Cx<float>; };
+
+template <class S> struct Upper { using type = S; };
+template <> struct Upper<float> { using type = double; };
+template <> struct Upper<Cx<float>> { using type = Cx<double>; };
 
 template <class S> constexpr int prec_of() { return std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1; }
 
@@ -931,7 +937,8 @@ bool gen_eig(int n, const std::vector<cd>& GA, const std::vector<cd>& GB, std::v
 
 template <class S>
 int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>& X0h, const double* Ddev,
-                Timer& tm, LobpcgRes& out, double switch_tol, int nperturb = 0) {
+                Timer& tm, LobpcgRes& out, double switch_tol, int nperturb = 0,
+                const typename Upper<S>::type* X0d = nullptr) {
     const int64_t N = op->N;
     const int W = cfg.w;
     DevBuf sA, sB, aA, aB, tmpb;
@@ -947,7 +954,9 @@ int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>
     S* AS2 = aB.as<S>();
     S* tmp = tmpb.as<S>();
     auto col = [&](S* b, int c) { return b + size_t(c) * N; };
-    int rc = upload_block<S>(op->ctx, X0h, N, W, Sb, N);
+    // the initial block: from the device seed (FP64, converted) or from the host
+    int rc = X0d ? blas_convert<typename Upper<S>::type, S>(w, X0d, N, Sb, N, W, N)
+                 : upload_block<S>(op->ctx, X0h, N, W, Sb, N);
     if (rc == KFC_OK) rc = blas_perturb<S>(w, Sb, N, nperturb, N, 1, 0, nullptr, 0x4b46);
     if (rc != KFC_OK) return rc;
     int kx = 0;
@@ -1182,37 +1191,20 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
         int rc = lobpcg_check_x0(op, X0, nx0, X0h);
         if (rc != KFC_OK) return rc;
     }
-    std::vector<cd> P;
-    if (seed_kind == KFC_SEED_PRODUCT) {
-        int rc = op_product_seed(op, W, P);
-        if (rc != KFC_OK) return rc;
-    }
-    // preconditioner diagonal on the device: D (and 1/(D - sigma) for :jacobi)
-    std::vector<double> D;
+    // preconditioner diagonal, formed on the device: D (and 1/(D - sigma) for :jacobi)
     DevBuf dD;
     if (precond != KFC_PRECOND_NONE) {
-        op_full_diagonal(op, D);
-        if (precond == KFC_PRECOND_JACOBI) {
-            double dmin = INFINITY;
-            for (double x : D) {
-                if (!std::isfinite(x)) return kfc_fail(op->ctx, KFC_EARG, "FixedJacobi: the diagonal has non-finite entries");
-                dmin = std::min(dmin, x);
-            }
-            const double sigma = dmin - 1;
-            if (!(sigma < dmin))
-                return kfc_fail(op->ctx, KFC_EARG, "FixedJacobi: the shift minimum(D) - 1 is not representable below minimum(D)");
-            const size_t n0 = D.size();
-            D.resize(2 * n0);
-            for (size_t i = 0; i < n0; ++i) D[n0 + i] = 1.0 / (D[i] - sigma);
-        }
-        KFC_HIP(op->ctx, dD.ensure(sizeof(double) * D.size()));
-        KFC_HIP(op->ctx, hipMemcpy(dD.p, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
+        int rc = op_precond_diag_dev(op, precond == KFC_PRECOND_JACOBI, dD);
+        if (rc != KFC_OK) return rc;
     }
+    // the seed block in device memory: the product seed is formed there, warm starts
+    // and random blocks are uploaded
+    DevBuf dX;
+    KFC_HIP(op->ctx, dX.ensure(sizeof(S) * size_t(N) * W));
     int nperturb = 0;
     auto seed_block = [&](bool random_only, uint64_t s) {
-        std::vector<cd> X;
-        lobpcg_seed_block(op, int(m), W, X0h, X0 ? nx0 : 0, P, random_only, s, X, nperturb);
-        return X;
+        return lobpcg_seed_dev<S>(op, int(m), W, X0h, X0 ? nx0 : 0, seed_kind == KFC_SEED_PRODUCT, random_only, s,
+                                  dX.as<S>(), nperturb);
     };
     Work w;
     w.ctx = op->ctx;
@@ -1221,7 +1213,9 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
     const double* Dp = dD.as<double>();
     LobpcgRes res;
     bool retried = false;
-    auto run = [&](const std::vector<cd>& Xs) -> int {
+    const std::vector<cd> none;
+    auto run = [&](int rc) -> int {
+        if (rc != KFC_OK) return rc;
         res = LobpcgRes();
         if (mode == KFC_MODE_MIXED) {
             using L = typename Lower<S>::type;
@@ -1229,7 +1223,7 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
             kfc_op_gershgorin(op, &hb);
             const double sw = std::max(tol, 256.0 * 5.96e-8 * std::max(std::fabs(hb), 1.0));
             LobpcgRes r32;
-            int rc = lobpcg_core<L>(op, w, cfg, Xs, Dp, tm, r32, sw, nperturb);
+            rc = lobpcg_core<L>(op, w, cfg, none, Dp, tm, r32, sw, nperturb, dX.as<S>());
             if (rc != KFC_OK) return rc;
             if (r32.breakdown) {
                 res = r32;
@@ -1241,7 +1235,7 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
             res.iterations += r32.iterations;
             return rc;
         }
-        return lobpcg_core<S>(op, w, cfg, Xs, Dp, tm, res, 0.0, nperturb);
+        return lobpcg_core<S>(op, w, cfg, none, Dp, tm, res, 0.0, nperturb, dX.as<S>());
     };
     int rc = run(seed_block(seed_kind == KFC_SEED_RANDOM && !X0, seed));
     if (rc != KFC_OK) return rc;
@@ -1330,6 +1324,52 @@ int lobpcg_check_x0(kfc_op* op, const double* X0, int64_t nx0, std::vector<cd>& 
 // m-column warm start from the product seed (else random); a random block when
 // neither exists or random_only (the retry).  The seeded columns (the first
 // nperturb) are perturbed on the device after upload (blas_perturb, seed 0x4b46).
+// The initial LOBPCG block of W columns in device memory (src/lobpcg.jl:236-262): the
+// caller's X0 (m columns, or W with the guards), the product seed's guard columns or
+// random ones after a partial X0, the product seed, or a random block.  nperturb is
+// the number of leading columns the seed perturbation applies to.
+template <class S>
+int lobpcg_seed_dev(kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0, bool product, bool random_only,
+                    uint64_t s, S* Xd, int& nperturb) {
+    const int64_t N = op->N;
+    const int g = W - m;
+    nperturb = 0;
+    std::vector<cd> R;
+    if (random_only) {
+        random_block(R, N, W, op->cplx, s);
+        return upload_block<S>(op->ctx, R, N, W, Xd, N);
+    }
+    if (nx0 == W) {   // the warm start carries its guards
+        nperturb = W;
+        return upload_block<S>(op->ctx, X0h, N, W, Xd, N);
+    }
+    if (nx0 > 0) {
+        nperturb = m;
+        int rc = KFC_OK;
+        if (product && g > 0) {
+            rc = op_product_seed_dev<S>(op, W, Xd, N);
+            nperturb = W;
+        } else if (g > 0) {
+            random_block(R, N, g, op->cplx, s);
+            rc = upload_block<S>(op->ctx, R, N, g, Xd + size_t(m) * N, N);
+        }
+        if (rc != KFC_OK) return rc;
+        return upload_block<S>(op->ctx, X0h, N, m, Xd, N);
+    }
+    if (product) {
+        nperturb = W;
+        return op_product_seed_dev<S>(op, W, Xd, N);
+    }
+    random_block(R, N, W, op->cplx, s);
+    return upload_block<S>(op->ctx, R, N, W, Xd, N);
+}
+
+
+template int lobpcg_seed_dev<double>(kfc_op*, int, int, const std::vector<cd>&, int64_t, bool, bool, uint64_t, double*,
+                                     int&);
+template int lobpcg_seed_dev<Cx<double>>(kfc_op*, int, int, const std::vector<cd>&, int64_t, bool, bool, uint64_t,
+                                         Cx<double>*, int&);
+
 void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0h, int64_t nx0,
                        const std::vector<cd>& P, bool random_only, uint64_t s, std::vector<cd>& X, int& nperturb) {
     const int64_t N = op->N;

@@ -96,6 +96,8 @@ SIGNATURES = {
     "kfc_op_diagonal": (C.c_int, [_P, _D]),
     "kfc_op_gershgorin": (C.c_int, [_P, _D]),
     "kfc_op_product_seed": (C.c_int, [_P, C.c_int64, _D]),
+    "kfc_op_diagonal_device": (C.c_int, [_P, _P]),
+    "kfc_op_product_seed_device": (C.c_int, [_P, C.c_int64, _P, C.c_int64]),
     "kfc_lanczos": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, _D, _D,
                               C.POINTER(Stats)]),
     "kfc_lobpcg": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int32, C.c_double, C.c_int32, _D,

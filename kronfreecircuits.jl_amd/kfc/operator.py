@@ -199,6 +199,25 @@ class KronFreeOperator:
         L.check(L.lib().kfc_op_product_seed(self.handle, int(k), buf.ctypes.data_as(L._D)), self.ctx.handle)
         return unpack_block(buf, self.N, k, self.dtype)
 
+    def diagonal_device(self):
+        """``_full_diagonal(H)`` formed on the device (kfc_op_diagonal_device): a torch
+        float64 tensor of length N on this operator's device."""
+        import torch
+        D = torch.empty(self.N, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        L.check(L.lib().kfc_op_diagonal_device(self.handle, C.c_void_p(D.data_ptr())), self.ctx.handle)
+        return D
+
+    def product_seed_device(self, k):
+        """``product_seed(H, k)`` formed on the device (kfc_op_product_seed_device): an
+        N x k torch tensor (column-major storage, float64 or complex128) on this
+        operator's device."""
+        import torch
+        dt = torch.complex128 if self.dtype == np.complex128 else torch.float64
+        X = torch.empty(int(k), self.N, dtype=dt, device=f"cuda:{self.ctx.device}")
+        L.check(L.lib().kfc_op_product_seed_device(self.handle, int(k), C.c_void_p(X.data_ptr()), self.N),
+                self.ctx.handle)
+        return X.t()
+
 
 def unpack_block(buf, N, k, dtype):
     if dtype == np.complex128:
