@@ -232,6 +232,114 @@ def ports_bench(args):
     print(json.dumps(res))
 
 
+def seeds_bench(args):
+    """Cold-start inputs of the block solvers at 256^3 (16.8 M states): the full
+    diagonal and the product seed (k = 16) of the three-fluxonia operator, and the
+    unit-vector seed of an operator whose axes are eigenbases (radix select of the
+    20 smallest of 16.8 M diagonal entries), each formed in device memory
+    (kfc_op_diagonal_device, kfc_op_product_seed_device) against the host forms
+    (kfc_op_diagonal, kfc_op_product_seed) that fill host memory."""
+    import numpy as np
+    import torch
+
+    import kfc
+    from kfc import configs
+
+    n = args.seed_n
+    H = configs.three_fluxonia_operator(configs.three_fluxonia_theta(args.seed), N=n)
+    rng = np.random.default_rng(1)
+    C0 = rng.standard_normal((n, n))
+    np.fill_diagonal(C0, 0.0)
+    Hu = kfc.KronFreeOperator([n] * 3, [
+        kfc.AxisTerm(1.0, [(a, kfc.DiagonalOp(rng.uniform(1, 2, n))) for a in range(3)]),
+        kfc.AxisTerm(0.1, [(0, kfc.BandedOp(n, [-1, 1], [np.ones(n), np.ones(n)])),
+                           (2, kfc.BandedOp(n, [-1, 1], [np.ones(n), np.ones(n)]))])])
+    N = H.N
+
+    def dev_time(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, out
+
+    def host_time(fn):
+        t0 = time.perf_counter()
+        out = fn()
+        return time.perf_counter() - t0, out
+
+    k, ku = 16, 20
+    t_dd, Dd = dev_time(H.diagonal_device, args.steps)
+    t_dh, Dh = host_time(H.diagonal)
+    t_sd, Pd = dev_time(lambda: H.product_seed_device(k), args.steps)
+    t_sh, Ph = host_time(lambda: H.product_seed(k))
+    t_ud, Ud = dev_time(lambda: Hu.product_seed_device(ku), args.steps)
+    t_uh, Uh = host_time(lambda: Hu.product_seed(ku))
+    same = bool(np.array_equal(Dd.cpu().numpy(), Dh) and np.array_equal(Pd.cpu().numpy(), Ph)
+                and np.array_equal(Ud.cpu().numpy(), Uh))
+    log(f"[seeds] diagonal {t_dd * 1e3:.2f} ms (host {t_dh * 1e3:.0f} ms) | product seed k={k} {t_sd * 1e3:.2f} ms "
+        f"(host {t_sh * 1e3:.0f} ms) | unit seed k={ku} {t_ud * 1e3:.2f} ms (host {t_uh * 1e3:.0f} ms) | equal {same}")
+    res = {"metric": f"product seeds/s (three fluxonia {n}^3, k = {k}, formed in device memory)",
+           "value": round(1.0 / t_sd, 2), "unit": "seeds/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": round(1e3 * t_sd, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "seeds", "dims": [n] * 3, "k": k, "k_unit": ku},
+           "roofline": {"bound": "hbm", "achieved": round(N * k * 8 / t_sd / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(N * k * 8 / t_sd / 1e9 / 8000.0, 4), "traffic": None, "kernel": "k_seed_product",
+                        "algorithmic_bytes_per_launch": N * k * 8},
+           "device_ms": {"diagonal": round(t_dd * 1e3, 3), "product_seed": round(t_sd * 1e3, 3),
+                         "unit_seed": round(t_ud * 1e3, 3)},
+           "host_ms": {"diagonal": round(t_dh * 1e3, 1), "product_seed": round(t_sh * 1e3, 1),
+                       "unit_seed": round(t_uh * 1e3, 1)},
+           "diagonal_gbs": round(N * 8 / t_dd / 1e9, 1), "device_equals_host": same}
+    print(json.dumps(res))
+
+
+def drive_bench(args):
+    """The matrix-free apply of time evolution (KronFreeApply, kfc_apply_affine): w =
+    alpha 2 pi (H - E0) v + w for complex kets on the real three-fluxonia operator
+    (--seed-n points per mode), one ket and a block of 4.  Algorithmic bytes per ket:
+    read v and w, write w (48 N)."""
+    import math
+
+    import torch
+
+    import kfc
+    from kfc import configs
+
+    n = args.seed_n
+    H = configs.three_fluxonia_operator(configs.three_fluxonia_theta(args.seed), N=n)
+    N = H.N
+    A = kfc.KronFreeApply(H, 2 * math.pi, -2.5)
+    out = {}
+    for b in (1, 4):
+        shape = (N,) if b == 1 else (b, N)
+        v = torch.randn(shape, dtype=torch.complex128, device="cuda")
+        w = torch.randn(shape, dtype=torch.complex128, device="cuda")
+        A(w, v, -1j, 1.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            A(w, v, -1j, 1.0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        out[b] = (dt, 48.0 * N * b / dt / 1e9)
+        log(f"[drive] {b} ket(s): {dt * 1e3:.3f} ms, {out[b][1]:.0f} GB/s algorithmic")
+    dt, gbs = out[4]
+    res = {"metric": f"kets/s of w = alpha 2pi (H - E0) v + w (three fluxonia {n}^3, real H, complex kets)",
+           "value": round(4 / dt, 2), "unit": "kets/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c128", "data": "synthetic (seeded circuit parameters, random kets)",
+           "config": {"workload": "drive", "dims": [n] * 3, "block": 4},
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(gbs / 8000.0, 4), "traffic": None, "kernel": "kfc_apply_affine (pack, stencil, combine)",
+                        "algorithmic_bytes_per_launch": 48 * N * 4},
+           "single_ket_ms": round(out[1][0] * 1e3, 3), "single_ket_gbs": round(out[1][1], 1)}
+    print(json.dumps(res))
+
+
 def transmon_bench(args):
     """512 coupled-transmon instances on periodic grids ({256..768} points per
     mode, complex leaves keeping 8, tops keeping 20 = 10 240 eigenpairs): plain
@@ -285,7 +393,8 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many cycles")
     ap.add_argument("--workload", default="three_fluxonia",
-                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports"])
+                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports", "seeds", "drive"])
+    ap.add_argument("--seed-n", type=int, default=256, help="seeds and drive workloads: points per mode")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise copy of known size (the reference for HBM byte counters)")
@@ -310,6 +419,10 @@ def main():
         return sweep_bench(args)
     if args.workload == "ports":
         return ports_bench(args)
+    if args.workload == "seeds":
+        return seeds_bench(args)
+    if args.workload == "drive":
+        return drive_bench(args)
     if args.workload == "transmon_sweep":
         return transmon_bench(args)
     import numpy as np

@@ -167,6 +167,27 @@ int kfc_op_diagonal_device(kfc_op* op, double* D);
  * to kfc_op_product_seed bit for bit.  Returns after the kernels finished. */
 int kfc_op_product_seed_device(kfc_op* op, int64_t k, void* X, int64_t ldx);
 
+/* ---- matrix-free time evolution (the port_operator consumers) -------------------- */
+/* Replaces KronFreeApply (ext/KronFreeCircuitsQuantumToolboxExt.jl:44-87), the apply
+ * behind drive_hamiltonian(...; basis=:product, frame=:lab): W = alpha scale
+ * (H - shift) V + beta W for complex128 device blocks V, W (interleaved, prod(dims)
+ * x ncols, leading dimension ld, 1 <= ncols <= 4096, W not overlapping V).  A real H
+ * on the fused stencil path runs the stencil on complex tables (one pass when beta is
+ * zero); any other real H applies once to the real block [Re V | Im V].  beta == 0
+ * leaves W unread.  Enqueued on the context stream (kfc_sync waits). */
+int kfc_apply_affine(kfc_op* op, const void* V, void* W, int64_t ncols, int64_t ld, double alpha_re, double alpha_im,
+                     double scale, double shift, double beta_re, double beta_im);
+/* Replaces InteractionApply (ext/KronFreeCircuitsQuantumToolboxExt.jl:211-251), the
+ * apply behind drive_hamiltonian(...; basis=:product, frame=:interaction): with
+ * ph = exp(-i Omega t), W = alpha conj(ph) .* [2 pi (H0 - E0) - diag Omega +
+ * sum_k f_k 2 pi O_k] (ph .* V) + beta W.  Omega is a device vector of prod(dims)
+ * doubles (2 pi (diag(H0) - E0) in the reference), f holds nops complex coefficients
+ * (re, im), ops the drive operators (port_operator) on H0's context and product
+ * basis.  Enqueued on the context stream (kfc_sync waits). */
+int kfc_apply_interaction(kfc_op* H0, double E0, const double* Omega, double t, kfc_op* const* ops, const double* f,
+                          int32_t nops, const void* V, void* W, int64_t ncols, int64_t ld, double alpha_re,
+                          double alpha_im, double beta_re, double beta_im);
+
 /* ---- solvers (the Solver plugin contract, src/solvers.jl:3-31) ------------------- */
 /* Each writes E[m] ascending and V[prod(dims) x m] (host, column-major,
  * complex interleaved for KFC_C128) and fills stats (may be NULL). */

@@ -903,10 +903,20 @@ __global__ __launch_bounds__(256) void k_stencil_a(const StArgs<S> a) {
     stencil_sweep<S, DP, true>(a);
 }
 
+// The device tables an apply on scalar S reads: the operator's own, or, for complex
+// blocks on a real operator, its promoted complex tables.
+template <class S>
+static DeviceData& dd_for(kfc_op* op) {
+    if constexpr (IsCx<S>::value) {
+        if (!op->cplx) return op->ddz;
+    }
+    return op->dd[std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1];
+}
+
 template <class S>
 static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld,
                           cd alpha, cd beta, cd gamma, hipStream_t st) {
-    const DeviceData& dd = op->dd[std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1];
+    const DeviceData& dd = dd_for<S>(op);
     const StencilPlan& P = op->plan;
     StArgs<S> a;
     std::memset(&a, 0, sizeof(a));
@@ -1038,8 +1048,7 @@ static unsigned grid_for(int64_t total) {
 
 template <class S>
 static int launch_chain(kfc_op* op, const void* Xv, void* Yv, int64_t ncols, int64_t ld, hipStream_t st) {
-    const int prec = std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1;
-    DeviceData& dd = op->dd[prec];
+    DeviceData& dd = dd_for<S>(op);
     const int64_t N = op->N, total = N * ncols;
     const S* X = static_cast<const S*>(Xv);
     S* Y = static_cast<S*>(Yv);
@@ -1149,10 +1158,11 @@ static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t nc
                    cd gamma, hipStream_t st) {
     if (op->plan.ok) {
         const int prec = std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1;
-        if (!op->stencil_pinned && op->tuned_cols[prec] != ncols) {
+        const int slot = (IsCx<S>::value && !op->cplx) ? 2 : prec;
+        if (!op->stencil_pinned && op->tuned_cols[slot] != ncols) {
             int rc = tune_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
             if (rc != KFC_OK) return rc;
-            op->tuned_cols[prec] = ncols;
+            op->tuned_cols[slot] = ncols;
         }
         return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
     }
@@ -1188,6 +1198,31 @@ static int dispatch(kfc_op* op, const void* X, const void* A, void* Y, int64_t n
 
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st) {
     return dispatch(op, X, nullptr, Y, ncols, ld, prec, cd(1), cd(0), cd(0), st);
+}
+
+int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
+                      hipStream_t st) {
+    if (op->cplx || !op->plan.ok)
+        return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "promoted apply: a real operator on the fused stencil path only");
+    if (ncols == 0 || op->N == 0) return KFC_OK;
+    if (!op->ddz.built) {
+        const int rc = build_device<Cx<double>>(op, op->ddz);
+        if (rc != KFC_OK) return rc;
+    }
+    // the promoted apply keeps its own tuned launch shape
+    const int v = op->stencil_variant, m = op->stencil_march;
+    if (op->variant_z >= 0 && !op->stencil_pinned) {
+        op->stencil_variant = op->variant_z;
+        op->stencil_march = op->march_z;
+    }
+    const int rc = apply_t<Cx<double>>(op, X, nullptr, Y, ncols, ld, alpha, beta, cd(0), st);
+    if (!op->stencil_pinned) {
+        op->variant_z = op->stencil_variant;
+        op->march_z = op->stencil_march;
+        op->stencil_variant = v;
+        op->stencil_march = m;
+    }
+    return rc;
 }
 
 int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec, double a,

@@ -1,0 +1,211 @@
+// Matrix-free applies for time evolution on a node's product basis, the consumers of
+// port_operator (src/portoperator.jl:22-39) in the QuantumToolbox extension
+// (ext/KronFreeCircuitsQuantumToolboxExt.jl):
+//   * KronFreeApply (:44-87): W = alpha scale (H - shift) V + beta W on complex kets or
+//     blocks; a real H on the fused stencil path runs the stencil on complex tables
+//     (alpha scale and the shift in its epilogue, one pass when beta is zero), any other
+//     real H applies once to the real block [Re V | Im V] as the reference does;
+//   * InteractionApply (:211-251): W = alpha e^{iOmega t} [2 pi (H0 - E0) - diag Omega
+//     + sum_k f_k 2 pi O_k] e^{-iOmega t} V + beta W.
+// The operator applies are the fused kernels of kfc_apply (the scale and shift ride in
+// the Chebyshev-step epilogue y = a H x + b x); packing, phases and the final
+// combination are elementwise kernels here.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kfc_internal.h"
+
+namespace {
+
+int evo_grid(int64_t n) { return int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
+
+// Xr[:, j] = Re V[:, j], Xr[:, m + j] = Im V[:, j]  (Xr leading dimension N)
+__global__ __launch_bounds__(256) void k_pack_ri(const double2* __restrict__ V, int64_t ld, int64_t N, int m,
+                                                 double* __restrict__ Xr) {
+    const int64_t tot = N * m;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+        const int64_t j = e / N, i = e - j * N;
+        const double2 v = V[i + j * ld];
+        Xr[i + j * N] = v.x;
+        Xr[i + (j + m) * N] = v.y;
+    }
+}
+
+// W = alpha Y - d .* V + beta W, Y complex (leading dimension ldy) or, with PACKED, the
+// real block [Re Y | Im Y] (leading dimension N); beta == 0 leaves W unread.
+template <bool PACKED>
+__global__ __launch_bounds__(256) void k_affine_combine(const void* __restrict__ Yp, int64_t ldy,
+                                                        const double2* __restrict__ V, double2* __restrict__ W,
+                                                        int64_t ld, int64_t N, int m, double ar, double ai,
+                                                        double br, double bi, int use_beta,
+                                                        const double* __restrict__ d) {
+    const int64_t tot = N * m;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+        const int64_t j = e / N, i = e - j * N;
+        double yr, yi;
+        if constexpr (PACKED) {
+            const double* Y = static_cast<const double*>(Yp);
+            yr = Y[i + j * N];
+            yi = Y[i + (j + m) * N];
+        } else {
+            const double2 y = static_cast<const double2*>(Yp)[i + j * ldy];
+            yr = y.x;
+            yi = y.y;
+        }
+        double rr = ar * yr - ai * yi, ri = ar * yi + ai * yr;
+        if (d) {
+            const double2 v = V[i + j * ld];
+            rr -= d[i] * v.x;
+            ri -= d[i] * v.y;
+        }
+        if (use_beta) {
+            const double2 w = W[i + j * ld];
+            rr += br * w.x - bi * w.y;
+            ri += br * w.y + bi * w.x;
+        }
+        W[i + j * ld] = make_double2(rr, ri);
+    }
+}
+
+// X = e^{-i Omega t} .* V  (X leading dimension N)
+__global__ __launch_bounds__(256) void k_phase_in(const double* __restrict__ Om, double t, const double2* __restrict__ V,
+                                                  int64_t ld, int64_t N, int m, double2* __restrict__ X) {
+    const int64_t tot = N * m;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+        const int64_t j = e / N, i = e - j * N;
+        double s, c;
+        sincos(-Om[i] * t, &s, &c);
+        const double2 v = V[i + j * ld];
+        X[i + j * N] = make_double2(c * v.x - s * v.y, c * v.y + s * v.x);
+    }
+}
+
+// W = alpha e^{+i Omega t} .* acc + beta W  (acc leading dimension N)
+__global__ __launch_bounds__(256) void k_phase_out(const double* __restrict__ Om, double t,
+                                                   const double2* __restrict__ acc, int64_t N, int m, double ar,
+                                                   double ai, double br, double bi, int use_beta,
+                                                   double2* __restrict__ W, int64_t ld) {
+    const int64_t tot = N * m;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+        const int64_t j = e / N, i = e - j * N;
+        double s, c;
+        sincos(-Om[i] * t, &s, &c);   // conj(e^{-i Omega t}) = c - i s
+        const double2 a = acc[i + j * N];
+        const double pr = c * a.x + s * a.y, pi = c * a.y - s * a.x;
+        double rr = ar * pr - ai * pi, ri = ar * pi + ai * pr;
+        if (use_beta) {
+            const double2 w = W[i + j * ld];
+            rr += br * w.x - bi * w.y;
+            ri += br * w.y + bi * w.x;
+        }
+        W[i + j * ld] = make_double2(rr, ri);
+    }
+}
+
+// W = alpha scale (H - shift) V - d .* V + beta W for complex128 blocks V, W (N x m,
+// leading dimension ld).  Y scratch comes from the operator.
+int affine_apply(kfc_op* op, const double2* V, double2* W, int64_t m, int64_t ld, cd alpha, double scale, double shift,
+                 cd beta, const double* d) {
+    const int64_t N = op->N;
+    hipStream_t st = op->ctx->stream;
+    const int use_beta = beta != cd(0);
+    const int nb = evo_grid(N * m);
+    if (!op->cplx && op->plan.ok) {
+        // the fused stencil on complex tables: (alpha scale) H V - (alpha scale shift) V in
+        // its epilogue, straight into W when nothing else is added
+        const cd a = alpha * scale, b = -alpha * scale * shift;
+        if (!use_beta && !d) return op_apply_promoted(op, V, W, m, ld, a, b, st);
+        KFC_HIP(op->ctx, op->evo[0].ensure(sizeof(double2) * size_t(ld) * m));
+        double2* Y = op->evo[0].as<double2>();
+        int rc = op_apply_promoted(op, V, Y, m, ld, a, b, st);
+        if (rc != KFC_OK) return rc;
+        hipLaunchKernelGGL(k_affine_combine<false>, dim3(nb), dim3(256), 0, st, Y, ld, V, W, ld, N, int(m), 1.0, 0.0,
+                           beta.real(), beta.imag(), use_beta, d);
+    } else if (op->cplx) {
+        KFC_HIP(op->ctx, op->evo[0].ensure(sizeof(double2) * size_t(ld) * m));
+        double2* Y = op->evo[0].as<double2>();
+        int rc = op_apply_cheb(op, V, nullptr, Y, m, ld, KFC_PREC_F64, scale, -scale * shift, 0.0, st);
+        if (rc != KFC_OK) return rc;
+        hipLaunchKernelGGL(k_affine_combine<false>, dim3(nb), dim3(256), 0, st, Y, ld, V, W, ld, N, int(m),
+                           alpha.real(), alpha.imag(), beta.real(), beta.imag(), use_beta, d);
+    } else {
+        KFC_HIP(op->ctx, op->evo[0].ensure(sizeof(double) * size_t(N) * 2 * m));
+        KFC_HIP(op->ctx, op->evo[1].ensure(sizeof(double) * size_t(N) * 2 * m));
+        double* Xr = op->evo[0].as<double>();
+        double* Yr = op->evo[1].as<double>();
+        hipLaunchKernelGGL(k_pack_ri, dim3(nb), dim3(256), 0, st, V, ld, N, int(m), Xr);
+        KFC_HIP(op->ctx, hipGetLastError());
+        int rc = op_apply_cheb(op, Xr, nullptr, Yr, 2 * m, N, KFC_PREC_F64, scale, -scale * shift, 0.0, st);
+        if (rc != KFC_OK) return rc;
+        hipLaunchKernelGGL(k_affine_combine<true>, dim3(nb), dim3(256), 0, st, Yr, N, V, W, ld, N, int(m),
+                           alpha.real(), alpha.imag(), beta.real(), beta.imag(), use_beta, d);
+    }
+    KFC_HIP(op->ctx, hipGetLastError());
+    return KFC_OK;
+}
+
+int check_block(kfc_op* op, const void* V, const void* W, int64_t m, int64_t ld, const char* who) {
+    if (!V || !W) return kfc_fail(op->ctx, KFC_EARG, std::string(who) + ": null block");
+    if (m < 1 || m > 4096) return kfc_fail(op->ctx, KFC_EARG, std::string(who) + ": the block width must lie in 1:4096");
+    if (ld < op->N) return kfc_fail(op->ctx, KFC_EARG, std::string(who) + ": the leading dimension is below the dimension");
+    return KFC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kfc_apply_affine(kfc_op* op, const void* V, void* W, int64_t ncols, int64_t ld, double alpha_re, double alpha_im,
+                     double scale, double shift, double beta_re, double beta_im) {
+    if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_apply_affine: null operator");
+    int rc = check_block(op, V, W, ncols, ld, "kfc_apply_affine");
+    if (rc != KFC_OK) return rc;
+    if (op->N == 0) return KFC_OK;
+    const char* vb = static_cast<const char*>(V);
+    const char* wb = static_cast<const char*>(W);
+    const size_t span = size_t((ncols - 1) * ld + op->N) * sizeof(double2);
+    if (vb < wb + span && wb < vb + span) return kfc_fail(op->ctx, KFC_EARG, "kfc_apply_affine: w must not alias v");
+    return affine_apply(op, static_cast<const double2*>(V), static_cast<double2*>(W), ncols, ld,
+                        cd(alpha_re, alpha_im), scale, shift, cd(beta_re, beta_im), nullptr);
+}
+
+int kfc_apply_interaction(kfc_op* H0, double E0, const double* Omega, double t, kfc_op* const* ops, const double* f,
+                          int32_t nops, const void* V, void* W, int64_t ncols, int64_t ld, double alpha_re,
+                          double alpha_im, double beta_re, double beta_im) {
+    if (!H0 || !Omega) return kfc_fail(H0 ? H0->ctx : nullptr, KFC_EARG, "kfc_apply_interaction: null argument");
+    int rc = check_block(H0, V, W, ncols, ld, "kfc_apply_interaction");
+    if (rc != KFC_OK) return rc;
+    if (nops < 0 || (nops > 0 && (!ops || !f)))
+        return kfc_fail(H0->ctx, KFC_EARG, "kfc_apply_interaction: drives need operators and coefficients");
+    for (int k = 0; k < nops; ++k) {
+        if (!ops[k] || ops[k]->N != H0->N)
+            return kfc_fail(H0->ctx, KFC_EDIM, "kfc_apply_interaction: drive " + std::to_string(k) +
+                                                   " is not an operator on the same product basis");
+        if (ops[k]->ctx != H0->ctx)
+            return kfc_fail(H0->ctx, KFC_EARG, "kfc_apply_interaction: every operator must be on one context");
+    }
+    const int64_t N = H0->N;
+    if (N == 0) return KFC_OK;
+    hipStream_t st = H0->ctx->stream;
+    const int nb = evo_grid(N * ncols);
+    KFC_HIP(H0->ctx, H0->evo[2].ensure(sizeof(double2) * size_t(N) * ncols));
+    KFC_HIP(H0->ctx, H0->evo[3].ensure(sizeof(double2) * size_t(N) * ncols));
+    double2* X = H0->evo[2].as<double2>();
+    double2* acc = H0->evo[3].as<double2>();
+    hipLaunchKernelGGL(k_phase_in, dim3(nb), dim3(256), 0, st, Omega, t, static_cast<const double2*>(V), ld, N,
+                       int(ncols), X);
+    KFC_HIP(H0->ctx, hipGetLastError());
+    // acc = 2 pi (H0 - E0) X - Omega .* X, then + f_k 2 pi O_k X
+    const double tp = 2.0 * M_PI;
+    rc = affine_apply(H0, X, acc, ncols, N, cd(1), tp, E0, cd(0), Omega);
+    for (int k = 0; rc == KFC_OK && k < nops; ++k)
+        rc = affine_apply(ops[k], X, acc, ncols, N, cd(f[2 * k], f[2 * k + 1]), tp, 0.0, cd(1), nullptr);
+    if (rc != KFC_OK) return rc;
+    hipLaunchKernelGGL(k_phase_out, dim3(nb), dim3(256), 0, st, Omega, t, acc, N, int(ncols), alpha_re, alpha_im,
+                       beta_re, beta_im, int(beta_re != 0 || beta_im != 0), static_cast<double2*>(W), ld);
+    KFC_HIP(H0->ctx, hipGetLastError());
+    return KFC_OK;
+}
+
+}  // extern "C"

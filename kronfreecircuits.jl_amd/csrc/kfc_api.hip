@@ -198,7 +198,8 @@ int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march) {
     if (variant < -1 || variant > 3) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: variant must lie in -1:3");
     if (march < 0 || march > 4096) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: march must lie in 0:4096");
     if (variant < 0) {
-        op->tuned_cols[0] = op->tuned_cols[1] = -1;
+        op->tuned_cols[0] = op->tuned_cols[1] = op->tuned_cols[2] = -1;
+        op->variant_z = -1;
         op->stencil_pinned = false;
         return KFC_OK;
     }
@@ -383,6 +384,12 @@ extern "C" int kfc_op_diagonal(kfc_op* op, double* out) {
 // sum of single-axis terms (kept and fused), the constant added to axis 0.
 int op_separable(kfc_op* op, std::vector<std::vector<cd>>& U, std::vector<std::vector<double>>& lam,
                  std::vector<bool>& ident) {
+    if (op->have_separable) {
+        U = op->sep_U;
+        lam = op->sep_lam;
+        ident = op->sep_ident;
+        return KFC_OK;
+    }
     const int K = op->ndims;
     std::vector<std::vector<cd>> A(K);
     for (int a = 0; a < K; ++a) A[a].assign(size_t(op->dims[a]) * op->dims[a], cd(0));
@@ -428,6 +435,10 @@ int op_separable(kfc_op* op, std::vector<std::vector<cd>>& U, std::vector<std::v
             herm_eig(int(d), Hm, lam[a], U[a]);
         }
     }
+    op->sep_U = U;
+    op->sep_lam = lam;
+    op->sep_ident = ident;
+    op->have_separable = true;
     return KFC_OK;
 }
 

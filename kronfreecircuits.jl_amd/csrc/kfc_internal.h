@@ -200,12 +200,21 @@ struct kfc_op {
     int stencil_variant = 1;        // 0: LDS plane tile, 1: register queue (chosen by the tuner)
     int stencil_march = 0;          // march chunk along the slowest axis (0: default)
     bool stencil_pinned = false;    // launch shape set by the caller: no autotuning
-    int64_t tuned_cols[2] = {-1, -1};   // column count the stencil variant was tuned for, per precision
+    int64_t tuned_cols[3] = {-1, -1, -1};   // column count the stencil variant was tuned for: FP64, FP32,
+                                            // and the complex (promoted) apply of a real operator
+    int variant_z = -1, march_z = 0;        // the promoted apply's own tuned launch shape
     PairPlan pairs;
+    DevBuf evo[4];                  // scratch of the time-evolution applies (evolve.hip)
     DeviceData dd[2];               // [KFC_PREC_F64], [KFC_PREC_F32]
+    DeviceData ddz;                 // complex FP64 tables of a real stencil operator (complex kets)
     double gershgorin = 0;
     double gershgorin_lo = 0;       // lower end of the Gershgorin interval
     bool have_gershgorin = false;
+    // separable_eigenbases, memoized as the reference memoizes H.separable
+    bool have_separable = false;
+    std::vector<std::vector<cd>> sep_U;
+    std::vector<std::vector<double>> sep_lam;
+    std::vector<bool> sep_ident;
 };
 
 // Gershgorin interval [lo, hi] holding the spectrum (kfc_api.hip)
@@ -260,6 +269,10 @@ int op_precond_diag_dev(kfc_op* op, bool jacobi, DevBuf& dD);
 int op_shift_inverse_dev(kfc_op* op, const double* D, double sigma, double* out);
 int op_ensure_device(kfc_op* op, int prec);
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st);
+// Y = alpha H X + beta X for complex128 blocks and a real operator on the fused stencil
+// path (op->plan.ok), the stencil running on complex tables (op->ddz).
+int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
+                      hipStream_t st);
 // Chebyshev step fused into the apply: C = a*(H B) + b*B + g*A  (A may be null)
 int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec,
                   double a, double b, double g, hipStream_t st);

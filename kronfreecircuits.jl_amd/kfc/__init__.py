@@ -8,11 +8,13 @@ from .operator import AxisTerm, KronFreeOperator
 from .solvers import (LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, residuals, resolve, solve, solve_batch, solve_chebyshev_batch,
                       solve_stats)
 from .tree import (MaterializedNode, Node, NumericSpec, System, circuit_terms, constant_term, materialize, operator,
-                   port_matrix)
+                   port_matrix, port_operator)
+from .evolution import DriveHamiltonian, InteractionApply, KronFreeApply, evolve_rk4
 
 __all__ = ["Context", "ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
            "UnsupportedError", "context", "lib", "missing_symbols", "ChargeBasis", "FockBasis", "PeriodicGrid",
            "PhaseGrid", "fd_weights", "localop", "BandedOp", "DenseOp", "DiagonalOp", "LocalOp", "ShiftOp", "dense",
            "imagpart", "realpart", "AxisTerm", "KronFreeOperator", "LOBPCG", "Auto", "ChebyshevFilter", "Dense",
            "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_batch", "solve_chebyshev_batch", "solve_stats", "MaterializedNode", "Node",
-           "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "operator", "port_matrix"]
+           "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "operator", "port_matrix",
+           "port_operator", "DriveHamiltonian", "InteractionApply", "KronFreeApply", "evolve_rk4"]

@@ -97,6 +97,10 @@ SIGNATURES = {
     "kfc_op_gershgorin": (C.c_int, [_P, _D]),
     "kfc_op_product_seed": (C.c_int, [_P, C.c_int64, _D]),
     "kfc_op_diagonal_device": (C.c_int, [_P, _P]),
+    "kfc_apply_affine": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_double]),
+    "kfc_apply_interaction": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, _P, C.c_int32, _P, _P, C.c_int64,
+                                        C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double]),
     "kfc_op_product_seed_device": (C.c_int, [_P, C.c_int64, _P, C.c_int64]),
     "kfc_lanczos": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, _D, _D,
                               C.POINTER(Stats)]),

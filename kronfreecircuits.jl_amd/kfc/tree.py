@@ -188,6 +188,48 @@ def joint_port(mn, factors, device=None):
     return V.conj().T @ W
 
 
+def _hermitian(M):
+    M = np.asarray(M)
+    return np.allclose(M, M.conj().T, rtol=1e-10, atol=1e-12 * max(1.0, float(np.max(np.sum(np.abs(M), axis=1)))))
+
+
+def port_operator(mn, key, ctx=None):
+    """``port_operator(sys, key, node)`` (src/portoperator.jl:22-39): the port
+    ``key = (mode, port)``, or the joint port ``[(mode, port), ...]`` of factors on
+    distinct modes, as a KronFreeOperator on the product basis of the materialized
+    node ``mn`` (its children's kept states, axis 0 fastest), identity on the axes the
+    key does not touch.  Each touched axis carries its child's operator: a basis
+    child's localop, or a node child's port (its joint port when several factors fall
+    in that child).  Real exactly when every factor is real; Hermitian when every
+    factor is, to rounding."""
+    keys = [key] if isinstance(key, tuple) else list(key)
+    if not keys:
+        raise L.ArgumentError("port_operator: the key is empty")
+    modes = mn.node.modes()
+    for m, _ in keys:
+        if m not in modes:
+            raise L.ArgumentError(f"mode :{m} is not a mode of the node on modes {modes}")
+    if len({m for m, _ in keys}) != len(keys):
+        raise L.ArgumentError(f"port_operator: the factors of a joint key must be on distinct modes, got {keys}")
+    groups = {}
+    for m, p in keys:
+        groups.setdefault(mn.node.child_of(m), []).append((m, p))
+    ops = []
+    for a in sorted(groups):
+        ch, fs = mn.node.children[a], groups[a]
+        if isinstance(ch, tuple):
+            op = localop(ch[1], fs[0][1])
+        elif len(fs) == 1:
+            op = DenseOp(port_matrix(mn.children[a], *fs[0]))
+        else:
+            op = DenseOp(joint_port(mn.children[a], fs))
+        ops.append((a, op))
+    cplx = any(np.iscomplexobj(dense(op)) for _, op in ops)
+    herm = all(_hermitian(dense(op)) for _, op in ops)
+    term = AxisTerm(1.0 + 0j if cplx else 1.0, ops)
+    return KronFreeOperator(list(mn.dims), [term], constant=0.0, hermitian=herm, ctx=ctx)
+
+
 def node_axisterms(node, kids, terms, real):
     """The AxisTerms of a node (``_node_operator`` / ``_real_node_terms``,
     src/materialize.jl:583-647): on a real node every routed term's
