@@ -340,6 +340,62 @@ def drive_bench(args):
     print(json.dumps(res))
 
 
+def sternheimer_bench(args):
+    """The derivatives' linear solve on the complement (kfc_complement_solve,
+    src/resolvent.jl:146-256): the three-fluxonia node at --seed-n^3 keeps its lowest
+    10 states (Lanczos), then d psi_n / d E_J1 for all 10 kept states in one lockstep
+    solve (shift E_n, right-hand side -Q dH psi_n, Jacobi preconditioner, tol 1e-10)."""
+    import numpy as np
+    import torch
+
+    import kfc
+    from kfc import configs
+    from kfc.resolvent import Complement, Resolvent, apply_resolvent
+
+    n = args.seed_n
+    th = configs.three_fluxonia_theta(args.seed)
+    H = configs.three_fluxonia_operator(th, N=n)
+    h = 1e-3
+    thp, thm = dict(th), dict(th)
+    thp["E_J"] = [th["E_J"][0] + h] + list(th["E_J"][1:])
+    thm["E_J"] = [th["E_J"][0] - h] + list(th["E_J"][1:])
+    Hp, Hm = configs.three_fluxonia_operator(thp, N=n), configs.three_fluxonia_operator(thm, N=n)
+    m = 10
+    t0 = time.perf_counter()
+    E, V = kfc.solve(kfc.Lanczos(tol=1e-10), H, m)
+    t_solve = time.perf_counter() - t0
+    Vt = torch.from_numpy(np.ascontiguousarray(V.T)).to("cuda")
+    R = ((Hp @ Vt) - (Hm @ Vt)) / (2 * h)      # dH/dE_J1 psi_n (H is linear in E_J1)
+    Hp.sync(), Hm.sync()
+    R = R.cpu().numpy().T
+    log_ = []
+    res = Resolvent(np.asarray(E), V, Complement(H, H.diagonal_device(), Vt, 1e-10, 20000, log_), 1e-9)
+    apply_resolvent(res, R[:, :2], E[:2], [[0], [1]])      # warm-up (tuner, buffers)
+    log_.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    W, clashes = apply_resolvent(res, R, E, [[j] for j in range(m)])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rec = log_[-1]
+    it = rec["iterations"]
+    # per CG iteration: one apply of the 10 columns (2 N s each) and about 16 block
+    # passes of N x 10 (projections, dots, updates); the apply dominates the flops
+    N = H.N
+    log(f"[sternheimer] {n}^3: Lanczos {t_solve:.2f} s; 10 columns {dt:.2f} s, {it} CG iterations, "
+        f"{dt / it * 1e3:.2f} ms per iteration, residual {rec['residual']:.2e}")
+    out = {"metric": f"derivative columns/s (three fluxonia {n}^3, d psi_n / d E_J1 of the 10 kept states)",
+           "value": round(m / dt, 3), "unit": "columns/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(dt * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "sternheimer", "dims": [n] * 3, "kept": m, "tol": 1e-10},
+           "cg_iterations": it, "ms_per_iteration": round(dt / it * 1e3, 3),
+           "max_true_residual": rec["residual"], "lanczos_seconds": round(t_solve, 2),
+           "from_start": rec["from_start"], "at_floor": rec["at_floor"],
+           "finite": bool(np.all(np.isfinite(W)))}
+    print(json.dumps(out))
+
+
 def transmon_bench(args):
     """512 coupled-transmon instances on periodic grids ({256..768} points per
     mode, complex leaves keeping 8, tops keeping 20 = 10 240 eigenpairs): plain
@@ -393,8 +449,9 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many cycles")
     ap.add_argument("--workload", default="three_fluxonia",
-                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports", "seeds", "drive"])
-    ap.add_argument("--seed-n", type=int, default=256, help="seeds and drive workloads: points per mode")
+                    choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports", "seeds", "drive",
+                             "sternheimer"])
+    ap.add_argument("--seed-n", type=int, default=256, help="seeds, drive and sternheimer workloads: points per mode")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise copy of known size (the reference for HBM byte counters)")
@@ -423,6 +480,8 @@ def main():
         return seeds_bench(args)
     if args.workload == "drive":
         return drive_bench(args)
+    if args.workload == "sternheimer":
+        return sternheimer_bench(args)
     if args.workload == "transmon_sweep":
         return transmon_bench(args)
     import numpy as np

@@ -167,6 +167,25 @@ int kfc_op_diagonal_device(kfc_op* op, double* D);
  * to kfc_op_product_seed bit for bit.  Returns after the kernels finished. */
 int kfc_op_product_seed_device(kfc_op* op, int64_t k, void* X, int64_t ldx);
 
+/* ---- the linear solve on the complement (the derivatives' Sternheimer step) ------- */
+/* Replaces _complement_solve(H, D, basis, B, shifts; tol, maxiter, eta, scale)
+ * (src/resolvent.jl:146-256): X[:, j] solves Q (H - shifts[j]) Q x = Q B[:, j] with x
+ * orthogonal to the k orthonormal columns of basis (Q = 1 - basis basis^H), by
+ * preconditioned CG on all n columns in lockstep (1 <= n <= 64), preconditioner
+ * 1 / hypot(D - shifts[j], eta), the reference's stopping rules (each column to
+ * tol |Q B_j|, or to the round-off floor 1000 eps scale[j]).  Device: D (prod(dims)
+ * doubles, H's real diagonal), basis (prod(dims) x k), B and X (prod(dims) x n), the
+ * operator's FP64 scalar, leading dimension prod(dims).  Host: shifts[n], scale[n] (NULL:
+ * the column norms of B), and the outputs iterations, residuals[n] (true residual
+ * relative to |Q B_j| for a column that met tol, else 0), flags[n] (bit 0 met tol,
+ * bit 1 converged from the start with x = 0, bit 2 stopped at the round-off floor) and
+ * floor_residuals[n] (relative to scale[j] for a column at the floor, else 0); the
+ * output arrays may be NULL.  KFC_EBREAKDOWN: H - shifts[j] is not positive on the
+ * complement; KFC_ENOCONV: a column did not converge. */
+int kfc_complement_solve(kfc_op* op, const double* D, const void* basis, int64_t k, const void* B, int64_t n,
+                         const double* shifts, double tol, int64_t maxiter, double eta, const double* scale, void* X,
+                         int64_t* iterations, double* residuals, int32_t* flags, double* floor_residuals);
+
 /* ---- matrix-free time evolution (the port_operator consumers) -------------------- */
 /* Replaces KronFreeApply (ext/KronFreeCircuitsQuantumToolboxExt.jl:44-87), the apply
  * behind drive_hamiltonian(...; basis=:product, frame=:lab): W = alpha scale

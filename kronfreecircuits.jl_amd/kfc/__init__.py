@@ -10,6 +10,7 @@ from .solvers import (LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, res
 from .tree import (MaterializedNode, Node, NumericSpec, System, circuit_terms, constant_term, materialize, operator,
                    port_matrix, port_operator)
 from .evolution import DriveHamiltonian, InteractionApply, KronFreeApply, evolve_rk4
+from .resolvent import Resolvent, apply_resolvent, complement_solve, resolvent
 
 __all__ = ["Context", "ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
            "UnsupportedError", "context", "lib", "missing_symbols", "ChargeBasis", "FockBasis", "PeriodicGrid",
@@ -17,4 +18,5 @@ __all__ = ["Context", "ArgumentError", "BreakdownError", "ConvergenceError", "De
            "imagpart", "realpart", "AxisTerm", "KronFreeOperator", "LOBPCG", "Auto", "ChebyshevFilter", "Dense",
            "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_batch", "solve_chebyshev_batch", "solve_stats", "MaterializedNode", "Node",
            "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "operator", "port_matrix",
-           "port_operator", "DriveHamiltonian", "InteractionApply", "KronFreeApply", "evolve_rk4"]
+           "port_operator", "DriveHamiltonian", "InteractionApply", "KronFreeApply", "evolve_rk4",
+           "Resolvent", "apply_resolvent", "complement_solve", "resolvent"]

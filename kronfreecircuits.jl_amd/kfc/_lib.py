@@ -97,6 +97,8 @@ SIGNATURES = {
     "kfc_op_gershgorin": (C.c_int, [_P, _D]),
     "kfc_op_product_seed": (C.c_int, [_P, C.c_int64, _D]),
     "kfc_op_diagonal_device": (C.c_int, [_P, _P]),
+    "kfc_complement_solve": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.c_int64, _D, C.c_double, C.c_int64, C.c_double,
+                                       _D, _P, C.POINTER(C.c_int64), _D, C.POINTER(C.c_int32), _D]),
     "kfc_apply_affine": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_double]),
     "kfc_apply_interaction": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, _P, C.c_int32, _P, _P, C.c_int64,

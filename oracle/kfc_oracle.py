@@ -885,6 +885,126 @@ def residual_floor(hbound, E1):
 
 
 # ----------------------------------------------------------------------------------------
+# The linear solve on the complement of a node's known states (the Sternheimer step of
+# the derivatives), src/resolvent.jl:119-256.  Pinned in tests/test_oracle.py against the
+# spectral solution x_j = sum over the states outside the basis of
+# psi_l <psi_l|Q b_j> / (E_l - s_j) on dense matrices, and against its own stopping rules.
+
+FLOOR_CAP = 1000       # src/resolvent.jl:129 (_FLOOR_CAP), units of eps |R_j|
+FLOOR_PATIENCE = 5     # src/resolvent.jl:131 (_FLOOR_PATIENCE)
+
+
+class ComplementError(Exception):
+    """The solve's ErrorException (src/resolvent.jl:199-203, 235-239, 246-252)."""
+
+
+def _colredot(A, B):
+    """Re<A_j, B_j> per column (src/resolvent.jl:124)."""
+    return np.real(np.sum(np.conj(A) * B, axis=0))
+
+
+def complement_solve(apply, D, basis, B, shifts, tol, maxiter, eta=0.01, scale=None):
+    """``_complement_solve`` (src/resolvent.jl:146-256): X[:, j] solves
+    Q (H - s_j) Q x = Q B[:, j] with x orthogonal to ``basis`` (Q = 1 - basis basis'),
+    by preconditioned CG on all columns in lockstep, each with its own scalars,
+    preconditioned by 1 / hypot(D - s_j, eta).  ``apply(X)`` is H X.  Returns
+    (X, iterations, residuals, from_start, at_floor, floor_residuals), indices 0-based."""
+    B = np.asarray(B)
+    n = B.shape[1]
+    shifts = np.asarray(shifts, dtype=float)
+    if len(shifts) != n:
+        raise ValueError(f"{n} columns and {len(shifts)} shifts")
+    scale = np.linalg.norm(B, axis=0) if scale is None else np.asarray(scale, dtype=float)
+    Phi = np.asarray(basis)
+
+    def project(Y):
+        return Y - Phi @ (Phi.conj().T @ Y)
+
+    S = shifts[None, :]
+    Minv = 1.0 / np.hypot(np.asarray(D)[:, None] - S, eta)
+    cap = FLOOR_CAP * np.finfo(float).eps * scale
+    QB = project(B.copy())
+    b = np.linalg.norm(QB, axis=0)
+    goal = tol * b
+    from_start = [j for j in range(n) if b[j] <= cap[j]]
+    met = np.zeros(n, dtype=bool)
+    atfloor = np.zeros(n, dtype=bool)
+    atfloor[from_start] = True
+    QB = QB * (~atfloor)[None, :]
+    X = np.zeros_like(QB)
+    R = QB.copy()
+    Z = project(Minv * R)
+    P = Z.copy()
+    rho = _colredot(R, Z)
+
+    def true_residuals():
+        T = apply(X) - X * S
+        T = project(T) - QB
+        return np.linalg.norm(T, axis=0)
+
+    rn = b.copy()
+    anchor = np.full(n, np.inf)
+    stalled = np.zeros(n, dtype=int)
+    it = 0
+    while not np.all(met | atfloor) and it < maxiter:
+        it += 1
+        AP = project(apply(P) - P * S)
+        kappa = _colredot(P, AP)
+        for j in range(n):
+            if met[j] or atfloor[j] or kappa[j] > 0:
+                continue
+            if rn[j] <= cap[j]:
+                atfloor[j] = True
+                continue
+            raise ComplementError(f"the linear solve on the complement is not positive on the complement at column {j + 1}")
+        active = ~(met | atfloor)
+        alpha = np.where(active, rho / np.where(active, kappa, 1.0), 0.0)
+        X = X + P * alpha[None, :]
+        R = R - AP * alpha[None, :]
+        rn = np.linalg.norm(R, axis=0)
+        met |= active & (rn <= goal)
+        near = ~(met | atfloor) & (rn <= cap)
+        if np.any(near):
+            tr = true_residuals()
+            for j in np.flatnonzero(near):
+                if tr[j] < anchor[j] / 2:
+                    anchor[j], stalled[j] = tr[j], 0
+                else:
+                    stalled[j] += 1
+                if stalled[j] >= FLOOR_PATIENCE and tr[j] <= cap[j]:
+                    atfloor[j] = True
+        if np.all(met | atfloor):
+            break
+        Z = project(Minv * R)
+        rhon = _colredot(R, Z)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            beta = np.where(met | atfloor, 0.0, rhon / rho)
+        P = Z + P * beta[None, :]
+        rho = rhon
+    tr = true_residuals()
+    for j in range(n):
+        if not (met[j] or atfloor[j]) and tr[j] <= cap[j]:
+            atfloor[j] = True
+    if not np.all(met | atfloor):
+        j = int(np.flatnonzero(~(met | atfloor))[0])
+        raise ComplementError(f"the linear solve on the complement did not converge at column {j + 1}")
+    met0 = met.copy()
+    for j in range(n):
+        if j in from_start:
+            continue
+        if met[j] and tr[j] > 10 * goal[j]:
+            met[j] = False
+            atfloor[j] = True
+        if atfloor[j] and tr[j] > cap[j]:
+            raise ComplementError(f"the linear solve on the complement did not converge at column {j + 1}"
+                                  + (" (met by the recurrence)" if met0[j] else ""))
+    residuals = np.array([tr[j] / b[j] if met[j] else 0.0 for j in range(n)])
+    at_floor = [j for j in range(n) if atfloor[j] and j not in from_start]
+    floor_residuals = np.array([tr[j] / scale[j] if j in at_floor else 0.0 for j in range(n)])
+    return X, it, residuals, from_start, at_floor, floor_residuals
+
+
+# ----------------------------------------------------------------------------------------
 # The warm-start network's FP64 oracle (no reference counterpart: the model
 # maps circuit parameters to an initial LOBPCG block).
 

@@ -222,3 +222,51 @@ def test_c_restatement_equals_numpy_chain():
     H2 = O.kron_operator(dims, terms, 0.2)
     x2 = rng.standard_normal(H2.N)
     assert np.allclose(c_apply.CApply(H2)(x2), O.apply_chain(H2, x2), atol=1e-12)
+
+
+# --- the linear solve on the complement (src/resolvent.jl:119-256) -----------------------
+
+def _spectral_case(n=60, k=4, cplx=False, seed=3):
+    r = np.random.default_rng(seed)
+    A = r.standard_normal((n, n)) + (1j * r.standard_normal((n, n)) if cplx else 0)
+    H = (A + A.conj().T) / 2 + np.diag(np.linspace(0, 40, n))
+    E, U = np.linalg.eigh(H)
+    return H, E, U
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_complement_solve_spectral_solution(cplx):
+    """Sternheimer columns: basis = the k lowest states, shift s_j = E_j, right-hand
+    side a random vector; the exact solution on the complement is
+    sum_{l >= k} psi_l <psi_l|b_j> / (E_l - s_j)."""
+    H, E, U = _spectral_case(cplx=cplx)
+    k, n = 4, 4
+    r = np.random.default_rng(7)
+    Bm = r.standard_normal((H.shape[0], n)) + (1j * r.standard_normal((H.shape[0], n)) if cplx else 0)
+    shifts = E[:n]
+    X, it, res, fs, af, fr = O.complement_solve(lambda Y: H @ Y, np.real(np.diag(H)), U[:, :k], Bm, shifts,
+                                                tol=1e-12, maxiter=500)
+    Uo = U[:, k:]
+    Xe = Uo @ ((Uo.conj().T @ Bm) / (E[k:, None] - shifts[None, :]))
+    assert np.max(np.abs(X - Xe)) <= 1e-9 * np.max(np.abs(Xe))
+    assert np.max(np.abs(U[:, :k].conj().T @ X)) <= 1e-12 * np.max(np.abs(X))
+    assert 0 < it < 200 and fs == [] and np.all(res <= 10 * 1e-12)
+
+
+def test_complement_solve_from_start_and_errors():
+    H, E, U = _spectral_case()
+    k = 4
+    Bm = np.zeros((H.shape[0], 3))
+    Bm[:, 1] = U[:, 10]               # on the complement
+    Bm[:, 2] = U[:, 0]                # inside the basis: nothing on the complement
+    X, it, res, fs, af, fr = O.complement_solve(lambda Y: H @ Y, np.diag(H), U[:, :k], Bm, E[:3],
+                                                tol=1e-10, maxiter=500, scale=np.array([1.0, 1.0, 1.0]))
+    assert fs == [0, 2] and np.all(X[:, [0, 2]] == 0)
+    assert np.allclose(X[:, 1], U[:, 10] / (E[10] - E[1]), atol=1e-9)
+    # a shift above a state outside the basis: H - s is not positive on the complement
+    with pytest.raises(O.ComplementError, match="not positive on the complement"):
+        O.complement_solve(lambda Y: H @ Y, np.diag(H), U[:, :2], U[:, [6]] + U[:, [3]], [E[5]], tol=1e-10,
+                           maxiter=500)
+    with pytest.raises(O.ComplementError, match="did not converge"):
+        O.complement_solve(lambda Y: H @ Y, np.diag(H), U[:, :k], np.ones((H.shape[0], 1)), [E[0]], tol=1e-14,
+                           maxiter=3)
