@@ -50,6 +50,20 @@ __global__ __launch_bounds__(256) void k_cs_dir(S* __restrict__ P, const S* __re
     }
 }
 
+// Z[:, j] = R[:, j] / hypot(D - s_j, eta)  (the Jacobi preconditioner, src/resolvent.jl:159)
+template <class S>
+__global__ __launch_bounds__(256) void k_cs_prec(S* __restrict__ Z, const S* __restrict__ R,
+                                                 const double* __restrict__ D, int64_t N, int n, ColScal s,
+                                                 double eta) {
+    const int64_t tot = N * n;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+        const int j = int(e / N);
+        const int64_t i = e - int64_t(j) * N;
+        const double f = 1.0 / hypot(D[i] - s.v[j], eta);
+        Z[e] = s_scale(typename RealOf<S>::type(f), R[e]);
+    }
+}
+
 int cs_grid(int64_t n) { return int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
 
 template <class S>
@@ -117,20 +131,30 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
     if ((rc = blas_scale_cols<S>(w, QB, N, n, N, keep)) != KFC_OK) return rc;   // x_j = 0 exactly there
     KFC_HIP(op->ctx, hipMemsetAsync(X, 0, sizeof(S) * size_t(N) * n, st));
     KFC_HIP(op->ctx, hipMemcpyAsync(R, QB, sizeof(S) * size_t(N) * n, hipMemcpyDeviceToDevice, st));
+    ColScal sh{};
+    for (int j = 0; j < n; ++j) sh.v[j] = shifts[j];
     // Z = Q (Minv .* R)
     auto precondition = [&]() -> int {
-        KFC_HIP(op->ctx, hipMemcpyAsync(Z, R, sizeof(S) * size_t(N) * n, hipMemcpyDeviceToDevice, st));
-        int r = blas_precond<S>(w, Z, N, n, N, D, shifts, eta);
-        return r == KFC_OK ? cw.project(Z) : r;
+        hipLaunchKernelGGL(k_cs_prec<S>, dim3(nb), dim3(256), 0, st, Z, R, D, N, n, sh, eta);
+        KFC_HIP(op->ctx, hipGetLastError());
+        return cw.project(Z);
+    };
+    // Y = (H - s) X, the shift of each column in the apply's epilogue
+    auto shifted_apply = [&](const S* Xs, S* Y) -> int {
+        for (int j = 0; j < n; ++j) {
+            const int r = op_apply_cheb(op, Xs + size_t(j) * N, nullptr, Y + size_t(j) * N, 1, N, KFC_PREC_F64, 1.0,
+                                        -shifts[j], 0.0, st);
+            if (r != KFC_OK) return r;
+        }
+        return KFC_OK;
     };
     if ((rc = precondition()) != KFC_OK) return rc;
     KFC_HIP(op->ctx, hipMemcpyAsync(P, Z, sizeof(S) * size_t(N) * n, hipMemcpyDeviceToDevice, st));
-    std::vector<double> rho, kappa, rn = b, tr, rhon, dummy;
+    std::vector<double> rho, kappa, rn = b, tr, rhon;
     if ((rc = cw.colredot(R, Z, rho)) != KFC_OK) return rc;
     // |Q (H - s) X - Q B| per column, with a fresh application of H
     auto true_residuals = [&](std::vector<double>& out) -> int {
-        int r = op_apply(op, X, T, n, N, KFC_PREC_F64, st);
-        if (r == KFC_OK) r = blas_residual<S>(w, T, N, T, N, X, N, n, N, shifts, dummy);
+        int r = shifted_apply(X, T);
         if (r == KFC_OK) r = cw.project(T);
         if (r == KFC_OK) r = blas_axpby<S>(w, QB, N, T, N, n, N, -1.0, 1.0);
         if (r == KFC_OK) r = blas_colnorms<S>(w, T, N, n, N, out);
@@ -147,8 +171,7 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
     while (!done() && it < maxiter) {
         ++it;
         // AP = Q (H - s) P
-        if ((rc = op_apply(op, P, AP, n, N, KFC_PREC_F64, st)) != KFC_OK) return rc;
-        if ((rc = blas_residual<S>(w, AP, N, AP, N, P, N, n, N, shifts, dummy)) != KFC_OK) return rc;
+        if ((rc = shifted_apply(P, AP)) != KFC_OK) return rc;
         if ((rc = cw.project(AP)) != KFC_OK) return rc;
         if ((rc = cw.colredot(P, AP, kappa)) != KFC_OK) return rc;
         for (int j = 0; j < n; ++j) {
