@@ -3,8 +3,9 @@
 TEST INFRASTRUCTURE ONLY.  This module restates, in numpy, the algorithm of
 KronFreeCircuits.jl for the hot path (local operators, basis port operators,
 KronFreeOperator assembly and the per-term two-buffer apply chain, derived
-quantities, circuit-term expansion and the real-node expansion, and a small
-tree materialization with Dense solves).  Only ``tests/``,
+quantities, circuit-term expansion and the real-node expansion, a small
+tree materialization with Dense solves, and the linear solve on the complement
+of a node's known states).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
 it, and only as the checker: no product code path calls it.
 
@@ -19,6 +20,7 @@ Pinning (what makes each part trustworthy; the checks are in tests/test_oracle.p
     full zero-pi, two transmons) at the reference's own tolerances
   * real expansion: real operator == complex operator on every config
   * solvers: numpy.linalg.eigh (LAPACK) on dense matrices
+  * complement solve: the spectral solution on dense matrices (eigh)
 
 Reference citations are ``src/...`` / ``test/...`` paths of KronFreeCircuits.jl.
 """
