@@ -25,19 +25,9 @@ struct ColScal {
     double v[CS_MAXC];
 };
 
-// X[:, j] += a_j P[:, j];  R[:, j] -= a_j AP[:, j]
-template <class S>
-__global__ __launch_bounds__(256) void k_cs_step(S* __restrict__ X, S* __restrict__ R, const S* __restrict__ P,
-                                                 const S* __restrict__ AP, int64_t N, int n, ColScal a) {
-    const int64_t tot = N * n;
-    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
-        const int j = int(e / N);
-        const double aj = a.v[j];
-        if (aj == 0.0) continue;
-        X[e] = s_add(X[e], s_scale(aj, P[e]));
-        R[e] = s_add(R[e], s_scale(-aj, AP[e]));
-    }
-}
+// Re(conj(a) b)
+__device__ inline double s_redot(double a, double b) { return a * b; }
+__device__ inline double s_redot(Cx<double> a, Cx<double> b) { return a.re * b.re + a.im * b.im; }
 
 // P[:, j] = Z[:, j] + b_j P[:, j]
 template <class S>
@@ -64,6 +54,71 @@ __global__ __launch_bounds__(256) void k_cs_prec(S* __restrict__ Z, const S* __r
     }
 }
 
+// X[:, j] += a_j P[:, j];  R[:, j] -= a_j AP[:, j], and the partial sums of |R[:, j]|^2
+// after the update (block (bx, j) of an nbx x n grid writes part[j nbx + bx]).  A
+// column with a_j == 0 is left as it is (its norm is unchanged).
+template <class S>
+__global__ __launch_bounds__(256) void k_cs_step_norm(S* __restrict__ X, S* __restrict__ R, const S* __restrict__ P,
+                                                      const S* __restrict__ AP, int64_t N, ColScal a,
+                                                      double* __restrict__ part) {
+    const int j = blockIdx.y;
+    const double aj = a.v[j];
+    if (aj == 0.0) return;
+    const int64_t off = int64_t(j) * N;
+    double acc = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < N; i += int64_t(gridDim.x) * 256) {
+        X[off + i] = s_add(X[off + i], s_scale(aj, P[off + i]));
+        const S r = s_add(R[off + i], s_scale(-aj, AP[off + i]));
+        R[off + i] = r;
+        acc += s_abs2(r);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[int64_t(j) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Y[:, c0 + c] -= sum_l Phi[:, l] C[l, c0 + c] for c < NC, and the partial sums of
+// Re <A[:, c0 + c], Y[:, c0 + c]> after the update (block b writes part[c nblk + b]).
+// One thread per row: the row of Phi is read once for the NC columns.
+template <class S, int NC>
+__global__ __launch_bounds__(256) void k_cs_proj_dot(S* __restrict__ Y, const S* __restrict__ A,
+                                                     const S* __restrict__ Phi, const S* __restrict__ Cm, int k,
+                                                     int ldc, int64_t N, int nc, double* __restrict__ part) {
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < N; i += int64_t(gridDim.x) * 256) {
+        S y[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] = c < nc ? Y[i + c * N] : S{};
+        for (int l = 0; l < k; ++l) {
+            const S ph = Phi[i + int64_t(l) * N];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (c < nc) y[c] = s_sub(y[c], s_mul(ph, Cm[l + c * ldc]));
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c < nc) {
+                Y[i + c * N] = y[c];
+                acc[c] += s_redot(A[i + c * N], y[c]);
+            }
+    }
+    __shared__ double red[4][NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        double v = acc[c];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
+    }
+    __syncthreads();
+    if (int(threadIdx.x) < nc)
+        part[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
 int cs_grid(int64_t n) { return int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))); }
 
 template <class S>
@@ -83,12 +138,60 @@ struct CsWork {
         for (auto& c : C) c = -c;
         return blas_gemm<S>(w, Phi, N, k, C, n, Y, N, N, 1.0);
     }
-    // Re <A_j, B_j>
-    int colredot(const S* A, const S* B, std::vector<double>& out) {
-        int rc = blas_gram<S>(w, A, N, n, B, N, n, N, G);
-        if (rc != KFC_OK) return rc;
-        out.resize(n);
-        for (int j = 0; j < n; ++j) out[j] = G[j + size_t(j) * n].real();
+    DevBuf dC, dpart;
+    std::vector<double> hpart;
+    // Y <- Y - Phi (Phi^H Y) with dots[j] = Re <A_j, Y_j> after it: the Gram, then
+    // one pass that updates Y and reduces the dots
+    int project_dot(S* Y, const S* A, std::vector<double>& dots) {
+        dots.assign(n, 0.0);
+        std::vector<S> hc;
+        if (k > 0) {
+            int rc = blas_gram<S>(w, Phi, N, k, Y, N, n, N, C);
+            if (rc != KFC_OK) return rc;
+            hc.resize(size_t(k) * n);
+            for (size_t i = 0; i < hc.size(); ++i) hc[i] = s_from<S>(C[i]);
+        } else {
+            hc.assign(1, S{});
+        }
+        KFC_HIP(op->ctx, dC.ensure(sizeof(S) * hc.size()));
+        KFC_HIP(op->ctx, hipMemcpyAsync(dC.p, hc.data(), sizeof(S) * hc.size(), hipMemcpyHostToDevice, op->ctx->stream));
+        const int nblk = int(std::max<int64_t>(1, std::min<int64_t>((N + 255) / 256, 2048)));
+        KFC_HIP(op->ctx, dpart.ensure(sizeof(double) * size_t(nblk) * 16));
+        hpart.resize(size_t(nblk) * 16);
+        for (int c0 = 0; c0 < n; c0 += 16) {
+            const int nc = std::min(16, n - c0);
+            hipLaunchKernelGGL((k_cs_proj_dot<S, 16>), dim3(nblk), dim3(256), 0, op->ctx->stream, Y + size_t(c0) * N,
+                               A + size_t(c0) * N, Phi, dC.as<S>() + size_t(c0) * std::max(k, 1), k, std::max(k, 1), N,
+                               nc, dpart.as<double>());
+            KFC_HIP(op->ctx, hipGetLastError());
+            KFC_HIP(op->ctx, hipMemcpyAsync(hpart.data(), dpart.p, sizeof(double) * size_t(nblk) * nc,
+                                            hipMemcpyDeviceToHost, op->ctx->stream));
+            KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
+            for (int c = 0; c < nc; ++c) {
+                double sum = 0;
+                for (int b = 0; b < nblk; ++b) sum += hpart[size_t(c) * nblk + b];
+                dots[c0 + c] = sum;
+            }
+        }
+        return KFC_OK;
+    }
+    // X += a P, R -= a AP with the norms of R after it (columns with a_j == 0 keep rn_j)
+    int step_norm(S* X, S* R, const S* P, const S* AP, const ColScal& a, std::vector<double>& rn) {
+        const int nbx = int(std::max<int64_t>(1, std::min<int64_t>((N + 255) / 256, (2048 + n - 1) / n)));
+        KFC_HIP(op->ctx, dpart.ensure(sizeof(double) * size_t(nbx) * std::max(n, 16)));
+        hipLaunchKernelGGL(k_cs_step_norm<S>, dim3(nbx, n), dim3(256), 0, op->ctx->stream, X, R, P, AP, N, a,
+                           dpart.as<double>());
+        KFC_HIP(op->ctx, hipGetLastError());
+        hpart.resize(size_t(nbx) * n);
+        KFC_HIP(op->ctx, hipMemcpyAsync(hpart.data(), dpart.p, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost,
+                                        op->ctx->stream));
+        KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
+        for (int j = 0; j < n; ++j) {
+            if (a.v[j] == 0.0) continue;
+            double sum = 0;
+            for (int b = 0; b < nbx; ++b) sum += hpart[size_t(j) * nbx + b];
+            rn[j] = std::sqrt(sum);
+        }
         return KFC_OK;
     }
 };
@@ -134,10 +237,11 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
     ColScal sh{};
     for (int j = 0; j < n; ++j) sh.v[j] = shifts[j];
     // Z = Q (Minv .* R)
-    auto precondition = [&]() -> int {
+    // Z = Q (Minv .* R) with rho_j = Re <R_j, Z_j>
+    auto precondition_dot = [&](std::vector<double>& dots) -> int {
         hipLaunchKernelGGL(k_cs_prec<S>, dim3(nb), dim3(256), 0, st, Z, R, D, N, n, sh, eta);
         KFC_HIP(op->ctx, hipGetLastError());
-        return cw.project(Z);
+        return cw.project_dot(Z, R, dots);
     };
     // Y = (H - s) X, the shift of each column in the apply's epilogue
     auto shifted_apply = [&](const S* Xs, S* Y) -> int {
@@ -148,10 +252,9 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
         }
         return KFC_OK;
     };
-    if ((rc = precondition()) != KFC_OK) return rc;
-    KFC_HIP(op->ctx, hipMemcpyAsync(P, Z, sizeof(S) * size_t(N) * n, hipMemcpyDeviceToDevice, st));
     std::vector<double> rho, kappa, rn = b, tr, rhon;
-    if ((rc = cw.colredot(R, Z, rho)) != KFC_OK) return rc;
+    if ((rc = precondition_dot(rho)) != KFC_OK) return rc;
+    KFC_HIP(op->ctx, hipMemcpyAsync(P, Z, sizeof(S) * size_t(N) * n, hipMemcpyDeviceToDevice, st));
     // |Q (H - s) X - Q B| per column, with a fresh application of H
     auto true_residuals = [&](std::vector<double>& out) -> int {
         int r = shifted_apply(X, T);
@@ -172,8 +275,7 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
         ++it;
         // AP = Q (H - s) P
         if ((rc = shifted_apply(P, AP)) != KFC_OK) return rc;
-        if ((rc = cw.project(AP)) != KFC_OK) return rc;
-        if ((rc = cw.colredot(P, AP, kappa)) != KFC_OK) return rc;
+        if ((rc = cw.project_dot(AP, P, kappa)) != KFC_OK) return rc;
         for (int j = 0; j < n; ++j) {
             if (met[j] || atfloor[j] || kappa[j] > 0) continue;
             if (rn[j] <= cap[j]) {
@@ -193,9 +295,7 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
         for (int j = 0; j < n; ++j) a.v[j] = (met[j] || atfloor[j]) ? 0.0 : rho[j] / kappa[j];
         std::vector<char> active(n);
         for (int j = 0; j < n; ++j) active[j] = !(met[j] || atfloor[j]);
-        hipLaunchKernelGGL(k_cs_step<S>, dim3(nb), dim3(256), 0, st, X, R, P, AP, N, n, a);
-        KFC_HIP(op->ctx, hipGetLastError());
-        if ((rc = blas_colnorms<S>(w, R, N, n, N, rn)) != KFC_OK) return rc;
+        if ((rc = cw.step_norm(X, R, P, AP, a, rn)) != KFC_OK) return rc;
         bool any_near = false;
         std::vector<char> near(n, 0);
         for (int j = 0; j < n; ++j) {
@@ -217,8 +317,7 @@ int complement_t(kfc_op* op, const double* D, const S* Phi, int k, const S* B, i
             }
         }
         if (done()) break;
-        if ((rc = precondition()) != KFC_OK) return rc;
-        if ((rc = cw.colredot(R, Z, rhon)) != KFC_OK) return rc;
+        if ((rc = precondition_dot(rhon)) != KFC_OK) return rc;
         ColScal bb{};
         for (int j = 0; j < n; ++j) bb.v[j] = (met[j] || atfloor[j]) ? 0.0 : rhon[j] / rho[j];
         hipLaunchKernelGGL(k_cs_dir<S>, dim3(nb), dim3(256), 0, st, P, Z, N, n, bb);
