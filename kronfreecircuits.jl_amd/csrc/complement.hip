@@ -160,9 +160,16 @@ struct CsWork {
         hpart.resize(size_t(nblk) * 16);
         for (int c0 = 0; c0 < n; c0 += 16) {
             const int nc = std::min(16, n - c0);
-            hipLaunchKernelGGL((k_cs_proj_dot<S, 16>), dim3(nblk), dim3(256), 0, op->ctx->stream, Y + size_t(c0) * N,
-                               A + size_t(c0) * N, Phi, dC.as<S>() + size_t(c0) * std::max(k, 1), k, std::max(k, 1), N,
-                               nc, dpart.as<double>());
+#define KFC_PD(NCv)                                                                                       \
+    hipLaunchKernelGGL((k_cs_proj_dot<S, NCv>), dim3(nblk), dim3(256), 0, op->ctx->stream, Y + size_t(c0) * N, \
+                       A + size_t(c0) * N, Phi, dC.as<S>() + size_t(c0) * std::max(k, 1), k, std::max(k, 1), N, nc, \
+                       dpart.as<double>())
+            // the column count of the launch rounded up to 4: no idle accumulators
+            if (nc <= 4) KFC_PD(4);
+            else if (nc <= 8) KFC_PD(8);
+            else if (nc <= 12) KFC_PD(12);
+            else KFC_PD(16);
+#undef KFC_PD
             KFC_HIP(op->ctx, hipGetLastError());
             KFC_HIP(op->ctx, hipMemcpyAsync(hpart.data(), dpart.p, sizeof(double) * size_t(nblk) * nc,
                                             hipMemcpyDeviceToHost, op->ctx->stream));
