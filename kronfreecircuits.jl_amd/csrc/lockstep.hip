@@ -668,6 +668,19 @@ struct Phase {
     hipEvent_t e0, e1;
     double t_wait = 0, t_rr = 0;   // host time blocked on the stream / in the Rayleigh-Ritz steps
     int64_t n_iter = 0, n_pointiter = 0, n_collapsed = 0;
+    int nchunk0 = 0;               // row chunks per point with every point active (the partials' capacity)
+
+    // Row chunks per point for a launch over `nact` points: with fewer active points
+    // each point's rows are split finer (up to 16x), so that the tail of a sweep, where
+    // a few slow points remain, still fills the device.  nact * nchunk stays within the
+    // partial-sum buffer (sized for every point at nchunk0 chunks).
+    void shape_for(int nact) {
+        if (nchunk0 == 0) nchunk0 = nchunk;
+        const int f = std::max(1, std::min(16, int(pts.size()) / std::max(1, nact)));
+        const int64_t want = int64_t(nchunk0) * f;
+        rows_per = std::max<int64_t>(LS_SR, ((N + want - 1) / want + LS_SR - 1) / LS_SR * LS_SR);
+        nchunk = int((N + rows_per - 1) / rows_per);
+    }
 
     LsArgs<T> args(int nact) {
         LsArgs<T> a;
@@ -799,7 +812,9 @@ struct Phase {
         int rc = upload_slots(sl, 1);
         if (rc != KFC_OK) return rc;
         const LsArgs<T> a = args(nr);
-        const unsigned gx = unsigned(std::max<int64_t>(1, (N + 2047) / 2048));
+        // rows of 2048 per block with every point active, finer (>= 64 rows) for fewer points
+        const int f = std::max(1, std::min(16, int(pts.size()) / std::max(1, nr)));
+        const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>((N + 63) / 64, ((N + 2047) / 2048) * f)));
         LsArgs<T> ar = a;
         ar.slot = dslot.as<LsSlot>() + pts.size();
         ar.rows_per = ((N + gx - 1) / gx + 63) / 64 * 64;
@@ -824,6 +839,7 @@ struct Phase {
         }
         if (act.empty()) return KFC_OK;
         // initial Rayleigh-Ritz of the start blocks: AX, G = X^T X, K = X^T A X
+        shape_for(int(act.size()));
         int rc = apply(act, 0, W, 0);
         if (rc != KFC_OK) return rc;
         {
@@ -847,6 +863,7 @@ struct Phase {
                 if (!pts[p].done && !pts[p].failed) act.push_back(int(p));
             if (act.empty()) break;
             const int nact = int(act.size());
+            shape_for(nact);
             std::vector<LsSlot> sl;
             std::vector<int> cs;
             for (int p : act) {
@@ -1123,7 +1140,8 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
             rc = ph.upload_theta();
             if (rc == KFC_OK) rc = ph.upload_slots(sl);
             if (rc != KFC_OK) return rc;
-            hipLaunchKernelGGL(k_ls_resid<double>, dim3(nchunk, int(fin.size())), dim3(256), 0, st,
+            ph.shape_for(int(fin.size()));
+            hipLaunchKernelGGL(k_ls_resid<double>, dim3(ph.nchunk, int(fin.size())), dim3(256), 0, st,
                                ph.args(int(fin.size())), int(m), dtheta.as<double>(), dtheta.as<double>(),
                                (const double*)nullptr, 0, 0.0, 1);
             KFC_HIP(ctx, hipGetLastError());

@@ -127,3 +127,26 @@ def test_sternheimer_derivative_vs_finite_differences():
     fd = (Vp - Vm) / (2 * h)
     # derivative of a normalized real eigenvector: orthogonal to it, the sum over l != n
     assert np.max(np.abs(W - fd)) <= 2e-5 * np.max(np.abs(fd))
+
+
+def test_round_off_floor_columns():
+    """A right-hand side almost inside the basis: its complement part (1e-11 of it)
+    cannot reach tol on its own scale, so the column stops at the round-off floor
+    (src/resolvent.jl:128-140), as the oracle's does; a column met by its tolerance
+    beside it is unaffected."""
+    import kfc
+    th, H = _fluxonia()
+    M = _dense_of(th, 8)
+    E, U = np.linalg.eigh(M)
+    k = 4
+    B = np.stack([U[:, 0] + 1e-11 * U[:, 7], U[:, 9]], axis=1)
+    args = dict(tol=1e-10, maxiter=800)
+    Xo, ito, reso, fso, afo, fro = O.complement_solve(lambda Y: M @ Y, np.diag(M), U[:, :k], B, E[:2], **args)
+    X, it, res, fs, af, fr = kfc.complement_solve(H, H.diagonal_device(), U[:, :k].T, B.T, E[:2], args["tol"],
+                                                  args["maxiter"])
+    assert afo == [0] and af == [0] and fs == fso == []
+    assert 0 < fr[0] <= 1000 * np.finfo(float).eps and res[0] == 0
+    assert res[1] <= 1e-9 and fr[1] == 0
+    X = X.cpu().numpy().T
+    assert np.allclose(X[:, 1], U[:, 9] / (E[9] - E[1]), atol=1e-9)
+    assert np.max(np.abs(X[:, 0] - Xo[:, 0])) <= 1e-12
