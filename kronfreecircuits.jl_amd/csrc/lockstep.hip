@@ -304,12 +304,28 @@ __global__ __launch_bounds__(256) void k_ls_rot(const LsArgs<T> a, const double*
     for (int e = tid; e < LS_CMAX * LS_MO; e += 256) sM[e] = M[e];
     const int64_t r0 = blockIdx.x * a.rows_per, r1 = min(N, r0 + a.rows_per);
     const int lr = tid & 63, cg = tid >> 6;
-    for (int64_t rb = r0; rb < r1; rb += 64) {
-        __syncthreads();   // the previous slab (and, first, sM) is consumed / written
+    // the next 64-row slab is fetched into registers while the current one is
+    // contracted (column cg + 4u of row lr in pf[u]; c4 <= 64 columns)
+    double pf[LS_CMAX / 4];
+    auto fetch = [&](int64_t rb) {
         const int64_t row = rb + lr;
         const bool ok = row < r1;
-        for (int col = cg; col < c4; col += 4) sl[col * LD + lr] = (ok && col < c) ? double(Src[row + col * N]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < LS_CMAX / 4; ++u) {
+            const int col = cg + 4 * u;
+            pf[u] = (ok && col < c) ? double(Src[row + col * N]) : 0.0;
+        }
+    };
+    if (r0 < r1) fetch(r0);
+    for (int64_t rb = r0; rb < r1; rb += 64) {
+        __syncthreads();   // the previous slab (and, first, sM) is consumed / written
+#pragma unroll
+        for (int u = 0; u < LS_CMAX / 4; ++u) {
+            const int col = cg + 4 * u;
+            if (col < c4) sl[col * LD + lr] = pf[u];
+        }
         __syncthreads();
+        if (rb + 64 < r1) fetch(rb + 64);
 #pragma unroll
         for (int O = 0; O < 3; ++O) {
             if (O >= nto) break;
