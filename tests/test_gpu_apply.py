@@ -196,3 +196,38 @@ def test_zero_pi_parent_operator_parity():
     rng = np.random.default_rng(0)
     X = rng.standard_normal((H.N, 20))
     assert _rel(_run(H, X), O.apply_chain(Ho, X)) < TOL64
+
+
+def test_three_fluxonia_full_size_parity():
+    """The bench node at its full size, 512^3 (134 M states): the fused stencil apply
+    in FP64 against the C restatement of the reference's per-term chain
+    (oracle/apply_chain.c) on the same random vector, the FP32 apply within its
+    tolerance, and the symmetry <u, H v> = <H u, v> of the real Hermitian operator."""
+    from kfc import configs
+    from oracle import c_apply
+    if not c_apply.available():
+        pytest.skip("the oracle's C restatement is not built (run __graft_entry__.build())")
+    th = configs.three_fluxonia_theta(20261203)
+    N = 512
+    H = configs.three_fluxonia_operator(th, N=N)
+    assert H.fused and H.N == N ** 3
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal(H.N)
+    xd = torch.from_numpy(x).to("cuda:0")
+    y = (H @ xd)
+    H.sync()
+    y = y.cpu().numpy()
+    Ho = O.three_fluxonia_operator(th["E_C"], th["E_L"], th["E_J"], th["phi_ext"], N=N, order=2)
+    yref = c_apply.CApply(Ho)(x)
+    del Ho
+    assert _rel(y, yref) < TOL64
+    y32 = H @ xd.float()
+    H.sync()
+    assert _rel(y32.cpu().numpy().astype(np.float64), yref) < 50 * TOL32
+    del y32, y, yref
+    u = torch.from_numpy(rng.standard_normal(H.N)).to("cuda:0")
+    Hu = H @ u
+    Hx = H @ xd
+    H.sync()
+    a, b = torch.dot(u, Hx).item(), torch.dot(Hu, xd).item()
+    assert abs(a - b) <= 1e-10 * (torch.linalg.norm(u) * torch.linalg.norm(Hx)).item()
