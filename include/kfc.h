@@ -143,6 +143,19 @@ int kfc_op_work(const kfc_op* op, double* bytes, double* flops);
  * (ArgumentError, src/apply.jl:412).  Enqueued on the context stream; returns
  * after the launch (kfc_sync waits). */
 int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision);
+
+/* One Chebyshev step of a real operator with its two inner products:
+ * C = a H B + b B + g A (A may be null) on one column, and host dots[0] = B.B,
+ * dots[1] = B.C (FP64 sums).  The recurrence is _chebyshev_step!
+ * (src/chebyshev.jl:25-30); the dots are the moments q^T T_j(H) q that the
+ * s-step Lanczos (kfc_lanczos) takes from its block applies, via
+ * T_i T_i = (T_2i + T_0) / 2 and T_i T_(i+1) = (T_2i+1 + T_1) / 2, in place of the
+ * reference's separate inner products (KrylovKit's Lanczos reorthogonalization).
+ * *fused (may be null) is 1 when the fused plane sweep computed the dots in the
+ * apply's own pass, 0 when they came from a separate pass.  C must not alias B.
+ * Synchronizes the context stream. */
+int kfc_apply_cheb_dots(kfc_op* op, const void* B, const void* A, void* C, double a, double b, double g,
+                        int32_t precision, double* dots, int32_t* fused);
 int kfc_sync(kfc_ctx* ctx);
 
 /* Replaces diagonal(H) / _full_diagonal(H) (src/apply.jl:455-478): the real

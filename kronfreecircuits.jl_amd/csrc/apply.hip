@@ -364,6 +364,7 @@ struct StArgs {
     const S* dp1[4];
     const S* dp2[4];
     S alpha, beta, gamma;
+    double* dots;   // per logical block: sum x^2 and sum x y over its outputs (DOT sweeps)
 };
 
 constexpr int ST_TX = 64, ST_TY = 4, ST_RW = ST_TX + 2, ST_RH = ST_TY + 2, ST_TILE = ST_RW * ST_RH;
@@ -676,8 +677,11 @@ constexpr int SA_D = 2;
 // straight from HBM into LDS with global_load_lds (no registers held for the
 // prefetch), SA_D planes per batch into one of two buffers; a batch is computed
 // without barriers while the next one streams in, with one barrier per batch.
-template <class S, bool DP, bool ASYNC>
+// DOT (real S): the block also writes sum x_i^2 and sum x_i y_i over its output
+// points to a.dots[2 lg], a.dots[2 lg + 1] (FP64), x the input at the output point.
+template <class S, bool DP, bool ASYNC, bool DOT>
 __device__ inline void stencil_sweep(const StArgs<S>& a) {
+    static_assert(!DOT || !IsCx<S>::value, "the fused dots are for real blocks");
     constexpr int DW = sizeof(S) / 4;                 // dwords per point
     constexpr int TDW = SP_TILE * DW;                 // dwords per plane tile
     constexpr int NCH = (TDW + 255) / 256;            // dword chunks per thread
@@ -784,6 +788,7 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
     S accM[SP_R], acc0[SP_R], accP[SP_R], ctrM[SP_R], ctr0[SP_R];
 #pragma unroll
     for (int r = 0; r < SP_R; ++r) accM[r] = acc0[r] = accP[r] = ctrM[r] = ctr0[r] = zero;
+    double sxx = 0, sxy = 0;
     auto plane = [&](const S* __restrict__ t, int zp) {
         // rows r0 .. r0 + SP_R + 1 of the patch, columns lane .. lane + 2
         S v[SP_R + 2][3];
@@ -845,6 +850,11 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
                     S out = s_fma(s_mul(a.alpha, accM[r]), a.beta, ctrM[r]);
                     if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
                     store_stream(Yc + idx, out);
+                    if constexpr (DOT) {
+                        const double xv = double(ctrM[r]);
+                        sxx = fma(xv, xv, sxx);
+                        sxy = fma(xv, double(out), sxy);
+                    }
                 }
             }
         }
@@ -891,16 +901,52 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
             __syncthreads();
         }
     }
+    if constexpr (DOT) {
+        __shared__ double red[2][4];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sxx += __shfl_xor(sxx, o);
+            sxy += __shfl_xor(sxy, o);
+        }
+        if (lane == 0) {
+            red[0][wv] = sxx;
+            red[1][wv] = sxy;
+        }
+        __syncthreads();
+        if (tid < 2) a.dots[2 * int64_t(lg) + tid] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+    }
 }
 
-template <class S, bool DP>
+template <class S, bool DP, bool DOT>
 __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
-    stencil_sweep<S, DP, false>(a);
+    stencil_sweep<S, DP, false, DOT>(a);
 }
 
-template <class S, bool DP>
+template <class S, bool DP, bool DOT>
 __global__ __launch_bounds__(256) void k_stencil_a(const StArgs<S> a) {
-    stencil_sweep<S, DP, true>(a);
+    stencil_sweep<S, DP, true, DOT>(a);
+}
+
+// out[0], out[1] = the sums of the n pairs of block partials, in a fixed order.
+__global__ __launch_bounds__(256) void k_dot_pairs(const double* __restrict__ part, int n, double* __restrict__ out) {
+    __shared__ double red[2][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double s0 = 0, s1 = 0;
+    for (int i = tid; i < n; i += 256) {
+        s0 += part[2 * int64_t(i)];
+        s1 += part[2 * int64_t(i) + 1];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        red[0][wv] = s0;
+        red[1][wv] = s1;
+    }
+    __syncthreads();
+    if (tid < 2) out[tid] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
 }
 
 // The device tables an apply on scalar S reads: the operator's own, or, for complex
@@ -914,8 +960,11 @@ static DeviceData& dd_for(kfc_op* op) {
 }
 
 template <class S>
+// dots (device, 2 doubles, one column, real S): also sum x^2 and sum x y into
+// dots[0], dots[1] when the plane sweep runs (*fused = true; false otherwise).
 static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld,
-                          cd alpha, cd beta, cd gamma, hipStream_t st) {
+                          cd alpha, cd beta, cd gamma, hipStream_t st, double* dots = nullptr,
+                          bool* fused = nullptr) {
     const DeviceData& dd = dd_for<S>(op);
     const StencilPlan& P = op->plan;
     StArgs<S> a;
@@ -969,12 +1018,31 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     if (nlog >= (1ll << 31) - 8) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "apply: grid too large");
     a.nlog = int(nlog);
     const int64_t nblk = (nlog + 7) / 8 * 8;
+    if (fused) *fused = false;
+    if constexpr (!IsCx<S>::value) {
+        if (dots && ncols == 1 && (variant == 2 || variant == 3)) {
+            KFC_HIP(op->ctx, op->dotpart.ensure(sizeof(double) * 2 * size_t(nlog)));
+            a.dots = op->dotpart.as<double>();
+            if (variant == 2) {
+                if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+                else hipLaunchKernelGGL((k_stencil_p<S, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+            } else {
+                if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_a<S, true, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+                else hipLaunchKernelGGL((k_stencil_a<S, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+            }
+            KFC_HIP(op->ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_dot_pairs, dim3(1), dim3(256), 0, st, a.dots, a.nlog, dots);
+            KFC_HIP(op->ctx, hipGetLastError());
+            *fused = true;
+            return KFC_OK;
+        }
+    }
     if (variant == 2) {
-        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_stencil_p<S, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_p<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (variant == 3) {
-        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_a<S, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_stencil_a<S, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_a<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_a<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (variant == 0) {
         hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (cw) {
@@ -1155,7 +1223,8 @@ static int tune_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64
 
 template <class S>
 static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
-                   cd gamma, hipStream_t st) {
+                   cd gamma, hipStream_t st, double* dots = nullptr, bool* fused = nullptr) {
+    if (fused) *fused = false;
     if (op->plan.ok) {
         const int prec = std::is_same<typename RealOf<S>::type, double>::value ? 0 : 1;
         const int slot = (IsCx<S>::value && !op->cplx) ? 2 : prec;
@@ -1164,7 +1233,7 @@ static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t nc
             if (rc != KFC_OK) return rc;
             op->tuned_cols[slot] = ncols;
         }
-        return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+        return launch_stencil<S>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st, dots, fused);
     }
     int rc;
     if constexpr (!IsCx<S>::value) {
@@ -1185,15 +1254,16 @@ static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t nc
 }
 
 static int dispatch(kfc_op* op, const void* X, const void* A, void* Y, int64_t ncols, int64_t ld, int prec, cd alpha,
-                    cd beta, cd gamma, hipStream_t st) {
+                    cd beta, cd gamma, hipStream_t st, double* dots = nullptr, bool* fused = nullptr) {
+    if (fused) *fused = false;
     if (ncols == 0 || op->N == 0) return KFC_OK;
     int rc = op_ensure_device(op, prec);
     if (rc != KFC_OK) return rc;
     if (prec == KFC_PREC_F64)
         return op->cplx ? apply_t<Cx<double>>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st)
-                        : apply_t<double>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+                        : apply_t<double>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st, dots, fused);
     return op->cplx ? apply_t<Cx<float>>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st)
-                    : apply_t<float>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
+                    : apply_t<float>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st, dots, fused);
 }
 
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st) {
@@ -1228,4 +1298,9 @@ int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t
 int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec, double a,
                   double b, double g, hipStream_t st) {
     return dispatch(op, B, A, C, ncols, ld, prec, cd(a), cd(b), cd(g), st);
+}
+
+int op_apply_cheb_dots(kfc_op* op, const void* B, const void* A, void* C, int64_t ld, int prec, double a, double b,
+                       double g, double* dots, bool& fused, hipStream_t st) {
+    return dispatch(op, B, A, C, 1, ld, prec, cd(a), cd(b), cd(g), st, dots, &fused);
 }

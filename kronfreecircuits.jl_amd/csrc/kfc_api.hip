@@ -4,6 +4,7 @@
 #include <cmath>
 #include <queue>
 
+#include "blas.h"
 #include "kfc_internal.h"
 
 static thread_local std::string g_noctx_err;
@@ -226,6 +227,49 @@ int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int
     const size_t span = size_t(((ncols - 1) * ld + op->N)) * es;
     if (xb < yb + span && yb < xb + span) return kfc_fail(op->ctx, KFC_EARG, "mul!: y must not alias x");
     return op_apply(op, X, Y, ncols, ld, precision, op->ctx->stream);
+}
+
+int kfc_apply_cheb_dots(kfc_op* op, const void* B, const void* A, void* C, double a, double b, double g,
+                        int32_t precision, double* dots, int32_t* fused) {
+    if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_apply_cheb_dots: null operator");
+    if (op->cplx) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "kfc_apply_cheb_dots: real operators only");
+    if (precision != KFC_PREC_F64 && precision != KFC_PREC_F32)
+        return kfc_fail(op->ctx, KFC_EARG, "kfc_apply_cheb_dots: unknown precision");
+    if (!B || !C || !dots) return kfc_fail(op->ctx, KFC_EARG, "kfc_apply_cheb_dots: null buffer");
+    const size_t span = size_t(op->N) * (precision == KFC_PREC_F64 ? 8 : 4);
+    const char* bb = static_cast<const char*>(B);
+    const char* cb = static_cast<const char*>(C);
+    if (bb < cb + span && cb < bb + span) return kfc_fail(op->ctx, KFC_EARG, "mul!: y must not alias x");
+    Work w;
+    w.ctx = op->ctx;
+    KFC_HIP(op->ctx, w.small2.ensure(2 * sizeof(double)));
+    bool f = false;
+    int rc = op_apply_cheb_dots(op, B, A, C, op->N, precision, a, b, g, w.small2.as<double>(), f, op->ctx->stream);
+    if (rc != KFC_OK) return rc;
+    if (f) {
+        KFC_HIP(op->ctx, hipMemcpyAsync(dots, w.small2.p, 2 * sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+        KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
+    } else {
+        std::vector<cd> g0, g1;
+        if (precision == KFC_PREC_F64) {
+            rc = blas_gram<double>(w, static_cast<const double*>(B), op->N, 1, static_cast<const double*>(B), op->N, 1,
+                                   op->N, g0);
+            if (rc == KFC_OK)
+                rc = blas_gram<double>(w, static_cast<const double*>(B), op->N, 1, static_cast<const double*>(C),
+                                       op->N, 1, op->N, g1);
+        } else {
+            rc = blas_gram<float>(w, static_cast<const float*>(B), op->N, 1, static_cast<const float*>(B), op->N, 1,
+                                  op->N, g0);
+            if (rc == KFC_OK)
+                rc = blas_gram<float>(w, static_cast<const float*>(B), op->N, 1, static_cast<const float*>(C), op->N,
+                                      1, op->N, g1);
+        }
+        if (rc != KFC_OK) return rc;
+        dots[0] = g0[0].real();
+        dots[1] = g1[0].real();
+    }
+    if (fused) *fused = f ? 1 : 0;
+    return KFC_OK;
 }
 
 }  // extern "C"

@@ -204,6 +204,7 @@ struct kfc_op {
                                             // and the complex (promoted) apply of a real operator
     int variant_z = -1, march_z = 0;        // the promoted apply's own tuned launch shape
     PairPlan pairs;
+    DevBuf dotpart;                 // block partials of the fused apply dots (apply.hip)
     DevBuf evo[4];                  // scratch of the time-evolution applies (evolve.hip)
     DeviceData dd[2];               // [KFC_PREC_F64], [KFC_PREC_F32]
     DeviceData ddz;                 // complex FP64 tables of a real stencil operator (complex kets)
@@ -276,6 +277,11 @@ int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t
 // Chebyshev step fused into the apply: C = a*(H B) + b*B + g*A  (A may be null)
 int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec,
                   double a, double b, double g, hipStream_t st);
+// One column: op_apply_cheb that also leaves sum B^2 and sum B C (FP64) in dots[0], dots[1]
+// (device) when the fused plane sweep runs the apply (fused = true); otherwise dots are
+// not written (fused = false).  Real blocks only.
+int op_apply_cheb_dots(kfc_op* op, const void* B, const void* A, void* C, int64_t ld, int prec, double a, double b,
+                       double g, double* dots, bool& fused, hipStream_t st);
 
 // Host linear algebra (linalg.cpp): Hermitian eigendecomposition, ascending.
 // A is n x n column-major (overwritten); w[n]; Z[n x n] eigenvectors.

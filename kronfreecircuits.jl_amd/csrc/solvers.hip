@@ -313,13 +313,19 @@ std::vector<cd> mm(int r, int t, int c, const std::vector<cd>& A, const std::vec
     return C;
 }
 
+// dots (device, real S): also sum B^2 and sum B C into dots[0], dots[1] when the fused
+// apply can (*fused says whether it did).
 template <class S>
-int apply_cheb_timed(kfc_op* op, const S* B, const S* A, S* C, double a, double b, double g, Timer& t) {
+int apply_cheb_timed(kfc_op* op, const S* B, const S* A, S* C, double a, double b, double g, Timer& t,
+                     double* dots = nullptr, bool* fused = nullptr) {
     auto t0 = clk::now();
     std::pair<hipEvent_t, hipEvent_t> pr;
     KFC_HIP(op->ctx, t.pair(pr));
     KFC_HIP(op->ctx, hipEventRecord(pr.first, op->ctx->stream));
-    int rc = op_apply_cheb(op, B, A, C, 1, op->N, prec_of<S>(), a, b, g, op->ctx->stream);
+    bool f = false;
+    int rc = dots ? op_apply_cheb_dots(op, B, A, C, op->N, prec_of<S>(), a, b, g, dots, f, op->ctx->stream)
+                  : op_apply_cheb(op, B, A, C, 1, op->N, prec_of<S>(), a, b, g, op->ctx->stream);
+    if (fused) *fused = f;
     if (rc != KFC_OK) return rc;
     KFC_HIP(op->ctx, hipEventRecord(pr.second, op->ctx->stream));
     t.pending.push_back(pr);
@@ -400,11 +406,23 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
         if (!ritz_known) s = std::min(s, 4);   // the first block runs on the loose interval
         S* K = V + size_t(k) * N;
         const double cc = 0.5 * (hi + lo), ee = 0.5 * (hi - lo);
+        // The first projection below needs the moments mu_j = q^H T_j q (j <= s) of the
+        // block's start q = K_0.  For real blocks the fused apply returns <K_i, K_i> and
+        // <K_i, K_{i+1}> with K_{i+1}, and T_i T_i = (T_2i + T_0) / 2, T_i T_{i+1} =
+        // (T_2i+1 + T_1) / 2 give mu_2i = 2 <K_i, K_i> - mu_0, mu_2i+1 = 2 <K_i, K_{i+1}> - mu_1
+        // without a pass over the block.
+        const bool want_moments = !IsCx<S>::value && k > 0 && relation_exact;
+        const int nmom = s / 2 + 1;   // applies 0 .. s/2 carry the dots
+        bool moments_fused = want_moments;
+        if (want_moments) KFC_HIP(op->ctx, w.small2.ensure(sizeof(double) * 2 * size_t(nmom)));
         for (int i = 0; i < s; ++i) {
-            rc = i == 0 ? apply_cheb_timed<S>(op, K, nullptr, K + N, 1.0 / ee, -cc / ee, 0.0, tm)
+            double* dots = want_moments && i < nmom ? w.small2.as<double>() + 2 * i : nullptr;
+            bool fz = false;
+            rc = i == 0 ? apply_cheb_timed<S>(op, K, nullptr, K + N, 1.0 / ee, -cc / ee, 0.0, tm, dots, &fz)
                         : apply_cheb_timed<S>(op, K + size_t(i) * N, K + size_t(i - 1) * N, K + size_t(i + 1) * N,
-                                              2.0 / ee, -2.0 * cc / ee, -1.0, tm);
+                                              2.0 / ee, -2.0 * cc / ee, -1.0, tm, dots, &fz);
             if (rc != KFC_OK) return rc;
+            if (dots && !fz) moments_fused = false;
         }
         const int nb0 = s + 1;
         std::vector<cd> pend_ri;   // a pending K <- K pend_ri (pend_n columns), see the Cholesky QR
@@ -423,8 +441,22 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                 // block's Chebyshev recurrence given q^H K (one short Gram of the block).
                 // The explicit second pass then removes what rounding left.
                 std::vector<cd> g0;
-                rc = blas_gram<S>(w, K, N, 1, K, N, nb0, N, g0);
-                if (rc != KFC_OK) return rc;
+                if (moments_fused) {
+                    std::vector<double> d(2 * size_t(nmom));
+                    KFC_HIP(op->ctx, hipMemcpyAsync(d.data(), w.small2.p, d.size() * sizeof(double),
+                                                    hipMemcpyDeviceToHost, op->ctx->stream));
+                    KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));
+                    g0.assign(nb0, cd(0));
+                    g0[0] = d[0];
+                    if (nb0 > 1) g0[1] = d[1];
+                    for (int j = 2; j < nb0; ++j) {
+                        const int i = j / 2;
+                        g0[j] = (j & 1) ? 2.0 * d[2 * i + 1] - d[1] : 2.0 * d[2 * i] - d[0];
+                    }
+                } else {
+                    rc = blas_gram<S>(w, K, N, 1, K, N, nb0, N, g0);
+                    if (rc != KFC_OK) return rc;
+                }
                 C1.assign(size_t(k) * nb0, cd(0));
                 auto hcol = [&](int i, std::vector<cd>& out) {   // Q^H (H - c) K_i
                     out.assign(k, cd(0));

@@ -173,6 +173,23 @@ class KronFreeOperator:
                 self.ctx.handle)
         return Y
 
+    def cheb_step_dots(self, Y, B, A=None, a=1.0, b=0.0, g=0.0, precision="f64"):
+        """``Y = a H B + b B + g A`` on one device vector (the step of
+        ``_chebyshev_step!``, src/chebyshev.jl:25-30) with ``(B.B, B.Y)`` as FP64
+        sums.  Returns (dots, fused): fused is True when the fused plane sweep
+        produced the sums in the apply's own pass."""
+        for T in (B, Y) + ((A,) if A is not None else ()):
+            if T.dim() != 1 or T.shape[0] != self.N or T.stride(0) != 1:
+                raise L.DimensionMismatch(f"cheb_step_dots: contiguous vectors of {self.N} entries")
+        prec = L.KFC_PREC_F64 if precision == "f64" else L.KFC_PREC_F32
+        d = np.zeros(2)
+        fz = C.c_int32(0)
+        L.check(L.lib().kfc_apply_cheb_dots(self.handle, C.c_void_p(B.data_ptr()),
+                                            C.c_void_p(A.data_ptr() if A is not None else 0),
+                                            C.c_void_p(Y.data_ptr()), float(a), float(b), float(g), prec,
+                                            d.ctypes.data_as(L._D), C.byref(fz)), self.ctx.handle)
+        return d, bool(fz.value)
+
     def __matmul__(self, X):
         import torch
         Y = torch.empty_like(X)

@@ -95,6 +95,37 @@ def test_stencil_variants(dims, cplx, variant, march):
     assert H.stencil == (variant, march)
 
 
+@pytest.mark.parametrize("variant,march", STENCIL_SHAPES)
+@pytest.mark.parametrize("dims", [[70, 19, 11], [65, 20, 150], [130, 7], [200]])
+def test_cheb_step_dots(dims, variant, march):
+    """The Chebyshev step C = a H B + b B + g A with its inner products (B.B, B.C):
+    C against the oracle's chain, the sums against numpy's over the same stored
+    vectors, for every launch shape (the plane sweeps fuse the sums into the apply,
+    the other shapes take a separate pass), FP64 and FP32 blocks, with and without a
+    product diagonal."""
+    rng = np.random.default_rng(hash((tuple(dims), variant, march, 3)) % 2**32)
+    terms = _fusable_terms(rng, dims, False)[:-1]
+    if len(dims) >= 2:
+        terms.append((0.7, [(0, "diag", rng.standard_normal(dims[0])), (1, "diag", rng.standard_normal(dims[1]))]))
+    H = to_kfc(dims, terms, constant=0.25)
+    assert H.fused
+    H.set_stencil(variant, march)
+    Ho = to_oracle(dims, terms, constant=0.25)
+    B, A = rng.standard_normal(H.N), rng.standard_normal(H.N)
+    for a, b, g, use_a in ((0.37, -1.2, 0.0, False), (0.74, -2.4, -1.0, True)):
+        Co = a * O.apply_chain(Ho, B[:, None])[:, 0] + b * B + (g * A if use_a else 0)
+        for prec, tol in (("f64", TOL64), ("f32", 50 * TOL32)):
+            dt = torch.float64 if prec == "f64" else torch.float32
+            Bt, At = (torch.tensor(v, dtype=dt, device="cuda:0") for v in (B, A))
+            Ct = torch.empty_like(Bt)
+            d, fused = H.cheb_step_dots(Ct, Bt, At if use_a else None, a, b, g, prec)
+            assert fused == (variant in (2, 3))
+            Cd, Bd = Ct.double().cpu().numpy(), Bt.double().cpu().numpy()
+            assert _rel(Cd, Co) < tol
+            ref = np.array([Bd @ Bd, Bd @ Cd])
+            assert np.all(np.abs(d - ref) <= 1e-12 * np.linalg.norm(Bd) * np.array([np.linalg.norm(Bd), np.linalg.norm(Cd)]))
+
+
 @pytest.mark.parametrize("variant,march", [(-1, 0)] + STENCIL_SHAPES)
 @pytest.mark.parametrize("N", [16, 33, 64, 100])
 def test_three_fluxonia_fused_parity(N, variant, march):

@@ -181,3 +181,22 @@ def test_lanczos_real_block_paths_vs_dense(N):
         assert np.max(r) < 1e-8
         assert np.max(np.abs(E - w[:8])) < 1e-9 * max(1.0, np.max(np.abs(w[:8])))
         assert np.allclose(V.T @ V, np.eye(8), atol=1e-10)
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+@pytest.mark.parametrize("variant,march", [(1, 32), (2, 32), (3, 32)])
+def test_lanczos_fused_moments_vs_dense(mode, variant, march):
+    """The s-step Lanczos takes each block's moments q^T T_j q from the plane
+    sweep's fused sums (variants 2 and 3) or from a Gram pass (variant 1): both
+    reach Dense's spectrum with the same orthonormality."""
+    import kfc
+    from kfc import configs
+    th = configs.three_fluxonia_theta(7)
+    H = configs.three_fluxonia_operator(th, N=18)
+    H.set_stencil(variant, march)
+    Ho = oracle_from_kfc_terms(H)
+    w = np.linalg.eigvalsh(O.to_matrix(Ho))
+    E, V, st = kfc.solve_stats(kfc.Lanczos(tol=1e-10, krylovdim=48, mode=mode), H, 8)
+    assert np.max(np.abs(E - w[:8])) < 1e-9 * max(1.0, np.max(np.abs(w[:8])))
+    assert np.max(O.true_residuals(Ho, E, V)) < 1e-8
+    assert np.allclose(V.T @ V, np.eye(8), atol=1e-10)
