@@ -32,6 +32,16 @@ int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M,
 template <class S>
 int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M, int q, S* Y, int64_t ldy, int64_t N,
               int gram, int pz, std::vector<cd>* G);
+// gram = 5 is gram = 3 without storing Y: only G leaves the call (real blocks with
+// pz + q = p <= 64, q <= 16).  Chained: Z = X M1 (q1 columns, rounded to S, the very
+// values a gram = 5 call forms from the same X and M1) replaces columns pz..pz+q1-1
+// of X for each row, then Y[:, 0:q2] = [X with Z] M2 with G = Y^H Y (q2 x q2); X is
+// not written.  M1 is p x q1, M2 p x q2; tsmm_chain_ok says whether a shape runs.
+template <class S>
+bool tsmm_chain_ok(int p, int pz, int q1, int q2);
+template <class S>
+int blas_tsmm_chain(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M1, int q1, int pz,
+                    const std::vector<cd>& M2, int q2, S* Y, int64_t ldy, int64_t N, std::vector<cd>& G);
 // Column 2-norms of A[:, 0:p] (host).
 template <class S>
 int blas_colnorms(Work& w, const S* A, int64_t lda, int p, int64_t N, std::vector<double>& nrm);

@@ -247,22 +247,37 @@ void gram_mfma(int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const
 // is in LDS.  Y may be columns of X: a slab is read completely before any of its
 // rows is written.  GR = 3 places the slab's Y rows (as stored) in columns
 // pz..pz+q-1 of the staged slab (pz + q <= p): each wave reads and then overwrites
-// only its own 16 rows there, so no other wave's product sees them.
+// only its own 16 rows there, so no other wave's product sees them.  GR = 5 is
+// GR = 3 without storing Y (only its Gram leaves the kernel).  GR = 6 chains two
+// products on the slab: Z = X M (q columns, rounded to S) replaces columns
+// pz..pz+q-1 as in GR = 3, then Y = [X with Z] M2 (q2 <= 16 columns) is stored with
+// its Gram Y^T Y; Z is the very value GR = 5 forms from the same X and M (same
+// products in the same order), so a GR = 5 Gram describes the Z of a later GR = 6.
 constexpr int TS_LDR = 65;
 
 template <class S, int TQ, int GR, int TPZ, int NPF>
 __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t ldx, int p, int ppad,
                                               const double* __restrict__ M, int q, S* Y, int64_t ldy, int64_t N,
-                                              int64_t rows_per, int pz, double* __restrict__ part) {
+                                              int64_t rows_per, int pz, double* __restrict__ part,
+                                              const double* __restrict__ M2, int q2) {
     constexpr int QP = 16 * TQ;
+    static_assert(GR != 6 || TQ == 1, "the chained product has one 16-column tile per stage");
     extern __shared__ double sm[];
     double* sX = sm;                      // ppad columns x 64 rows (stride TS_LDR)
     double* sM = sX + ppad * TS_LDR;      // ppad x QP, row k at sM[k * QP]
     double* sY = sM + ppad * QP;          // 16 columns x 64 rows (GR != 0)
+    double* sM2 = sY + 16 * TS_LDR;       // ppad x 16 (GR = 6)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lo = lane & 15, hi = lane >> 4;
+    const int qy = GR == 6 ? q2 : q;      // columns of Y
     for (int e = tid; e < ppad * QP; e += 256) {
         const int k = e / QP, j = e - k * QP;
         sM[e] = (k < p && j < q) ? M[k + (int64_t)j * p] : 0.0;
+    }
+    if (GR == 6) {
+        for (int e = tid; e < ppad * 16; e += 256) {
+            const int k = e / 16, j = e - k * 16;
+            sM2[e] = (k < p && j < q2) ? M2[k + (int64_t)j * p] : 0.0;
+        }
     }
     const int64_t r0 = blockIdx.x * rows_per, r1 = min(N, r0 + rows_per);
     gd4 g[TPZ];
@@ -280,6 +295,24 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
             pf[j] = (rok && c < p) ? double(X[row + c * ldx]) : 0.0;
         }
     };
+    // M's operands for this lane held in registers (k step i: row 4 i + hi), the
+    // same values in the same order as the LDS reads they replace
+    constexpr bool HOIST = TQ * NPF <= 16;
+    double mr[HOIST ? TQ : 1][HOIST ? NPF : 1];
+    double m2r[GR == 6 ? NPF : 1];
+    if (HOIST || GR == 6) {
+        __syncthreads();   // sM, sM2 in place
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int j = 0; j < TQ; ++j)
+#pragma unroll
+                for (int i = 0; i < NPF; ++i) mr[j][i] = 4 * i < ppad ? sM[(4 * i + hi) * QP + 16 * j + lo] : 0.0;
+        }
+        if constexpr (GR == 6) {
+#pragma unroll
+            for (int i = 0; i < NPF; ++i) m2r[i] = 4 * i < ppad ? sM2[(4 * i + hi) * 16 + lo] : 0.0;
+        }
+    }
     if (r0 < r1) fetch(r0);
     for (int64_t rb = r0; rb < r1; rb += 64) {
         __syncthreads();   // the previous slab's readers are done (and sM is in place)
@@ -293,25 +326,53 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
         gd4 acc[TQ];
 #pragma unroll
         for (int j = 0; j < TQ; ++j) acc[j] = gd4{0, 0, 0, 0};
-        for (int t = 0; t < ppad; t += 4) {
-            const int k = t + hi;
-            const double b = sX[k * TS_LDR + 16 * wv + lo];
+        if constexpr (HOIST) {
 #pragma unroll
-            for (int j = 0; j < TQ; ++j)
-                acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[k * QP + 16 * j + lo], b, acc[j], 0, 0, 0);
+            for (int i = 0; i < NPF; ++i) {
+                if (4 * i < ppad) {
+                    const double b = sX[(4 * i + hi) * TS_LDR + 16 * wv + lo];
+#pragma unroll
+                    for (int j = 0; j < TQ; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(mr[j][i], b, acc[j], 0, 0, 0);
+                }
+            }
+        } else {
+            for (int t = 0; t < ppad; t += 4) {
+                const int k = t + hi;
+                const double b = sX[k * TS_LDR + 16 * wv + lo];
+#pragma unroll
+                for (int j = 0; j < TQ; ++j)
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[k * QP + 16 * j + lo], b, acc[j], 0, 0, 0);
+            }
         }
         // register r of lane l holds Y[16 wv + l % 16][16 j + l / 16 + 4 r]
         const int64_t yrow = rb + 16 * wv + lo;
+        if (GR == 6) {
+            // Z replaces slab columns pz..pz+q-1 of this wave's rows, then Y = slab M2
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = hi + 4 * r;
+                if (col < q) sX[(pz + col) * TS_LDR + 16 * wv + lo] = yrow < r1 ? double(S(acc[0][r])) : 0.0;
+            }
+            __syncthreads();
+            gd4 a2 = gd4{0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < NPF; ++i)
+                if (4 * i < ppad)
+                    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(m2r[i], sX[(4 * i + hi) * TS_LDR + 16 * wv + lo], a2, 0, 0, 0);
+            acc[0] = a2;
+        }
 #pragma unroll
         for (int j = 0; j < TQ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = 16 * j + hi + 4 * r;
-                if (yrow < r1 && col < q) Y[yrow + col * ldy] = S(acc[j][r]);
+                if (GR != 5 && yrow < r1 && col < qy) Y[yrow + col * ldy] = S(acc[j][r]);
                 if (GR != 0 && j == 0) {
-                    const double yv = (yrow < r1 && col < q) ? (GR == 3 ? double(S(acc[0][r])) : acc[0][r]) : 0.0;
+                    const double yv = (yrow < r1 && col < qy)
+                                          ? ((GR == 3 || GR == 5 || GR == 6) ? double(S(acc[0][r])) : acc[0][r])
+                                          : 0.0;
                     sY[col * TS_LDR + 16 * wv + lo] = yv;
-                    if (GR == 3 && col < q) sX[(pz + col) * TS_LDR + 16 * wv + lo] = yv;
+                    if ((GR == 3 || GR == 5) && col < q) sX[(pz + col) * TS_LDR + 16 * wv + lo] = yv;
                 }
             }
         if (GR != 0) {
@@ -320,7 +381,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
             for (int u = 0; u < 4; ++u) {
                 const int rr = 4 * (wv + 4 * u) + hi;
                 const double b = sY[lo * TS_LDR + rr];
-                if (GR == 1 || GR == 3) {
+                if (GR == 1 || GR == 3 || GR == 5) {
 #pragma unroll
                     for (int z = 0; z < TPZ; ++z)
                         g[z] = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[(16 * z + lo) * TS_LDR + rr], b, g[z], 0, 0, 0);
@@ -333,7 +394,7 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
     if (GR == 0) return;
     // sum the waves' Gram tiles in LDS
     double* red = sm;
-    const int gp = GR == 1 ? pz : (GR == 3 ? pz + q : q);
+    const int gp = GR == 1 ? pz : ((GR == 3 || GR == 5) ? pz + q : qy);
     __syncthreads();
     for (int w = 0; w < 4; ++w) {
         if (wv == w) {
@@ -351,34 +412,35 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
     for (int e = tid; e < TPZ * 256; e += 256) {
         const int z = e >> 8, rc = e & 255;
         const int gi = 16 * z + (rc >> 4), gj = rc & 15;
-        if (gi < gp && gj < q) out[gi + (int64_t)gj * gp] = red[e];
+        if (gi < gp && gj < qy) out[gi + (int64_t)gj * gp] = red[e];
     }
 }
 
 template <class S, int TQ, int GR, int TPZ>
 void launch_tsmm(int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p, int ppad, const double* M,
-                 int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz, double* part) {
+                 int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz, double* part, const double* M2,
+                 int q2) {
     // prefetch registers sized to the block width: fewer registers, more waves
     if (ppad <= 16)
         hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 4>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part);
+                           ldy, N, rows_per, pz, part, M2, q2);
     else if (ppad <= 32)
         hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 8>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part);
+                           ldy, N, rows_per, pz, part, M2, q2);
     else if (ppad <= 48)
         hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 12>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part);
+                           ldy, N, rows_per, pz, part, M2, q2);
     else
         hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 20>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part);
+                           ldy, N, rows_per, pz, part, M2, q2);
 }
 
 template <class S>
 void tsmm_dispatch(int gr, int tq, int tpz, int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p,
                    int ppad, const double* M, int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz,
-                   double* part) {
+                   double* part, const double* M2 = nullptr, int q2 = 0) {
 #define KFC_TS(TQv, GRv, TPZv) \
-    launch_tsmm<S, TQv, GRv, TPZv>(nchunk, shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N, rows_per, pz, part)
+    launch_tsmm<S, TQv, GRv, TPZv>(nchunk, shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N, rows_per, pz, part, M2, q2)
     if (gr == 0) {
         switch (tq) {
             case 1: KFC_TS(1, 0, 1); break;
@@ -388,12 +450,21 @@ void tsmm_dispatch(int gr, int tq, int tpz, int nchunk, size_t shmem, hipStream_
         }
     } else if (gr == 2) {
         KFC_TS(1, 2, 1);
+    } else if (gr == 6) {
+        KFC_TS(1, 6, 1);
     } else if (gr == 3) {
         switch (tpz) {
             case 1: KFC_TS(1, 3, 1); break;
             case 2: KFC_TS(1, 3, 2); break;
             case 3: KFC_TS(1, 3, 3); break;
             default: KFC_TS(1, 3, 4); break;
+        }
+    } else if (gr == 5) {
+        switch (tpz) {
+            case 1: KFC_TS(1, 5, 1); break;
+            case 2: KFC_TS(1, 5, 2); break;
+            case 3: KFC_TS(1, 5, 3); break;
+            default: KFC_TS(1, 5, 4); break;
         }
     } else {
         switch (tpz) {
@@ -498,13 +569,14 @@ int blas_gemm(Work& w, const S* A, int64_t lda, int p, const std::vector<cd>& M,
 template <class S>
 int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M, int q, S* Y, int64_t ldy, int64_t N,
               int gram, int pz, std::vector<cd>* G) {
-    const int gp = gram == 1 ? pz : (gram == 3 ? pz + q : q);   // rows of G
+    const int gp = gram == 1 ? pz : ((gram == 3 || gram == 5) ? pz + q : q);   // rows of G
     if (q == 0 || N == 0) {
         if (G) G->assign(size_t(gp) * q, cd(0));
         return KFC_OK;
     }
-    const int gz = gram == 1 || gram == 3 ? gp : 0;   // columns the Gram takes from the slab
+    const int gz = gram == 1 || gram == 3 || gram == 5 ? gp : 0;   // columns the Gram takes from the slab
     bool mfma = !IsCx<S>::value && p <= 80 && q <= 64 && (gram == 0 || (q <= 16 && gz <= 64 && gz <= p));
+    if (gram == 5 && !mfma) return kfc_fail(w.ctx, KFC_EUNSUPPORTED, "tsmm: Gram-only product outside its shapes");
     if (!mfma) {
         // general path: product into scratch, copy, then the Gram
         KFC_HIP(w.ctx, w.tmp.ensure(sizeof(S) * size_t(N) * q));
@@ -557,6 +629,50 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
         } else {
             KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));   // the staged M dies with this call
         }
+    }
+    return KFC_OK;
+}
+
+template <class S>
+bool tsmm_chain_ok(int p, int pz, int q1, int q2) {
+    return !IsCx<S>::value && p <= 64 && q1 >= 1 && q1 <= 16 && q2 >= 1 && q2 <= 16 && pz >= 0 && pz + q1 <= p;
+}
+
+template <class S>
+int blas_tsmm_chain(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M1, int q1, int pz,
+                    const std::vector<cd>& M2, int q2, S* Y, int64_t ldy, int64_t N, std::vector<cd>& G) {
+    if (!tsmm_chain_ok<S>(p, pz, q1, q2)) return kfc_fail(w.ctx, KFC_EUNSUPPORTED, "tsmm: chained product outside its shapes");
+    if (N == 0) {
+        G.assign(size_t(q2) * q2, cd(0));
+        return KFC_OK;
+    }
+    if constexpr (!IsCx<S>::value) {
+        const int ppad = (p + 3) / 4 * 4;
+        const int nchunk = nchunks_for(N, 4096, 2048);
+        const int64_t rows_per = (N + nchunk - 1) / nchunk;
+        std::vector<double> h(size_t(p) * (q1 + q2));
+        for (size_t i = 0; i < size_t(p) * q1; ++i) h[i] = M1[i].real();
+        for (size_t i = 0; i < size_t(p) * q2; ++i) h[size_t(p) * q1 + i] = M2[i].real();
+        KFC_HIP(w.ctx, w.small2.ensure(sizeof(double) * h.size()));
+        KFC_HIP(w.ctx, hipMemcpyAsync(w.small2.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                                      w.ctx->stream));
+        KFC_HIP(w.ctx, w.part.ensure(sizeof(double) * size_t(nchunk) * q2 * q2));
+        KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(q2) * q2));
+        size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 + 16 * TS_LDR + size_t(ppad) * 16;
+        words = std::max<size_t>(words, 256);
+        tsmm_dispatch<S>(6, 1, 1, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
+                         w.small2.as<double>(), q1, Y, ldy, N, rows_per, pz, w.part.as<double>(),
+                         w.small2.as<double>() + size_t(p) * q1, q2);
+        KFC_HIP(w.ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_gram_reduce<double>, dim3(unsigned(q2 * q2)), dim3(256), 0, w.ctx->stream,
+                           w.part.as<double>(), nchunk, int64_t(q2) * q2, w.small.as<double>());
+        KFC_HIP(w.ctx, hipGetLastError());
+        std::vector<double> hg(size_t(q2) * q2);
+        KFC_HIP(w.ctx, hipMemcpyAsync(hg.data(), w.small.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      w.ctx->stream));
+        KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));
+        G.resize(hg.size());
+        for (size_t i = 0; i < hg.size(); ++i) G[i] = cd(hg[i], 0.0);
     }
     return KFC_OK;
 }
@@ -1024,6 +1140,9 @@ int lz_update(Work& w, const S* V, int64_t ldv, int k, const std::vector<cd>& hs
                               double);                                                                              \
     template int blas_tsmm<S>(Work&, const S*, int64_t, int, const std::vector<cd>&, int, S*, int64_t, int64_t, int, \
                               int, std::vector<cd>*);                                                               \
+    template bool tsmm_chain_ok<S>(int, int, int, int);                                                             \
+    template int blas_tsmm_chain<S>(Work&, const S*, int64_t, int, const std::vector<cd>&, int, int,                \
+                                    const std::vector<cd>&, int, S*, int64_t, int64_t, std::vector<cd>&);            \
     template int blas_colnorms<S>(Work&, const S*, int64_t, int, int64_t, std::vector<double>&);                    \
     template int blas_scale_cols<S>(Work&, S*, int64_t, int, int64_t, const std::vector<double>&);                  \
     template int blas_residual<S>(Work&, S*, int64_t, const S*, int64_t, const S*, int64_t, int, int64_t,            \

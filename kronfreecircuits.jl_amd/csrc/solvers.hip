@@ -432,7 +432,11 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
         // subtraction K -= Q C2 is left to the first Cholesky QR scaling, which
         // reads [Q K] anyway; its Gram follows from this pass's: with Q^H Q = I,
         // (K - Q C2)^H (K - Q C2) = K^H K - C2^H C2.
-        std::vector<cd> Cm(size_t(k) * nb0, cd(0)), Gk, C2;
+        std::vector<cd> Cm(size_t(k) * nb0, cd(0)), Gk, C2, M1;
+        // chained passes: the first subtraction's pass only measures [Q K']^T K'
+        // (K' = K - Q C1 is not stored) and the first Cholesky scaling re-forms K' in
+        // registers, bit for bit, before subtracting Q C2 (one write of K per block)
+        const bool chain = k > 0 && tsmm_chain_ok<S>(k + nb0, k, nb0, nb0);
         if (k > 0) {
             std::vector<cd> C1;
             if (relation_exact) {
@@ -486,7 +490,8 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                 return M;
             };
             std::vector<cd> G1;
-            rc = blas_tsmm<S>(w, V, N, k + nb0, upd(C1), nb0, K, N, N, 3, k, &G1);
+            M1 = upd(C1);
+            rc = blas_tsmm<S>(w, V, N, k + nb0, M1, nb0, K, N, N, chain ? 5 : 3, k, &G1);
             if (rc != KFC_OK) return rc;
             const int gp = k + nb0;
             C2.assign(size_t(k) * nb0, cd(0));
@@ -562,7 +567,16 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                     for (int l = 0; l <= j; ++l) Mq[k + l + size_t(j) * pr] = Ri[l + size_t(j) * nind];
                 }
                 std::vector<cd> Gn;
-                rc = blas_tsmm<S>(w, V, N, pr, Mq, nind, K, N, N, 2, 0, &Gn);
+                if (chain) {
+                    // the same map on [Q K'] with K' = K M1 re-formed from the unchanged K
+                    const int pc = k + nb0;
+                    std::vector<cd> M2(size_t(pc) * nind, cd(0));
+                    for (int j = 0; j < nind; ++j)
+                        for (int i = 0; i < pr; ++i) M2[i + size_t(j) * pc] = Mq[i + size_t(j) * pr];
+                    rc = blas_tsmm_chain<S>(w, V, N, pc, M1, nb0, k, M2, nind, K, N, N, Gn);
+                } else {
+                    rc = blas_tsmm<S>(w, V, N, pr, Mq, nind, K, N, N, 2, 0, &Gn);
+                }
                 if (rc != KFC_OK) return rc;
                 G.swap(Gn);
             } else {
