@@ -650,8 +650,8 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
             const double far = resid[std::min(m, n) - 1];
             const double kap = cond_upper(s, Rss);
             const double noise = eps_s * std::max(std::fabs(hi), std::fabs(lo)) * kap;
-            int want = far > 1e3 * tol ? LZ_SMAX : (far > 30 * tol ? 6 : 3);
-            if (noise > far * 1e-2 || kap > 1e5) want = std::max(1, std::min(want, s - 2));
+            int want = far > 1e3 * tol ? LZ_SMAX : (far > 3 * tol ? 6 : 3);
+            if (noise > far * 1e-1 || kap > 1e5) want = std::max(1, std::min(want, s - 2));
             s_cap = std::min(want, s_limit);
         }
         bool stop = conv >= m || dep_last || (n >= kd && restarts >= maxiter);
