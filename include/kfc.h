@@ -99,6 +99,10 @@ typedef struct {
     int32_t warm;            /* 1 when an X0 was used */
     int32_t degree_last;     /* Chebyshev: degree of the last filter */
     int32_t retried;         /* LOBPCG: 1 when the random-block retry ran */
+    double seconds_ortho_gpu; /* Lanczos: HIP-event time of the matrix-core block passes
+                                 (orthogonalization, Cholesky scaling, restart) */
+    double bytes_ortho;       /* Lanczos: their algorithmic bytes (columns read + written) */
+    int64_t n_ortho_calls;    /* Lanczos: their launches */
 } kfc_stats;
 
 typedef struct kfc_ctx kfc_ctx;

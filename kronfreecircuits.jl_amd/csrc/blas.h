@@ -16,6 +16,36 @@ struct Work {
     DevBuf small2;
     DevBuf tmp;       // N x q scratch of the general tall-skinny product
     std::vector<unsigned char> host;
+    // the matrix-core tall-skinny products (k_tsmm): HIP-event time on the context
+    // stream and algorithmic bytes (block columns read + columns written)
+    double tsmm_gpu = 0, tsmm_bytes = 0;
+    int64_t tsmm_calls = 0;
+    hipEvent_t te0 = nullptr, te1 = nullptr;
+    Work() = default;
+    Work(const Work&) = delete;
+    Work& operator=(const Work&) = delete;
+    ~Work() {
+        if (te0) (void)hipEventDestroy(te0);
+        if (te1) (void)hipEventDestroy(te1);
+    }
+    hipError_t tsmm_begin() {
+        if (!te0) {
+            hipError_t e = hipEventCreate(&te0);
+            if (e == hipSuccess) e = hipEventCreate(&te1);
+            if (e != hipSuccess) return e;
+        }
+        return hipEventRecord(te0, ctx->stream);
+    }
+    // after the stream has been synchronized past the kernel
+    hipError_t tsmm_end_synced(double bytes) {
+        float ms = 0;
+        hipError_t e = hipEventElapsedTime(&ms, te0, te1);
+        if (e != hipSuccess) return e;
+        tsmm_gpu += 1e-3 * ms;
+        tsmm_bytes += bytes;
+        ++tsmm_calls;
+        return hipSuccess;
+    }
 };
 
 // G (p x q, column-major, host) = A^H B over N rows.

@@ -613,9 +613,11 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
         }
         size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 * tq + (gram ? 16 * TS_LDR : 0);
         words = std::max<size_t>(words, size_t(gz > 0 ? tpz : 1) * 256);
+        KFC_HIP(w.ctx, w.tsmm_begin());
         tsmm_dispatch<S>(gram, tq, tpz, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
                          w.small2.as<double>(), q, Y, ldy, N, rows_per, pz, gram ? w.part.as<double>() : nullptr);
         KFC_HIP(w.ctx, hipGetLastError());
+        KFC_HIP(w.ctx, hipEventRecord(w.te1, w.ctx->stream));
         if (gram) {
             hipLaunchKernelGGL(k_gram_reduce<double>, dim3(unsigned(gp * q)), dim3(256), 0, w.ctx->stream,
                                w.part.as<double>(), nchunk, int64_t(gp) * q, w.small.as<double>());
@@ -629,6 +631,7 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
         } else {
             KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));   // the staged M dies with this call
         }
+        KFC_HIP(w.ctx, w.tsmm_end_synced(double(N) * sizeof(S) * (p + (gram == 5 ? 0 : q))));
     }
     return KFC_OK;
 }
@@ -660,10 +663,12 @@ int blas_tsmm_chain(Work& w, const S* X, int64_t ldx, int p, const std::vector<c
         KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(q2) * q2));
         size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 + 16 * TS_LDR + size_t(ppad) * 16;
         words = std::max<size_t>(words, 256);
+        KFC_HIP(w.ctx, w.tsmm_begin());
         tsmm_dispatch<S>(6, 1, 1, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
                          w.small2.as<double>(), q1, Y, ldy, N, rows_per, pz, w.part.as<double>(),
                          w.small2.as<double>() + size_t(p) * q1, q2);
         KFC_HIP(w.ctx, hipGetLastError());
+        KFC_HIP(w.ctx, hipEventRecord(w.te1, w.ctx->stream));
         hipLaunchKernelGGL(k_gram_reduce<double>, dim3(unsigned(q2 * q2)), dim3(256), 0, w.ctx->stream,
                            w.part.as<double>(), nchunk, int64_t(q2) * q2, w.small.as<double>());
         KFC_HIP(w.ctx, hipGetLastError());
@@ -673,6 +678,7 @@ int blas_tsmm_chain(Work& w, const S* X, int64_t ldx, int p, const std::vector<c
         KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));
         G.resize(hg.size());
         for (size_t i = 0; i < hg.size(); ++i) G[i] = cd(hg[i], 0.0);
+        KFC_HIP(w.ctx, w.tsmm_end_synced(double(N) * sizeof(S) * (p + q2)));
     }
     return KFC_OK;
 }

@@ -863,6 +863,9 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
         st->seconds_apply_gpu = tm.apply_gpu;
         st->n_converged = int(m);
         st->seconds_total = secs(t0);
+        st->seconds_ortho_gpu = w.tsmm_gpu;
+        st->bytes_ortho = w.tsmm_bytes;
+        st->n_ortho_calls = w.tsmm_calls;
     }
     return KFC_OK;
 }

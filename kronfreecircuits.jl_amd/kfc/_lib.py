@@ -70,7 +70,8 @@ class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("n_matvec", C.c_int64), ("n_matvec_f32", C.c_int64),
                 ("n_apply_calls", C.c_int64), ("seconds_total", C.c_double), ("seconds_apply", C.c_double), ("seconds_apply_gpu", C.c_double),
                 ("max_residual", C.c_double), ("n_converged", C.c_int32), ("warm", C.c_int32),
-                ("degree_last", C.c_int32), ("retried", C.c_int32)]
+                ("degree_last", C.c_int32), ("retried", C.c_int32), ("seconds_ortho_gpu", C.c_double),
+                ("bytes_ortho", C.c_double), ("n_ortho_calls", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
