@@ -428,6 +428,8 @@ def kron_operator(dims, terms, constant=0.0):
     T = np.complex128 if any(_is_complex(v) for v in vals) else np.float64
     D = np.full(dims, constant, dtype=T, order="F")
     kept, fused = [], []
+    if constant != 0:    # the constant is a fused term with no factors (src/apply.jl:237)
+        fused.append(AxisTerm(constant, []))
     for t in terms:
         if not t.factors or all(isinstance(op, DiagonalOp) for _, op in t.factors):
             _add_term_diagonal(D, t.coeff, t.factors)
@@ -528,7 +530,11 @@ def separable_eigenbases(H):
         elif len(t.factors) == 0:
             A[0] = A[0] + t.coeff * np.eye(H.dims[0])
     U, lam = [], []
-    for M in A:
+    for k, M in enumerate(A):
+        asym, nrm = np.linalg.norm(M - M.conj().T), np.linalg.norm(M)
+        if not asym <= math.sqrt(np.finfo(float).eps) * nrm:
+            raise ValueError(f"separable_eigenbases: the single-axis terms on axis {k} sum to a non-Hermitian "
+                             f"matrix (|M - M'| = {asym}, |M| = {nrm}), so the operator is not Hermitian")
         if np.count_nonzero(M - np.diag(np.diag(M))) == 0:
             U.append(np.eye(M.shape[0], dtype=T))
             lam.append(np.real(np.diag(M)).copy())
