@@ -28,14 +28,19 @@ enum kfc_status {
     KFC_EBREAKDOWN = 4,  /* eigensolver breakdown (src/lobpcg.jl:195-198, src/chebyshev.jl:124-125) */
     KFC_EDEVICE = 5,     /* HIP runtime failure */
     KFC_ENOMEM = 6,      /* device allocation failure */
-    KFC_EUNSUPPORTED = 7 /* a combination this build does not serve */
+    KFC_EUNSUPPORTED = 7, /* a combination this build does not serve */
+    KFC_EABORT = 8       /* the solver callback asked to stop: the reference's callback threw,
+                            and its exception propagates with no retry (src/lobpcg.jl:247-253, 279) */
 };
 
 /* ---- element types: the operator's type T (src/apply.jl:232-233) ---------------- */
 enum kfc_dtype { KFC_F64 = 0, KFC_C128 = 1 };
 
 /* ---- arithmetic precision of an application (FP64 or the FP32 bulk path) --------- */
-enum kfc_precision { KFC_PREC_F64 = 0, KFC_PREC_F32 = 1 };
+/* KFC_PREC_Z64: interleaved complex128 blocks whatever the operator's type; on a real
+ * (KFC_F64) operator this is _mul_real_operator! (src/apply.jl:397-407), H applied to
+ * the real and imaginary parts, on a complex one the same as KFC_PREC_F64. */
+enum kfc_precision { KFC_PREC_F64 = 0, KFC_PREC_F32 = 1, KFC_PREC_Z64 = 2 };
 
 /* ---- solver precision mode ------------------------------------------------------- */
 enum kfc_mode {
@@ -115,6 +120,36 @@ const char* kfc_last_error(const kfc_ctx* ctx);
 /* The HIP stream every call of this context runs on (a hipStream_t). */
 void* kfc_ctx_stream(kfc_ctx* ctx);
 
+/* ---- per-iteration solver callback ----------------------------------------------------
+ * The record a solver hands its callback once per iteration: the fields of the `info`
+ * the reference passes to `callback(info)` from LOBPCG (n_iter, n_matvec, n_locked,
+ * lambdas of the active columns; LOBPCGEigensolver 0.1.4, src/lobpcg.jl:1-8, 144-157)
+ * and from ChebyshevFilter (iteration, lambda, residuals, n_locked, lambda_cut,
+ * lambda_max, degree; src/chebyshev.jl:353-363).  The arrays are valid during the call. */
+typedef struct {
+    int32_t solver;          /* 0 = LOBPCG, 1 = ChebyshevFilter */
+    int32_t precision;       /* KFC_PREC_F64, or KFC_PREC_F32 in the mixed path's bulk phase */
+    int64_t iteration;       /* LOBPCG: n_iter (1 at the first call); Chebyshev: filters applied so far */
+    int64_t n_matvec;        /* H columns applied so far in this solve */
+    int32_t width;           /* entries of lambda and residuals (the block width) */
+    int32_t n_locked;        /* converged columns (a prefix), counted this iteration */
+    int32_t active0;         /* first column active during the iteration (LOBPCG's lambdas = lambda[active0:]) */
+    int32_t degree;          /* ChebyshevFilter: degree of the last filter (0 before the first) */
+    const double* lambda;    /* Ritz values, ascending within the active columns */
+    const double* residuals; /* residual norms ||H x - theta x|| */
+    double lambda_cut;       /* ChebyshevFilter: lower end of the damped interval (NaN before the first filter) */
+    double lambda_max;       /* ChebyshevFilter: upper end of the filter interval */
+} kfc_iter_info;
+
+/* Returns 0 to continue; any other value stops the solve with KFC_EABORT. */
+typedef int (*kfc_callback)(const kfc_iter_info* info, void* user);
+
+/* Replaces the `callback` keyword of LOBPCG and ChebyshevFilter (src/tree.jl:66-87,
+ * 293-321): kfc_lobpcg and kfc_chebyshev on this context call cb(info, user) once per
+ * iteration from the calling host thread (both phases of the mixed mode).  NULL clears
+ * it.  The batched and lockstep solvers do not call it. */
+int kfc_ctx_set_callback(kfc_ctx* ctx, kfc_callback cb, void* user);
+
 /* ---- operator ------------------------------------------------------------------- */
 /* Replaces KronFreeOperator(dims, terms; constant, block_capacity, on)
  * (src/apply.jl:222-253) with on = a device array type, and _on_storage
@@ -143,7 +178,8 @@ int kfc_op_work(const kfc_op* op, double* bytes, double* flops);
 /* Replaces LinearAlgebra.mul!(y, H, x) (src/apply.jl:416-432):
  * Y = H X for ncols columns with leading dimension ld (>= prod(dims)).
  * X, Y are device pointers of FP64 (or interleaved complex128) when precision is
- * KFC_PREC_F64, FP32 (complex64) when KFC_PREC_F32.  Y must not alias X
+ * KFC_PREC_F64, FP32 (complex64) when KFC_PREC_F32, complex128 for KFC_PREC_Z64 (a real
+ * operator on complex blocks, src/apply.jl:397-407).  Y must not alias X
  * (ArgumentError, src/apply.jl:412).  Enqueued on the context stream; returns
  * after the launch (kfc_sync waits). */
 int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision);

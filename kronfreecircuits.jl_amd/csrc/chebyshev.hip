@@ -260,6 +260,8 @@ struct ChebCfg {
     int m, degree, extra, rescale_every, ortho;
     int64_t maxiter;
     double bound, tol, hbound, lmax;
+    bool callback = false;   // hand each iteration to the context's callback
+    int64_t it0 = 0;         // iterations of an earlier phase (the mixed path's FP32 run)
 };
 
 // One ChFSI run on scalar S from the seed (N x nb): the FP64 device block X0d, else the
@@ -324,6 +326,23 @@ int cheb_run(kfc_op* op, const ChebCfg& c, const std::vector<cd>& X0h, uint64_t 
         const bool filtered = it >= 1 || nb == N || seed_locked_ok;
         const double floor = 10 * eps_s * std::max(std::fabs(c.hbound), std::fabs(*std::min_element(th.begin(), th.end())));
         while (filtered && nlock < nb && res[nlock] <= LOCK_FRACTION * std::max(c.tol, floor)) ++nlock;
+        if (c.callback) {   // src/chebyshev.jl:362-363
+            kfc_iter_info info{};
+            info.solver = 1;
+            info.precision = prec_of<S>();
+            info.iteration = c.it0 + it;
+            info.n_matvec = tm.cols;
+            info.width = nb;
+            info.n_locked = nlock;
+            info.active0 = 0;
+            info.degree = deg;
+            info.lambda = th.data();
+            info.residuals = res.data();
+            info.lambda_cut = lcut;
+            info.lambda_max = lmax;
+            rc = ctx_callback(op->ctx, info, "ChebyshevFilter");
+            if (rc != KFC_OK) return rc;
+        }
         const int need = m + counted_guards(th, m, g, lcut, lmax, deg);
         if (chebyshev_done(th, nlock, m, need)) break;
         if (switch_tol > 0 && it >= 1) {
@@ -421,6 +440,7 @@ int chebyshev_t(kfc_op* op, int64_t m, int degree, int64_t extra, int64_t maxite
     int rc = kfc_op_gershgorin(op, &hb);
     if (rc != KFC_OK) return rc;
     ChebCfg c{int(m), degree, int(extra), rescale_every, ortho, maxiter, bound, tol, hb, std::isnan(bound) ? hb : bound};
+    c.callback = op->ctx->cb != nullptr;
     // seed, formed on the device: the product seed plus a 1e-2 random admixture
     // (_admixed_seed, src/chebyshev.jl:252-261), or a random block
     DevBuf dX0;
@@ -451,6 +471,7 @@ int chebyshev_t(kfc_op* op, int64_t m, int degree, int64_t extra, int64_t maxite
         if (rc != KFC_OK) return rc;
         ChebCfg c2 = c;
         c2.maxiter = std::max<int64_t>(1, maxiter - it32);
+        c2.it0 = it32 + 1;
         std::vector<cd> Xs = X;
         rc = cheb_run<S>(op, c2, Xs, seed + 1, tm, th, X, nlock, it, deg, 0.0, false);
         it += it32;

@@ -154,6 +154,18 @@ int check_block(kfc_op* op, const void* V, const void* W, int64_t m, int64_t ld,
 
 }  // namespace
 
+// Y = H X for complex128 blocks and a real H (_mul_real_operator!, src/apply.jl:397-407):
+// the affine apply with alpha = scale = 1, no shift, no beta, at most 4096 columns a launch.
+int op_apply_real_on_complex(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld) {
+    for (int64_t c0 = 0; c0 < ncols; c0 += 4096) {
+        const int64_t m = std::min<int64_t>(4096, ncols - c0);
+        int rc = affine_apply(op, static_cast<const double2*>(X) + size_t(c0) * ld, static_cast<double2*>(Y) + size_t(c0) * ld,
+                              m, ld, cd(1), 1.0, 0.0, cd(0), nullptr);
+        if (rc != KFC_OK) return rc;
+    }
+    return KFC_OK;
+}
+
 extern "C" {
 
 int kfc_apply_affine(kfc_op* op, const void* V, void* W, int64_t ncols, int64_t ld, double alpha_re, double alpha_im,

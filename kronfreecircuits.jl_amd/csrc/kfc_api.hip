@@ -15,6 +15,12 @@ int kfc_fail(kfc_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
+int ctx_callback(kfc_ctx* ctx, const kfc_iter_info& info, const char* who) {
+    if (!ctx->cb || ctx->cb(&info, ctx->cb_user) == 0) return KFC_OK;
+    return kfc_fail(ctx, KFC_EABORT, std::string(who) + ": the callback stopped the solve at iteration " +
+                                         std::to_string(info.iteration));
+}
+
 extern "C" {
 
 int kfc_ctx_create(kfc_ctx** out, int device) {
@@ -51,6 +57,13 @@ int kfc_ctx_destroy(kfc_ctx* ctx) {
 const char* kfc_last_error(const kfc_ctx* ctx) { return ctx ? ctx->err.c_str() : g_noctx_err.c_str(); }
 
 void* kfc_ctx_stream(kfc_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int kfc_ctx_set_callback(kfc_ctx* ctx, kfc_callback cb, void* user) {
+    if (!ctx) return kfc_fail(nullptr, KFC_EARG, "kfc_ctx_set_callback: null context");
+    ctx->cb = cb;
+    ctx->cb_user = cb ? user : nullptr;
+    return KFC_OK;
+}
 
 int kfc_sync(kfc_ctx* ctx) {
     if (!ctx) return kfc_fail(nullptr, KFC_EARG, "kfc_sync: null context");
@@ -212,8 +225,12 @@ int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march) {
 
 int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision) {
     if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_apply: null operator");
-    if (precision != KFC_PREC_F64 && precision != KFC_PREC_F32)
+    if (precision != KFC_PREC_F64 && precision != KFC_PREC_F32 && precision != KFC_PREC_Z64)
         return kfc_fail(op->ctx, KFC_EARG, "kfc_apply: unknown precision");
+    // complex FP64 blocks: a complex operator's own FP64 apply, or a real one's
+    // _mul_real_operator! (src/apply.jl:397-407)
+    const bool real_on_complex = precision == KFC_PREC_Z64 && !op->cplx;
+    if (precision == KFC_PREC_Z64 && op->cplx) precision = KFC_PREC_F64;
     if (ncols < 0) return kfc_fail(op->ctx, KFC_EARG, "kfc_apply: negative column count");
     if (ld < op->N)
         return kfc_fail(op->ctx, KFC_EDIM, "KronFreeOperator of size (" + std::to_string(op->N) + ", " +
@@ -221,11 +238,12 @@ int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int
                                                std::to_string(ld));
     if (ncols == 0) return KFC_OK;
     if (!X || !Y) return kfc_fail(op->ctx, KFC_EARG, "kfc_apply: null buffer");
-    const size_t es = (op->cplx ? 2 : 1) * (precision == KFC_PREC_F64 ? 8 : 4);
+    const size_t es = (op->cplx || real_on_complex ? 2 : 1) * (precision == KFC_PREC_F32 ? 4 : 8);
     const char* xb = static_cast<const char*>(X);
     const char* yb = static_cast<const char*>(Y);
     const size_t span = size_t(((ncols - 1) * ld + op->N)) * es;
     if (xb < yb + span && yb < xb + span) return kfc_fail(op->ctx, KFC_EARG, "mul!: y must not alias x");
+    if (real_on_complex) return op_apply_real_on_complex(op, X, Y, ncols, ld);
     return op_apply(op, X, Y, ncols, ld, precision, op->ctx->stream);
 }
 

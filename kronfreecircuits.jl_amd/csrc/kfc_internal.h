@@ -81,9 +81,14 @@ struct kfc_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int num_cus = 256;
+    kfc_callback cb = nullptr;      // per-iteration solver callback (kfc_ctx_set_callback)
+    void* cb_user = nullptr;
 };
 
 int kfc_fail(kfc_ctx* ctx, int code, const std::string& msg);
+// Hands info to the context's solver callback, if any: KFC_EABORT (with the message set)
+// when the callback asks to stop, else KFC_OK.
+int ctx_callback(kfc_ctx* ctx, const kfc_iter_info& info, const char* who);
 
 #define KFC_HIP(ctx, call)                                                                     \
     do {                                                                                       \
@@ -274,6 +279,9 @@ int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int 
 // path (op->plan.ok), the stencil running on complex tables (op->ddz).
 int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
                       hipStream_t st);
+// Y = H X for interleaved complex128 blocks X, Y (leading dimension ld) and a real H
+// (evolve.hip): the reference's _mul_real_operator!.
+int op_apply_real_on_complex(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld);
 // Chebyshev step fused into the apply: C = a*(H B) + b*B + g*A  (A may be null)
 int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t ncols, int64_t ld, int prec,
                   double a, double b, double g, hipStream_t st);
@@ -282,6 +290,12 @@ int op_apply_cheb(kfc_op* op, const void* B, const void* A, void* C, int64_t nco
 // not written (fused = false).  Real blocks only.
 int op_apply_cheb_dots(kfc_op* op, const void* B, const void* A, void* C, int64_t ld, int prec, double a, double b,
                        double g, double* dots, bool& fused, hipStream_t st);
+
+// kfc_lobpcg with the context's callback on or off (solvers.hip); the lockstep sweep
+// re-solves collapsed points through it with the callback off.
+int lobpcg_solve(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond, double eta,
+                 int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode, double* E, double* V,
+                 kfc_stats* stats, bool use_callback);
 
 // Host linear algebra (linalg.cpp): Hermitian eigendecomposition, ascending.
 // A is n x n column-major (overwritten); w[n]; Z[n x n] eigenvectors.

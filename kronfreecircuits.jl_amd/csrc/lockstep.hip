@@ -1209,8 +1209,8 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
     // points whose lockstep basis collapsed: the one-point path, with its retry
     for (int i = 0; i < n; ++i) {
         if (!pts[i].failed) continue;
-        rc = kfc_lobpcg(ops[i], m, guard, tol, maxiter, precond, eta, seed_kind, X0s ? X0s[i] : nullptr, seed + i,
-                        mode, E + size_t(i) * m, Vs[i], stats ? stats + i : nullptr);
+        rc = lobpcg_solve(ops[i], m, guard, tol, maxiter, precond, eta, seed_kind, X0s ? X0s[i] : nullptr, seed + i,
+                          mode, E + size_t(i) * m, Vs[i], stats ? stats + i : nullptr, false);
         if (rc != KFC_OK) return kfc_fail(ctx, rc, "instance " + std::to_string(i) + ": " + ops[i]->ctx->err);
     }
     if (stats) {

@@ -881,6 +881,8 @@ struct LobpcgCfg {
     int64_t maxiter;
     int precond;
     double eta;
+    bool callback = false;   // hand each iteration to the context's callback
+    int64_t it0 = 0;         // iterations of an earlier phase (the mixed path's FP32 run)
 };
 
 struct LobpcgRes {
@@ -1052,6 +1054,24 @@ int lobpcg_core(kfc_op* op, Work& w, const LobpcgCfg& cfg, const std::vector<cd>
         int newly = 0;
         while (nlock + newly < W && res[nlock + newly] <= cfg.tol) ++newly;
         out.iterations = it;
+        if (cfg.callback) {
+            // info after the locking count, lambdas of the columns active during the
+            // iteration (src/lobpcg.jl:5-8)
+            kfc_iter_info info{};
+            info.solver = 0;
+            info.precision = prec_of<S>();
+            info.iteration = cfg.it0 + it + 1;
+            info.n_matvec = tm.cols;
+            info.width = W;
+            info.n_locked = nlock + newly;
+            info.active0 = nlock;
+            info.lambda = theta.data();
+            info.residuals = res.data();
+            info.lambda_cut = NAN;
+            info.lambda_max = NAN;
+            rc = ctx_callback(op->ctx, info, "LOBPCG");
+            if (rc != KFC_OK) return rc;
+        }
         bool done = nlock + newly >= W;
         double worst = 0;
         for (int j = nlock; j < W; ++j) worst = std::max(worst, res[j]);
@@ -1217,7 +1237,8 @@ void random_block(std::vector<cd>& X, int64_t N, int p, bool cplx, uint64_t seed
 
 template <class S>
 int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int precond, double eta, int seed_kind,
-             const double* X0, uint64_t seed, int mode, double* Eout, double* Vout, kfc_stats* st) {
+             const double* X0, uint64_t seed, int mode, double* Eout, double* Vout, kfc_stats* st,
+             bool use_callback = false) {
     auto t0 = clk::now();
     const int64_t N = op->N;
     if (m < 1) return kfc_fail(op->ctx, KFC_EARG, "LOBPCG requires m >= 1, got " + std::to_string(m));
@@ -1259,6 +1280,7 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
     w.ctx = op->ctx;
     Timer tm;
     LobpcgCfg cfg{int(m), W, tol, maxiter, precond, eta};
+    cfg.callback = use_callback && op->ctx->cb != nullptr;
     const double* Dp = dD.as<double>();
     LobpcgRes res;
     bool retried = false;
@@ -1280,6 +1302,7 @@ int lobpcg_t(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, 
             }
             LobpcgCfg c2 = cfg;
             c2.maxiter = std::max<int64_t>(1, cfg.maxiter - r32.iterations);
+            c2.it0 = r32.iterations + 1;
             rc = lobpcg_core<S>(op, w, c2, r32.X, Dp, tm, res, 0.0);
             res.iterations += r32.iterations;
             return rc;
@@ -1451,6 +1474,16 @@ void lobpcg_seed_block(const kfc_op* op, int m, int W, const std::vector<cd>& X0
     }
 }
 
+int lobpcg_solve(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond, double eta,
+                 int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode, double* E, double* V,
+                 kfc_stats* stats, bool use_callback) {
+    if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_lobpcg: null argument");
+    return op->cplx ? lobpcg_t<Cx<double>>(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V,
+                                           stats, use_callback)
+                    : lobpcg_t<double>(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V, stats,
+                                       use_callback);
+}
+
 extern "C" {
 
 int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats) {
@@ -1469,9 +1502,7 @@ int kfc_lanczos(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t ma
 int kfc_lobpcg(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond, double eta,
                int32_t seed_kind, const double* X0, uint64_t seed, int32_t mode, double* E, double* V,
                kfc_stats* stats) {
-    if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_lobpcg: null argument");
-    return op->cplx ? lobpcg_t<Cx<double>>(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V, stats)
-                    : lobpcg_t<double>(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V, stats);
+    return lobpcg_solve(op, m, guard, tol, maxiter, precond, eta, seed_kind, X0, seed, mode, E, V, stats, true);
 }
 
 int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid) {

@@ -1,6 +1,6 @@
 """kfc: the Kronecker-free apply and eigensolver loops of KronFreeCircuits.jl on
 the MI355X, over the C ABI of libkfc.so (include/kfc.h)."""
-from ._lib import (Context, ArgumentError, BreakdownError, ConvergenceError, DeviceError, DimensionMismatch,
+from ._lib import (Context, AbortError, ArgumentError, BreakdownError, ConvergenceError, DeviceError, DimensionMismatch,
                    UnsupportedError, context, lib, missing_symbols)
 from .bases import ChargeBasis, FockBasis, PeriodicGrid, PhaseGrid, fd_weights, localop
 from .localops import BandedOp, DenseOp, DiagonalOp, LocalOp, ShiftOp, dense, imagpart, realpart
@@ -12,7 +12,7 @@ from .tree import (MaterializedNode, Node, NumericSpec, System, circuit_terms, c
 from .evolution import DriveHamiltonian, InteractionApply, KronFreeApply, evolve_rk4
 from .resolvent import Resolvent, apply_resolvent, complement_solve, resolvent
 
-__all__ = ["Context", "ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
+__all__ = ["Context", "AbortError", "ArgumentError", "BreakdownError", "ConvergenceError", "DeviceError", "DimensionMismatch",
            "UnsupportedError", "context", "lib", "missing_symbols", "ChargeBasis", "FockBasis", "PeriodicGrid",
            "PhaseGrid", "fd_weights", "localop", "BandedOp", "DenseOp", "DiagonalOp", "LocalOp", "ShiftOp", "dense",
            "imagpart", "realpart", "AxisTerm", "KronFreeOperator", "LOBPCG", "Auto", "ChebyshevFilter", "Dense",

@@ -10,9 +10,9 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkfc.so")
 
-KFC_OK, KFC_EARG, KFC_EDIM, KFC_ENOCONV, KFC_EBREAKDOWN, KFC_EDEVICE, KFC_ENOMEM, KFC_EUNSUPPORTED = range(8)
+KFC_OK, KFC_EARG, KFC_EDIM, KFC_ENOCONV, KFC_EBREAKDOWN, KFC_EDEVICE, KFC_ENOMEM, KFC_EUNSUPPORTED, KFC_EABORT = range(9)
 KFC_F64, KFC_C128 = 0, 1
-KFC_PREC_F64, KFC_PREC_F32 = 0, 1
+KFC_PREC_F64, KFC_PREC_F32, KFC_PREC_Z64 = 0, 1, 2
 KFC_MODE_F64, KFC_MODE_MIXED = 0, 1
 KFC_DIAG, KFC_BANDED, KFC_SHIFT, KFC_DENSE = 0, 1, 2, 3
 KFC_PRECOND = {"ritz": 0, "jacobi": 1, "none": 2}
@@ -43,9 +43,14 @@ class UnsupportedError(RuntimeError):
     """A combination this build does not serve."""
 
 
+class AbortError(RuntimeError):
+    """A solve stopped by its callback (the callback's own exception is re-raised
+    in its place when there is one)."""
+
+
 _ERR = {KFC_EARG: ArgumentError, KFC_EDIM: DimensionMismatch, KFC_ENOCONV: ConvergenceError,
         KFC_EBREAKDOWN: BreakdownError, KFC_EDEVICE: DeviceError, KFC_ENOMEM: DeviceError,
-        KFC_EUNSUPPORTED: UnsupportedError}
+        KFC_EUNSUPPORTED: UnsupportedError, KFC_EABORT: AbortError}
 
 
 class FactorDesc(C.Structure):
@@ -77,6 +82,16 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class IterInfo(C.Structure):
+    """kfc_iter_info: one iteration of a LOBPCG or ChebyshevFilter solve."""
+    _fields_ = [("solver", C.c_int32), ("precision", C.c_int32), ("iteration", C.c_int64), ("n_matvec", C.c_int64),
+                ("width", C.c_int32), ("n_locked", C.c_int32), ("active0", C.c_int32), ("degree", C.c_int32),
+                ("lambda_", C.POINTER(C.c_double)), ("residuals", C.POINTER(C.c_double)),
+                ("lambda_cut", C.c_double), ("lambda_max", C.c_double)]
+
+
+CALLBACK = C.CFUNCTYPE(C.c_int, C.POINTER(IterInfo), C.c_void_p)
+
 # every symbol include/kfc.h declares, with its signature (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -85,6 +100,7 @@ SIGNATURES = {
     "kfc_ctx_destroy": (C.c_int, [_P]),
     "kfc_last_error": (C.c_char_p, [_P]),
     "kfc_ctx_stream": (_P, [_P]),
+    "kfc_ctx_set_callback": (C.c_int, [_P, CALLBACK, _P]),
     "kfc_op_create": (C.c_int, [_P, C.POINTER(OpDesc), C.POINTER(_P)]),
     "kfc_op_destroy": (C.c_int, [_P]),
     "kfc_op_size": (C.c_int64, [_P]),
@@ -182,6 +198,7 @@ class Context:
         check(lib().kfc_ctx_create(C.byref(h), int(device)), None)
         self.handle = h
         self.device = device
+        self._torch_stream = None
 
     def sync(self):
         check(lib().kfc_sync(self.handle), self.handle)
@@ -189,6 +206,26 @@ class Context:
     @property
     def stream(self):
         return lib().kfc_ctx_stream(self.handle)
+
+    def torch_stream(self):
+        """The library's stream as a torch stream (torch.cuda.ExternalStream)."""
+        if self._torch_stream is None:
+            import torch
+            self._torch_stream = torch.cuda.ExternalStream(self.stream, device=torch.device("cuda", self.device))
+        return self._torch_stream
+
+    def wait_torch(self):
+        """Order the library's stream after the work torch has queued on its current
+        stream of this device: the library may then read tensors torch just wrote and
+        write tensors torch's caching allocator just handed out."""
+        import torch
+        self.torch_stream().wait_stream(torch.cuda.current_stream(torch.device("cuda", self.device)))
+
+    def torch_wait(self):
+        """Order torch's current stream of this device after the work queued on the
+        library's stream, so torch sees the library's results."""
+        import torch
+        torch.cuda.current_stream(torch.device("cuda", self.device)).wait_stream(self.torch_stream())
 
     def __del__(self):
         try:

@@ -159,19 +159,57 @@ class KronFreeOperator:
             raise L.DimensionMismatch("a block is a (b, N) tensor or an N-vector")
         return X.shape[0], X.stride(0)
 
+    def _check_tensors(self, name, prec, *tensors):
+        """Device tensors of this operator's device whose element type matches the
+        operator's type at precision ``prec`` (the reference dispatches on the element
+        type and refuses what it cannot apply)."""
+        import torch
+        if prec not in ("f64", "f32"):
+            raise L.ArgumentError(f"{name}: precision must be 'f64' or 'f32', got {prec!r}")
+        cplx = self.dtype == np.complex128
+        want = {("f64", False): torch.float64, ("f32", False): torch.float32,
+                ("f64", True): torch.complex128, ("f32", True): torch.complex64}[(prec, cplx)]
+        for T in tensors:
+            if not T.is_cuda or (T.device.index or 0) != self.ctx.device:
+                raise L.ArgumentError(f"{name}: blocks must live on cuda:{self.ctx.device}, got {T.device}")
+            if T.dtype != want:
+                raise L.ArgumentError(f"{name}: a {'complex' if cplx else 'real'} operator at precision {prec!r} "
+                                      f"takes {want} blocks, got {T.dtype}")
+
     def mul(self, Y, X, precision="f64"):
-        """``mul!(Y, H, X)`` (src/apply.jl:416-432) on device tensors."""
+        """``mul!(Y, H, X)`` (src/apply.jl:416-432) on device tensors.  Complex128
+        blocks on a real operator take ``_mul_real_operator!`` (src/apply.jl:397-407):
+        H applied to the real and imaginary parts.  The library's stream is ordered
+        after torch's current stream before the launch, and torch's current stream
+        after the launch, so torch tensors go in and out with no explicit sync."""
+        import torch
         if X.shape != Y.shape:
             raise L.DimensionMismatch(f"mul!: size(y) = {tuple(Y.shape)} differs from size(x) = {tuple(X.shape)}")
         if X.shape[-1] != self.N:
             raise L.DimensionMismatch(f"KronFreeOperator of size {self.shape} applied to x with {X.shape[-1]} rows")
         if X.stride(-1) != 1 or Y.stride(-1) != 1 or (X.dim() == 2 and X.stride(0) != Y.stride(0)):
             raise L.ArgumentError("mul!: blocks must be row-contiguous with equal row strides")
-        prec = L.KFC_PREC_F64 if precision == "f64" else L.KFC_PREC_F32
+        if self.dtype == np.float64 and X.dtype == torch.complex128 and precision == "f64":
+            self._check_cplx_on_real(X, Y)
+            prec = L.KFC_PREC_Z64
+        else:
+            self._check_tensors("mul!", precision, X, Y)
+            prec = L.KFC_PREC_F64 if precision == "f64" else L.KFC_PREC_F32
         b, ld = self._blk(X)
+        self.ctx.wait_torch()
         L.check(L.lib().kfc_apply(self.handle, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), b, ld, prec),
                 self.ctx.handle)
+        self.ctx.torch_wait()
         return Y
+
+    def _check_cplx_on_real(self, X, Y):
+        import torch
+        for T in (X, Y):
+            if not T.is_cuda or (T.device.index or 0) != self.ctx.device:
+                raise L.ArgumentError(f"mul!: blocks must live on cuda:{self.ctx.device}, got {T.device}")
+            if T.dtype != torch.complex128:
+                raise L.ArgumentError(f"mul!: a real operator takes complex128 blocks only as complex128 in and out, "
+                                      f"got {T.dtype}")
 
     def cheb_step_dots(self, Y, B, A=None, a=1.0, b=0.0, g=0.0, precision="f64"):
         """``Y = a H B + b B + g A`` on one device vector (the step of
@@ -181,16 +219,20 @@ class KronFreeOperator:
         for T in (B, Y) + ((A,) if A is not None else ()):
             if T.dim() != 1 or T.shape[0] != self.N or T.stride(0) != 1:
                 raise L.DimensionMismatch(f"cheb_step_dots: contiguous vectors of {self.N} entries")
+        self._check_tensors("cheb_step_dots", precision, B, Y, *((A,) if A is not None else ()))
         prec = L.KFC_PREC_F64 if precision == "f64" else L.KFC_PREC_F32
         d = np.zeros(2)
         fz = C.c_int32(0)
+        self.ctx.wait_torch()
         L.check(L.lib().kfc_apply_cheb_dots(self.handle, C.c_void_p(B.data_ptr()),
                                             C.c_void_p(A.data_ptr() if A is not None else 0),
                                             C.c_void_p(Y.data_ptr()), float(a), float(b), float(g), prec,
                                             d.ctypes.data_as(L._D), C.byref(fz)), self.ctx.handle)
+        self.ctx.torch_wait()
         return d, bool(fz.value)
 
     def __matmul__(self, X):
+        """``H * x`` (src/apply.jl:434-435): a new block of the promoted element type."""
         import torch
         Y = torch.empty_like(X)
         self.mul(Y, X, "f64" if X.dtype in (torch.float64, torch.complex128) else "f32")
@@ -221,6 +263,7 @@ class KronFreeOperator:
         float64 tensor of length N on this operator's device."""
         import torch
         D = torch.empty(self.N, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        self.ctx.wait_torch()
         L.check(L.lib().kfc_op_diagonal_device(self.handle, C.c_void_p(D.data_ptr())), self.ctx.handle)
         return D
 
@@ -231,6 +274,7 @@ class KronFreeOperator:
         import torch
         dt = torch.complex128 if self.dtype == np.complex128 else torch.float64
         X = torch.empty(int(k), self.N, dtype=dt, device=f"cuda:{self.ctx.device}")
+        self.ctx.wait_torch()
         L.check(L.lib().kfc_op_product_seed_device(self.handle, int(k), C.c_void_p(X.data_ptr()), self.N),
                 self.ctx.handle)
         return X.t()
@@ -243,9 +287,17 @@ def unpack_block(buf, N, k, dtype):
     return buf.reshape(N, k, order="F")
 
 
-def pack_block(X, dtype):
+def pack_block(X, dtype, name="block"):
+    """A host block in the C ABI's layout (column-major, complex interleaved).  A
+    complex block for a real operator is accepted only when it is exactly real
+    (src/lobpcg.jl:226-230); anything else raises instead of losing its imaginary part."""
     X = np.asarray(X)
     if dtype == np.complex128:
         Xf = np.asfortranarray(X.astype(np.complex128)).reshape(-1, order="F")
         return np.ascontiguousarray(np.stack([Xf.real, Xf.imag], axis=-1).reshape(-1))
-    return np.ascontiguousarray(np.asfortranarray(np.real(X).astype(np.float64)).reshape(-1, order="F"))
+    if np.iscomplexobj(X):
+        if np.any(np.imag(X) != 0):
+            raise L.ArgumentError(f"{name} is complex but the operator is real; pass a real block or complexify "
+                                  "the operator")
+        X = np.real(X)
+    return np.ascontiguousarray(np.asfortranarray(X.astype(np.float64)).reshape(-1, order="F"))

@@ -46,7 +46,10 @@ def complement_solve(H, D, basis, B, shifts, tol, maxiter, eta=0.01, scale=None)
     n = B.shape[0]
     basis = _dev(H, basis, "complement_solve") if basis is not None and len(basis) else None
     k = 0 if basis is None else basis.shape[0]
-    D = torch.as_tensor(D).to(device=B.device, dtype=torch.float64).contiguous()
+    D = torch.as_tensor(D).to(device=B.device, dtype=torch.float64).contiguous().reshape(-1)
+    if D.numel() != H.N:
+        raise L.DimensionMismatch(f"complement_solve: the diagonal has {D.numel()} entries for an operator of "
+                                  f"size {H.N}")
     shifts = np.ascontiguousarray(np.asarray(shifts, dtype=np.float64).reshape(-1))
     if len(shifts) != n:
         raise L.DimensionMismatch(f"{n} columns and {len(shifts)} shifts")

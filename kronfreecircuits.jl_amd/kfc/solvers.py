@@ -121,6 +121,60 @@ def resolve(s, info):
     return LOBPCG(preconditioner="ritz", seed="product", mode=s.mode)
 
 
+def _iter_info(ci):
+    """The ``info`` a callback receives: LOBPCG's fields (n_iter, n_matvec, n_locked,
+    λs of the active columns, their residual norms; src/lobpcg.jl:1-8) or
+    ChebyshevFilter's (iteration, λ, residuals, n_locked, λ_cut, λ_max, degree;
+    src/chebyshev.jl:362-363), copies the callback may keep."""
+    import types
+    w = ci.width
+    lam = np.ctypeslib.as_array(ci.lambda_, shape=(w,)).copy() if w else np.zeros(0)
+    res = np.ctypeslib.as_array(ci.residuals, shape=(w,)).copy() if w else np.zeros(0)
+    prec = "f32" if ci.precision == L.KFC_PREC_F32 else "f64"
+    if ci.solver == 0:
+        a = ci.active0
+        return types.SimpleNamespace(**{"n_iter": ci.iteration, "iteration": ci.iteration, "n_matvec": ci.n_matvec,
+                                        "n_locked": ci.n_locked, "λs": lam[a:], "lambdas": lam, "residuals": res,
+                                        "residual_norms": res[a:], "precision": prec})
+    return types.SimpleNamespace(**{"iteration": ci.iteration, "λ": lam, "residuals": res, "n_locked": ci.n_locked,
+                                    "λ_cut": ci.lambda_cut, "λ_max": ci.lambda_max, "degree": ci.degree,
+                                    "n_matvec": ci.n_matvec, "precision": prec})
+
+
+class _Callback:
+    """The solver's ``callback`` installed on the operator's context for one solve
+    (``kfc_ctx_set_callback``): called once per iteration with the info record, its
+    return value ignored; an exception it raises stops the solve and propagates
+    unchanged, with no retry (src/lobpcg.jl:247-253, 279)."""
+
+    def __init__(self, H, user):
+        self.H, self.user, self.exc, self.fn = H, user, None, None
+
+    def __enter__(self):
+        if self.user is not None:
+            def trampoline(pinfo, _):
+                try:
+                    self.user(_iter_info(pinfo.contents))
+                    return 0
+                except BaseException as e:      # re-raised after the library returns
+                    self.exc = e
+                    return 1
+            self.fn = L.CALLBACK(trampoline)
+            L.check(L.lib().kfc_ctx_set_callback(self.H.ctx.handle, self.fn, None), self.H.ctx.handle)
+        return self
+
+    def __exit__(self, *a):
+        if self.fn is not None:
+            L.lib().kfc_ctx_set_callback(self.H.ctx.handle, L.CALLBACK(), None)
+            self.fn = None
+        return False
+
+    def check(self, rc):
+        if self.exc is not None:
+            raise self.exc
+        L.check(rc, self.H.ctx.handle)
+
+
 def _out(H, m):
     E = np.empty(m)
     V = np.empty(H.N * m * (2 if H.dtype == np.complex128 else 1))
@@ -152,23 +206,21 @@ def solve_stats(s, H, m, X0=None):
                 seed_kind = L.KFC_SEED["warm_guards"]   # the warm start carries the guard columns
             elif X0.shape != (H.N, m):
                 raise L.DimensionMismatch(f"LOBPCG: X0 has size {X0.shape}, expected ({H.N}, {m})")
-            if H.dtype == np.float64 and np.iscomplexobj(X0):
-                if np.any(np.imag(X0) != 0):
-                    raise L.ArgumentError("LOBPCG: the warm-start block X0 is complex but the operator is real; "
-                                          "pass a real block or complexify the operator")
-                X0 = np.real(X0)
-            x0 = pack_block(X0, H.dtype)
-        rc = lib.kfc_lobpcg(H.handle, int(m), int(s.guard), float(s.tol), int(s.maxiter),
-                            L.KFC_PRECOND[s.preconditioner], float(s.eta), seed_kind,
-                            x0.ctypes.data_as(L._D) if x0 is not None else None, int(s.rng_seed), _mode(s.mode),
-                            E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
+            x0 = pack_block(X0, H.dtype, "LOBPCG: the warm-start block X0")
+        with _Callback(H, s.callback) as cb:
+            rc = lib.kfc_lobpcg(H.handle, int(m), int(s.guard), float(s.tol), int(s.maxiter),
+                                L.KFC_PRECOND[s.preconditioner], float(s.eta), seed_kind,
+                                x0.ctypes.data_as(L._D) if x0 is not None else None, int(s.rng_seed), _mode(s.mode),
+                                E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
+        cb.check(rc)
     elif isinstance(s, ChebyshevFilter):
-        nb = min(m + s.extra, H.N)
-        rc = lib.kfc_chebyshev(H.handle, int(m), int(s.degree), int(s.extra), int(s.maxiter), int(s.rescale_every),
-                               0 if s.ortho == "svqb" else 1, math.nan if s.bound == "gershgorin" else float(s.bound),
-                               L.KFC_SEED[s.seed], float(s.tol), int(s.rng_seed), _mode(s.mode),
-                               E.ctypes.data_as(L._D), V.ctypes.data_as(L._D), C.byref(st))
-        del nb
+        with _Callback(H, s.callback) as cb:
+            rc = lib.kfc_chebyshev(H.handle, int(m), int(s.degree), int(s.extra), int(s.maxiter),
+                                   int(s.rescale_every), 0 if s.ortho == "svqb" else 1,
+                                   math.nan if s.bound == "gershgorin" else float(s.bound), L.KFC_SEED[s.seed],
+                                   float(s.tol), int(s.rng_seed), _mode(s.mode), E.ctypes.data_as(L._D),
+                                   V.ctypes.data_as(L._D), C.byref(st))
+        cb.check(rc)
     else:
         raise L.ArgumentError(f"no device method for solver {type(s).__name__}")
     L.check(rc, H.ctx.handle)
@@ -190,7 +242,7 @@ def residuals(H, E, V):
     """Fresh FP64 true residuals ||H v_j - E_j v_j|| (src/solvers.jl:65-69)."""
     m = len(E)
     E = np.ascontiguousarray(E, dtype=np.float64)
-    Vb = pack_block(V, H.dtype)
+    Vb = pack_block(V, H.dtype, "residuals: V")
     out = np.empty(m)
     L.check(L.lib().kfc_residuals(H.handle, m, E.ctypes.data_as(L._D), Vb.ctypes.data_as(L._D),
                                   out.ctypes.data_as(L._D)), H.ctx.handle)
@@ -206,6 +258,9 @@ def solve_batch(s, Hs, m, X0s=None, lockstep=False):
     Returns lists of (E, V) and the stats."""
     if not isinstance(s, LOBPCG):
         raise L.ArgumentError("solve_batch serves LOBPCG sweeps")
+    if s.callback is not None:
+        raise L.UnsupportedError("solve_batch runs many solves at once and calls no per-iteration callback; "
+                                 "solve the points one at a time with solve() to use one")
     n = len(Hs)
     ops = (C.c_void_p * n)(*[H.handle.value for H in Hs])
     E = np.empty(n * m)
@@ -225,7 +280,7 @@ def solve_batch(s, Hs, m, X0s=None, lockstep=False):
                 if X0.shape not in ((H.N, m), (H.N, w)):
                     raise L.DimensionMismatch(f"LOBPCG: X0 of instance {i} has size {X0.shape}, expected ({H.N}, {m})")
                 widths.add(X0.shape[1] > m)
-                b = pack_block(X0, H.dtype)
+                b = pack_block(X0, H.dtype, f"LOBPCG: the warm-start block X0 of instance {i}")
                 keep.append(b)
                 x0p[i] = b.ctypes.data_as(L._D)
         if len(widths) > 1:

@@ -146,6 +146,7 @@ def sandwich(V, dims, ops, ctx=None):
         raise L.ArgumentError("sandwich: operator and block types disagree")
     Vt = torch.from_numpy(np.ascontiguousarray(V.T.astype(np.complex128 if cplx else np.float64))).to("cuda")
     G = np.empty(2 * k * k if cplx else k * k)
+    H.ctx.wait_torch()
     L.check(L.lib().kfc_sandwich(H.handle, C.c_void_p(Vt.data_ptr()), int(k), int(N), G.ctypes.data_as(L._D)),
             H.ctx.handle)
     if cplx:
