@@ -839,11 +839,14 @@ int lanczos_t(kfc_op* op, int64_t m, int64_t krylovdim, double tol, int64_t maxi
     }
     if (rc != KFC_OK) return rc;
     const int n = int(out.theta.size());
-    if (n < m || out.converged < m)
+    if (n < m || out.converged < m) {
+        char nr[64];
+        snprintf(nr, sizeof nr, "%.3e", out.resid.empty() ? 0.0 : out.resid[std::min<int64_t>(m, n) - 1]);
         return kfc_fail(op->ctx, KFC_ENOCONV,
                         "Lanczos: only " + std::to_string(std::min(out.converged, int(m))) + " of " + std::to_string(m) +
-                            " eigenpairs converged (normres = " +
-                            std::to_string(out.resid.empty() ? 0.0 : out.resid[std::min<int64_t>(m, n) - 1]) + ")");
+                            " eigenpairs converged (normres = " + nr + ", after " + std::to_string(out.restarts) +
+                            " restarts)");
+    }
     // the FP64 (or complex FP64) result block has the caller's host layout: one copy
     KFC_HIP(op->ctx, hipMemcpyAsync(Vout, res64.p, sizeof(S) * size_t(N) * m, hipMemcpyDeviceToHost, op->ctx->stream));
     KFC_HIP(op->ctx, hipStreamSynchronize(op->ctx->stream));

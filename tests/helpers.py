@@ -31,7 +31,8 @@ def random_ops(rng, dims, nterms, cplx, kinds=("dense", "diag", "banded", "shift
     return terms
 
 
-def to_oracle(dims, terms, constant=0.0):
+def oracle_terms(dims, terms):
+    """The oracle's AxisTerms of a random_ops term list."""
     ts = []
     for c, fs in terms:
         ops = []
@@ -45,7 +46,11 @@ def to_oracle(dims, terms, constant=0.0):
             else:
                 ops.append((a, O.ShiftOp(dims[a], data[0], data[1])))
         ts.append(O.AxisTerm(c, ops))
-    return O.kron_operator(dims, ts, constant)
+    return ts
+
+
+def to_oracle(dims, terms, constant=0.0):
+    return O.kron_operator(dims, oracle_terms(dims, terms), constant)
 
 
 def to_kfc(dims, terms, constant=0.0, block_capacity=1):

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle import kfc_oracle as O
-from tests.helpers import random_ops, to_kfc, to_oracle
+from tests.helpers import oracle_terms, random_ops, to_kfc, to_oracle
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -173,7 +173,9 @@ def test_matrix_equals_kron_reference():
     H = to_kfc(dims, terms, 1.5)
     Ho = to_oracle(dims, terms, 1.5)
     M = _run(H, np.eye(H.N, dtype=complex))
-    assert np.allclose(M, O.kron_reference(dims, Ho.terms + Ho.diagterms, 1.5), atol=1e-12)
+    # the constant is one of the fused terms (a term with no factors)
+    assert np.allclose(M, O.kron_reference(dims, Ho.terms + Ho.diagterms), atol=1e-12)
+    assert np.allclose(M, O.kron_reference(dims, oracle_terms(dims, terms), 1.5), atol=1e-12)
 
 
 def test_derived_quantities():
