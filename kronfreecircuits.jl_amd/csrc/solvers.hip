@@ -397,7 +397,7 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
     int k = 0;
     std::vector<cd> T, b;   // k x k and k
     int64_t restarts = 0;
-    int s_limit = std::max(1, std::min(s_max, LZ_SMAX)), s_cap = s_limit, anchors = 0;
+    int s_limit = std::max(1, std::min(s_max, LZ_SMAX)), s_cap = s_limit, anchors = 0, fresh_starts = 0;
     // H Q = Q T + q b^T holds to rounding (false after an explicit check found the
     // basis polluted: then every first projection is computed from Q itself)
     bool relation_exact = true;
@@ -708,6 +708,33 @@ int lanczos_core(kfc_op* op, Work& w, int m, int kd, double tol, int64_t maxiter
                 fprintf(stderr, "[kfc lanczos] explicit check %d: converged %d of %d, resid[m-1] %.3e, true max %.3e\n",
                         anchors, conv, m, resid[q - 1], *std::max_element(tr.begin(), tr.end()));
             if (conv >= m || (n >= kd && restarts >= maxiter) || anchors >= 16) return KFC_OK;
+            if (polluted && fresh_starts < 3) {
+                // H V has components outside [V q]: the Krylov relation a thick restart
+                // relies on no longer holds, and restarts from it keep that error.  Start
+                // afresh from one vector, a combination of the wanted Ritz vectors (all of
+                // them present in it), with one-vector blocks (plain Lanczos with full
+                // reorthogonalization), so the relation is exact again from its first step
+                ++fresh_starts;
+                std::vector<cd> comb(q);
+                std::mt19937_64 g(seed ^ (0x9e3779b97f4a7c15ull * uint64_t(fresh_starts)));
+                std::normal_distribution<double> nd;
+                for (int i = 0; i < q; ++i) comb[i] = 1.0 + 0.25 * nd(g);
+                rc = blas_gemm<S>(w, result.as<S>(), N, q, comb, 1, V, N, N, 0.0);
+                std::vector<double> nrm;
+                if (rc == KFC_OK) rc = blas_colnorms<S>(w, V, N, 1, N, nrm);
+                if (rc != KFC_OK) return rc;
+                if (!(nrm[0] > 0)) return kfc_fail(op->ctx, KFC_EBREAKDOWN, "Lanczos: zero restart vector");
+                rc = blas_scale_cols<S>(w, V, N, 1, N, std::vector<double>{1.0 / nrm[0]});
+                if (rc != KFC_OK) return rc;
+                k = 0;
+                T.clear();
+                b.clear();
+                relation_exact = true;
+                s_limit = 1;
+                s_cap = 1;
+                ++restarts;
+                continue;
+            }
             // not there yet: continue from the exact decomposition with shorter blocks
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i) Tn[i + size_t(j) * n] = 0.5 * (Tx[i + size_t(j) * n] + std::conj(Tx[j + size_t(i) * n]));
