@@ -184,6 +184,19 @@ int kfc_op_work(const kfc_op* op, double* bytes, double* flops);
  * after the launch (kfc_sync waits). */
 int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision);
 
+/* No counterpart in the reference: kfc_apply for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and result as kfc_apply; it calls no hipMalloc, hipFree or
+ * allocating routine.  The chain path's two intermediate blocks come from a static
+ * device array of 64 MiB that every GPU the library runs on holds until the process
+ * ends: 2 x prod(dims) x ncols x element size (plus 256 bytes) must fit, else KFC_EARG;
+ * the fused-stencil and dense-pair paths need no scratch.  The operator's tables for the
+ * precision must be on the device (kfc_op_create places the FP64 ones, the first FP32
+ * kfc_apply the FP32 ones), the stencil keeps the launch shape the tuner last chose, and
+ * complex blocks on a real operator (KFC_PREC_Z64) are refused (KFC_EUNSUPPORTED).  All
+ * calls on one GPU share the array, so calls that overlap on one GPU race. */
+int kfc_apply_noalloc(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision);
+
 /* One Chebyshev step of a real operator with its two inner products:
  * C = a H B + b B + g A (A may be null) on one column, and host dots[0] = B.B,
  * dots[1] = B.C (FP64 sums).  The recurrence is _chebyshev_step!

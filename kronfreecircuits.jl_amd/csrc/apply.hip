@@ -1114,19 +1114,30 @@ static unsigned grid_for(int64_t total) {
     return unsigned(std::max<int64_t>(1, std::min<int64_t>(g, 65536)));
 }
 
+// Whether the chain path chains factors through intermediate blocks (a kept term with
+// two or more factors): then it needs two scratch blocks of N x ncols scalars.
+static bool chain_needs_scratch(const kfc_op* op) {
+    for (const Term& t : op->terms)
+        if (t.factors.size() > 1) return true;
+    return false;
+}
+
+// The chain path; its two intermediate blocks are the operator's own (grown here) or,
+// when buf1/buf2 are given, the caller's (N x ncols scalars each).
 template <class S>
-static int launch_chain(kfc_op* op, const void* Xv, void* Yv, int64_t ncols, int64_t ld, hipStream_t st) {
+static int launch_chain(kfc_op* op, const void* Xv, void* Yv, int64_t ncols, int64_t ld, hipStream_t st,
+                        void* buf1 = nullptr, void* buf2 = nullptr) {
     DeviceData& dd = dd_for<S>(op);
     const int64_t N = op->N, total = N * ncols;
     const S* X = static_cast<const S*>(Xv);
     S* Y = static_cast<S*>(Yv);
     hipLaunchKernelGGL(k_diag_init<S>, dim3(grid_for(total)), dim3(256), 0, st, dd.diag.as<S>(), X, Y, N, ld, total);
     size_t need = size_t(total) * sizeof(S);
-    bool multi = false;
-    for (const Term& t : op->terms) multi |= t.factors.size() > 1;
-    if (multi) {
+    if (chain_needs_scratch(op) && !buf1) {
         KFC_HIP(op->ctx, dd.buf1.ensure(need));
         KFC_HIP(op->ctx, dd.buf2.ensure(need));
+        buf1 = dd.buf1.p;
+        buf2 = dd.buf2.p;
     }
     std::vector<int64_t> stride(op->ndims, 1);
     for (int a = 1; a < op->ndims; ++a) stride[a] = stride[a - 1] * op->dims[a - 1];
@@ -1138,7 +1149,7 @@ static int launch_chain(kfc_op* op, const void* Xv, void* Yv, int64_t ncols, int
             const Factor& f = t.factors[k];
             const DeviceData::FDev& fd = dd.fdev[ti][k];
             const bool last = k + 1 == t.factors.size();
-            S* dst = last ? Y : (k % 2 == 0 ? dd.buf1.as<S>() : dd.buf2.as<S>());
+            S* dst = last ? Y : static_cast<S*>(k % 2 == 0 ? buf1 : buf2);
             const int64_t ldd = last ? ld : N;
             hipLaunchKernelGGL(k_factor<S>, dim3(grid_for(total)), dim3(256), 0, st, src, lds, dst, ldd, N, total,
                                stride[f.axis], f.d, f.kind, int(f.offsets.size()), fd.offs,
@@ -1250,6 +1261,47 @@ static int apply_t(kfc_op* op, const void* X, const void* A, void* Y, int64_t nc
                            s_from<S>(beta), s_from<S>(gamma));
         KFC_HIP(op->ctx, hipGetLastError());
     }
+    return KFC_OK;
+}
+
+// Y = H X with no device allocation and no tuning: the tables of precision prec must be
+// on the device already, the fused stencil keeps its current launch shape, and the chain
+// path's intermediate blocks are buf1, buf2 (each at least N x ncols scalars; unused by
+// the fused and dense-pair paths).
+template <class S>
+static int apply_fixed(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, void* buf1, void* buf2,
+                       hipStream_t st) {
+    if (op->plan.ok) return launch_stencil<S>(op, X, nullptr, Y, ncols, ld, cd(1), cd(0), cd(0), st);
+    if constexpr (!IsCx<S>::value)
+        if (op->pairs.ok) return launch_pairs<S>(op, X, Y, ncols, ld, st);
+    return launch_chain<S>(op, X, Y, ncols, ld, st, buf1, buf2);
+}
+
+size_t op_chain_scratch_bytes(const kfc_op* op, int64_t ncols, int prec) {
+    if (op->plan.ok || (!op->cplx && op->pairs.ok) || !chain_needs_scratch(op)) return 0;
+    const size_t es = (op->cplx ? 2 : 1) * (prec == KFC_PREC_F64 ? 8 : 4);
+    return 2 * size_t(op->N) * size_t(ncols) * es;
+}
+
+int op_apply_fixed(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, void* scratch,
+                   hipStream_t st) {
+    if (ncols == 0 || op->N == 0) return KFC_OK;
+    if (!op->dd[prec].built)
+        return kfc_fail(op->ctx, KFC_EARG, std::string("the operator's ") + (prec == KFC_PREC_F64 ? "FP64" : "FP32") +
+                                               " tables are not on the device yet (kfc_op_create places the FP64 "
+                                               "ones, the first FP32 kfc_apply the FP32 ones)");
+    const size_t half = op_chain_scratch_bytes(op, ncols, prec) / 2;
+    void* b1 = scratch;
+    void* b2 = half ? static_cast<char*>(scratch) + ((half + 255) & ~size_t(255)) : nullptr;
+    int rc;
+    if (prec == KFC_PREC_F64)
+        rc = op->cplx ? apply_fixed<Cx<double>>(op, X, Y, ncols, ld, b1, b2, st)
+                      : apply_fixed<double>(op, X, Y, ncols, ld, b1, b2, st);
+    else
+        rc = op->cplx ? apply_fixed<Cx<float>>(op, X, Y, ncols, ld, b1, b2, st)
+                      : apply_fixed<float>(op, X, Y, ncols, ld, b1, b2, st);
+    if (rc != KFC_OK) return rc;
+    KFC_HIP(op->ctx, hipGetLastError());
     return KFC_OK;
 }
 

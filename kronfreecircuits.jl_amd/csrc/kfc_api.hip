@@ -40,6 +40,14 @@ int kfc_ctx_create(kfc_ctx** out, int device) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    // the per-device address of kfc_apply_noalloc's scratch array, taken with the device current
+    const int rs = noalloc_setup(c);
+    if (rs != KFC_OK) {
+        g_noctx_err = c->err;
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rs;
+    }
     *out = c;
     return KFC_OK;
 }
@@ -164,6 +172,10 @@ int kfc_op_create(kfc_ctx* ctx, const kfc_op_desc* desc, kfc_op** out) {
     }
     int rc = op_build_plan(op.get());
     if (rc == KFC_OK) rc = op_build_pairs(op.get());
+    // the FP64 tables go to the device with the operator, as the reference adapts its
+    // data to `on` at construction (src/apply.jl:248-250); the FP32 ones follow on the
+    // first FP32 apply
+    if (rc == KFC_OK) rc = op_ensure_device(op.get(), KFC_PREC_F64);
     if (rc != KFC_OK) return rc;
     *out = op.release();
     return KFC_OK;
@@ -223,26 +235,38 @@ int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march) {
     return KFC_OK;
 }
 
-int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision) {
-    if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_apply: null operator");
+}  // extern "C"
+
+int check_apply_args(kfc_op* op, const void* X, const void* Y, int64_t ncols, int64_t ld, int32_t& precision,
+                     bool& real_on_complex, const char* who) {
     if (precision != KFC_PREC_F64 && precision != KFC_PREC_F32 && precision != KFC_PREC_Z64)
-        return kfc_fail(op->ctx, KFC_EARG, "kfc_apply: unknown precision");
+        return kfc_fail(op->ctx, KFC_EARG, std::string(who) + ": unknown precision");
     // complex FP64 blocks: a complex operator's own FP64 apply, or a real one's
     // _mul_real_operator! (src/apply.jl:397-407)
-    const bool real_on_complex = precision == KFC_PREC_Z64 && !op->cplx;
+    real_on_complex = precision == KFC_PREC_Z64 && !op->cplx;
     if (precision == KFC_PREC_Z64 && op->cplx) precision = KFC_PREC_F64;
-    if (ncols < 0) return kfc_fail(op->ctx, KFC_EARG, "kfc_apply: negative column count");
+    if (ncols < 0) return kfc_fail(op->ctx, KFC_EARG, std::string(who) + ": negative column count");
     if (ld < op->N)
         return kfc_fail(op->ctx, KFC_EDIM, "KronFreeOperator of size (" + std::to_string(op->N) + ", " +
                                                std::to_string(op->N) + ") applied to x with leading dimension " +
                                                std::to_string(ld));
     if (ncols == 0) return KFC_OK;
-    if (!X || !Y) return kfc_fail(op->ctx, KFC_EARG, "kfc_apply: null buffer");
+    if (!X || !Y) return kfc_fail(op->ctx, KFC_EARG, std::string(who) + ": null buffer");
     const size_t es = (op->cplx || real_on_complex ? 2 : 1) * (precision == KFC_PREC_F32 ? 4 : 8);
     const char* xb = static_cast<const char*>(X);
     const char* yb = static_cast<const char*>(Y);
     const size_t span = size_t(((ncols - 1) * ld + op->N)) * es;
     if (xb < yb + span && yb < xb + span) return kfc_fail(op->ctx, KFC_EARG, "mul!: y must not alias x");
+    return KFC_OK;
+}
+
+extern "C" {
+
+int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int32_t precision) {
+    if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_apply: null operator");
+    bool real_on_complex = false;
+    int rc = check_apply_args(op, X, Y, ncols, ld, precision, real_on_complex, "kfc_apply");
+    if (rc != KFC_OK || ncols == 0) return rc;
     if (real_on_complex) return op_apply_real_on_complex(op, X, Y, ncols, ld);
     return op_apply(op, X, Y, ncols, ld, precision, op->ctx->stream);
 }

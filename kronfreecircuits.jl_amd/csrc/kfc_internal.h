@@ -279,6 +279,20 @@ int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int 
 // path (op->plan.ok), the stencil running on complex tables (op->ddz).
 int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
                       hipStream_t st);
+// The argument checks of kfc_apply (precision, sizes, aliasing); a KFC_PREC_Z64 precision
+// on a complex operator becomes KFC_PREC_F64, on a real one sets real_on_complex.
+int check_apply_args(kfc_op* op, const void* X, const void* Y, int64_t ncols, int64_t ld, int32_t& precision,
+                     bool& real_on_complex, const char* who);
+// Bytes of chain-path scratch one apply of ncols columns at precision prec needs (0 on the
+// fused-stencil and dense-pair paths, and when no kept term has two factors).
+size_t op_chain_scratch_bytes(const kfc_op* op, int64_t ncols, int prec);
+// Y = H X with no device allocation and no tuning: the tables of precision prec must be on
+// the device already, and the chain path's two intermediate blocks are taken from scratch
+// (op_chain_scratch_bytes, the second block at the next 256-byte boundary after the first).
+int op_apply_fixed(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, void* scratch,
+                   hipStream_t st);
+// Takes the current device's address of kfc_apply_noalloc's scratch array (noalloc.hip).
+int noalloc_setup(kfc_ctx* ctx);
 // Y = H X for interleaved complex128 blocks X, Y (leading dimension ld) and a real H
 // (evolve.hip): the reference's _mul_real_operator!.
 int op_apply_real_on_complex(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld);

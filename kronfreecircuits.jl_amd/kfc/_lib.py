@@ -109,6 +109,7 @@ SIGNATURES = {
     "kfc_op_stencil_set": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "kfc_op_work": (C.c_int, [_P, _D, _D]),
     "kfc_apply": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_int32]),
+    "kfc_apply_noalloc": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_int32]),
     "kfc_apply_cheb_dots": (C.c_int, [_P, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_int32, _D,
                                       C.POINTER(C.c_int32)]),
     "kfc_sync": (C.c_int, [_P]),
