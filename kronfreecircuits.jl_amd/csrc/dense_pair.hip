@@ -12,6 +12,11 @@
 // Terms sharing one factor are merged on the host first
 // (c1 A(x)B1 + c2 A(x)B2 = A (x) (c1 B1 + c2 B2)).
 #include <algorithm>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <cstdlib>
+#include <string>
 
 #include "kfc_internal.h"
 
@@ -360,6 +365,174 @@ __device__ inline void dense_pair_slabs_reg(const PairArgs<T>& a, int64_t s0, in
     }
 }
 
+// Register-chained variant for groups of at most NT terms (P <= 48).  A block has
+// P / 16 waves and wave w owns output tile column w of every slab:
+//   T_t[:, w] = X_s (c_t B_t)^T[:, w]      (GEMM1: X_s from LDS, the wave's 16 rows of
+//                                           c_t B_t in registers, loaded once)
+//   Y_s[:, w] += A_t T_t[:, w]              (GEMM2: A_t from LDS)
+// GEMM1's accumulator tiles are GEMM2's B operands as they stand: lane (r, k) of a
+// 16 x 16 tile holds rows out_row(k, i), i < 4, of column r, which is the B-operand
+// layout of a 16 x 16 x 4 step whose contraction index for that lane is out_row(k, i);
+// GEMM2's A operand is read with the same index.  So the intermediate never leaves
+// registers, and the waves of a block meet only to stage a slab and to write the
+// result back in the slab's memory order.  The contraction loops stop at the real
+// extents (ceil(d / 4) steps); only the output tiles are padded to 16.
+template <class T, int P, int NT>
+__device__ inline void dense_pair_slabs_chain(const PairArgs<T>& a, int64_t s0, int64_t step) {
+    using v4 = typename V4<T>::type;
+    constexpr int NW = P / 16, NTH = 64 * NW, KS = P / 4;
+    constexpr int NPF = (P * P + NTH - 1) / NTH;
+    __shared__ T Xs[P * P];       // the slab, column-major (leading dimension P), zero outside dp x dq
+    __shared__ T Rs[P * P];       // the result tiles, same layout
+    __shared__ T As[NT * P * P];  // A_t, column-major (leading dimension P), zero outside dp x dp
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int tm = a.Mp / 16, tn = a.Nq / 16, ksq = (a.dq + 3) / 4;
+    const int nterm = a.nterm;
+    // A_t into LDS: every load issued before the stores
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        T av[NPF];
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH, i = e % P, k = e / P;
+            av[u] = (e < P * P && t < nterm && i < a.dp && k < a.dp) ? a.A[t][i + (int64_t)k * a.dp] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH;
+            if (e < P * P) As[t * P * P + e] = av[u];
+        }
+    }
+    // the wave's rows of c_t B_t: Bf[t][ks] = c_t B_t[w * 16 + r, 4 ks + k]
+    T Bf[NT][KS];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + k4, j = w * 16 + r16;
+            Bf[t][ks] = (t < nterm && j < a.dq && k < a.dq) ? a.c[t] * a.B[t][j + (int64_t)k * a.dq] : T(0);
+        }
+    const int64_t perc = a.L * a.M * a.R;
+    auto base_of = [&](int64_t s, int64_t& col) {
+        col = s / perc;
+        int64_t rem = s - col * perc;
+        const int64_t l = rem % a.L;
+        rem /= a.L;
+        const int64_t m = rem % a.M;
+        const int64_t r = rem / a.M;
+        return col * a.ld + l + m * a.L * a.dp + r * a.sr;
+    };
+    T pf[NPF];
+    auto fetch = [&](int64_t s) {
+        int64_t col;
+        const int64_t base = base_of(s, col);
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH, i = e % P, j = e / P;
+            pf[u] = (e < P * P && i < a.dp && j < a.dq) ? a.X[base + i * a.sp + j * a.sq] : T(0);
+        }
+    };
+    if (s0 < a.nslab) fetch(s0);
+    for (int64_t s = s0; s < a.nslab; s += step) {
+        int64_t col;
+        const int64_t base = base_of(s, col);
+        __syncthreads();   // the previous slab's readers of Xs and Rs are done (and As is staged)
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH;
+            if (e < P * P) Xs[e] = pf[u];
+        }
+        __syncthreads();
+        if (s + step < a.nslab) fetch(s + step);
+        if (w < tn) {
+            v4 acc[NW];
+#pragma unroll
+            for (int tp = 0; tp < NW; ++tp) acc[tp] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                if (t < nterm) {   // uniform across the block
+                    v4 Tt[NW];
+#pragma unroll
+                    for (int ti = 0; ti < NW; ++ti) Tt[ti] = v4{0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+                        if (ks < ksq)
+#pragma unroll
+                            for (int ti = 0; ti < NW; ++ti)
+                                if (ti < tm) Tt[ti] = mfma(Xs[(ti * 16 + r16) + (4 * ks + k4) * P], Bf[t][ks], Tt[ti]);
+                    const T* At = As + t * P * P;
+#pragma unroll
+                    for (int ti = 0; ti < NW; ++ti)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            // the step's lowest contraction index; rows of T at or beyond dp are zero
+                            if (ti < tm && ti * 16 + out_row(T(0), 0, i) < a.dp)
+#pragma unroll
+                                for (int tp = 0; tp < NW; ++tp)
+                                    if (tp < tm)
+                                        acc[tp] = mfma(At[(tp * 16 + r16) + (ti * 16 + out_row(T(0), k4, i)) * P],
+                                                       Tt[ti][i], acc[tp]);
+                        }
+                }
+            }
+#pragma unroll
+            for (int tp = 0; tp < NW; ++tp)
+                if (tp < tm)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) Rs[(tp * 16 + out_row(T(0), k4, i)) + (w * 16 + r16) * P] = acc[tp][i];
+        }
+        __syncthreads();
+        // the epilogue's loads (D, or Y to accumulate into) all issued before any store, so
+        // the slab pays one memory round trip, not one per element
+        T yv[NPF];
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH, i = e % P, j = e / P;
+            const int64_t idx = base + i * a.sp + j * a.sq;
+            yv[u] = (e < P * P && i < a.dp && j < a.dq) ? (a.D ? a.D[idx - col * a.ld] : a.Y[idx]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int e = tid + u * NTH, i = e % P, j = e / P;
+            if (e < P * P && i < a.dp && j < a.dq) {
+                const int64_t idx = base + i * a.sp + j * a.sq;
+                a.Y[idx] = (a.D ? yv[u] * Xs[e] : yv[u]) + Rs[e];
+            }
+        }
+    }
+}
+
+template <class T, int P, int NT>
+__global__ __launch_bounds__(64 * (P / 16)) void k_dense_pair_chain(const PairArgs<T> a) {
+    dense_pair_slabs_chain<T, P, NT>(a, blockIdx.x, gridDim.x);
+}
+
+template <class T, int P, int NT>
+__global__ __launch_bounds__(64 * (P / 16)) void k_dense_pair_chain_batch(const PairArgs<T>* __restrict__ tab) {
+    const PairArgs<T> a = tab[blockIdx.y];
+    dense_pair_slabs_chain<T, P, NT>(a, blockIdx.x, gridDim.x);
+}
+
+// Blocks of kernel K that fit on the device at once (occupancy x compute units): a
+// chain-kernel grid is capped there, so that every block works through many slabs and
+// its setup (A_t into LDS, the B_t fragments) is paid once per block, not per slab.
+std::mutex g_resident_mu;
+std::map<const void*, int> g_resident;   // kernel -> resident blocks per compute unit
+
+template <class K>
+int64_t resident_blocks(K kernel, int threads, int cus) {
+    const void* key = reinterpret_cast<const void*>(kernel);
+    std::lock_guard<std::mutex> g(g_resident_mu);
+    auto it = g_resident.find(key);
+    if (it == g_resident.end()) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, threads, 0) != hipSuccess || b < 1) b = 1;
+        it = g_resident.emplace(key, b).first;
+    }
+    return int64_t(it->second) * cus;
+}
+
 template <class T, int P, int NT>
 __global__ __launch_bounds__(64 * (P / 16)) void k_dense_pair_reg(const PairArgs<T> a) {
     dense_pair_slabs_reg<T, P, NT>(a, blockIdx.x, gridDim.x);
@@ -540,6 +713,145 @@ bool pairs_same_structure(const kfc_op* a, const kfc_op* b) {
     return true;
 }
 
+// Launch shapes of the register-resident kernels (groups of at most two terms, extents
+// up to 48): the kernel family (0: register-chained, 1: register fragments with the
+// intermediate through LDS) and the grid (0: one block per slab, at most 4096 blocks in
+// all; k > 0: k times the blocks that fit on the device at once, each looping over
+// slabs).  Which is fastest depends on the work in flight (a single operator's columns
+// or a whole lockstep sweep), so the launcher times them on the first application of
+// each shape and keeps the winner.
+struct PairShape {
+    int family, grid;
+};
+constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2}};
+constexpr int kNumPairShapes = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
+
+template <class T, int P, int NT>
+int launch_reg_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus,
+                     hipStream_t st) {
+    constexpr int threads = 64 * (P / 16);
+    auto grid_for = [&](auto kernel) -> unsigned {
+        if (sh.grid == 0) return unsigned(std::max<int64_t>(1, std::min<int64_t>(a.nslab, std::max(1, 4096 / n))));
+        const int64_t cap = resident_blocks(kernel, threads, cus) * sh.grid;
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>(a.nslab, (cap + n - 1) / n)));
+    };
+    if (tab) {
+        if (sh.family == 0) {
+            auto k = k_dense_pair_chain_batch<T, P, NT>;
+            hipLaunchKernelGGL(k, dim3(grid_for(k), unsigned(n)), dim3(threads), 0, st, tab);
+        } else {
+            auto k = k_dense_pair_reg_batch<T, P, NT>;
+            hipLaunchKernelGGL(k, dim3(grid_for(k), unsigned(n)), dim3(threads), 0, st, tab);
+        }
+    } else {
+        if (sh.family == 0) {
+            auto k = k_dense_pair_chain<T, P, NT>;
+            hipLaunchKernelGGL(k, dim3(grid_for(k)), dim3(threads), 0, st, a);
+        } else {
+            auto k = k_dense_pair_reg<T, P, NT>;
+            hipLaunchKernelGGL(k, dim3(grid_for(k)), dim3(threads), 0, st, a);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
+}
+
+// One group on n problems (tab: their device records, or null with n = 1 and a passed
+// by value).  Groups of at most two terms with extents up to 48 take shape sh.
+template <class T>
+int launch_group(const PairShape& sh, const PairArgs<T>& a, int ntmax, const PairArgs<T>* tab, int n, int cus,
+                 hipStream_t st) {
+    const int Pm = std::max(a.Mp, a.Nq);
+    if (Pm <= 48 && ntmax <= 2) {
+        if (Pm <= 16) return ntmax <= 1 ? launch_reg_group<T, 16, 1>(sh, a, tab, n, cus, st)
+                                        : launch_reg_group<T, 16, 2>(sh, a, tab, n, cus, st);
+        if (Pm <= 32) return ntmax <= 1 ? launch_reg_group<T, 32, 1>(sh, a, tab, n, cus, st)
+                                        : launch_reg_group<T, 32, 2>(sh, a, tab, n, cus, st);
+        return ntmax <= 1 ? launch_reg_group<T, 48, 1>(sh, a, tab, n, cus, st)
+                          : launch_reg_group<T, 48, 2>(sh, a, tab, n, cus, st);
+    }
+    const dim3 grid(unsigned(std::max<int64_t>(1, std::min<int64_t>(a.nslab, std::max(1, 4096 / n)))), unsigned(n));
+    if (tab) {
+        if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair_batch<T, 16>), grid, dim3(256), 0, st, tab);
+        else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair_batch<T, 32>), grid, dim3(256), 0, st, tab);
+        else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair_batch<T, 48>), grid, dim3(256), 0, st, tab);
+        else hipLaunchKernelGGL((k_dense_pair_batch<T, 64>), grid, dim3(256), 0, st, tab);
+    } else {
+        if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair<T, 16>), dim3(grid.x), dim3(256), 0, st, a);
+        else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair<T, 32>), dim3(grid.x), dim3(256), 0, st, a);
+        else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair<T, 48>), dim3(grid.x), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_dense_pair<T, 64>), dim3(grid.x), dim3(256), 0, st, a);
+    }
+    return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
+}
+
+// The tuner's cache: (dims, each group's axes, extents and term count, scalar size,
+// total slabs rounded up to a power of two) -> index into kPairShapes.
+struct PairTuneKey {
+    std::vector<int64_t> sig;
+    bool operator<(const PairTuneKey& o) const { return sig < o.sig; }
+};
+std::mutex g_pair_tune_mu;
+std::map<PairTuneKey, int> g_pair_tune;
+
+PairTuneKey pair_tune_key(const kfc_op* op, size_t scalar, int64_t slabs_total) {
+    PairTuneKey k;
+    k.sig = op->dims;
+    for (const PairGroup& g : op->pairs.groups) {
+        k.sig.push_back(g.p);
+        k.sig.push_back(g.q);
+        k.sig.push_back(g.dp);
+        k.sig.push_back(g.dq);
+        k.sig.push_back(int64_t(g.A.size()));
+    }
+    int64_t b = 1;
+    while (b < slabs_total) b <<= 1;
+    k.sig.push_back(int64_t(scalar));
+    k.sig.push_back(b);
+    return k;
+}
+
+// Applies every group with the tuned shape: on the first application of a key each
+// candidate runs the whole apply (idempotent: the first group writes Y, the rest add to
+// it) once to warm up and once timed with HIP events on the stream.
+int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, hipStream_t st, const std::function<int(const PairShape&)>& all) {
+    int pick = -1;
+    {
+        std::lock_guard<std::mutex> g(g_pair_tune_mu);
+        auto it = g_pair_tune.find(key);
+        if (it != g_pair_tune.end()) pick = it->second;
+    }
+    if (pick >= 0) return all(kPairShapes[pick]);
+    hipEvent_t e0, e1;
+    KFC_HIP(ctx, hipEventCreate(&e0));
+    KFC_HIP(ctx, hipEventCreate(&e1));
+    float best = 1e30f;
+    pick = 0;
+    int rc = KFC_OK;
+    for (int c = 0; c < kNumPairShapes && rc == KFC_OK; ++c) {
+        rc = all(kPairShapes[c]);
+        if (rc != KFC_OK) break;
+        KFC_HIP(ctx, hipEventRecord(e0, st));
+        rc = all(kPairShapes[c]);
+        if (rc != KFC_OK) break;
+        KFC_HIP(ctx, hipEventRecord(e1, st));
+        KFC_HIP(ctx, hipEventSynchronize(e1));
+        float ms = 0;
+        KFC_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != KFC_OK) return kfc_fail(ctx, rc, "dense pair launch failed");
+    {
+        std::lock_guard<std::mutex> g(g_pair_tune_mu);
+        g_pair_tune[key] = pick;
+    }
+    return all(kPairShapes[pick]);   // Y from the chosen shape
+}
+
 template <class T>
 int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
                        hipStream_t st, void* host_tab, void* dev_tab) {
@@ -551,32 +863,21 @@ int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* co
     for (size_t gi = 0; gi < ng; ++gi)
         for (int q = 0; q < n; ++q) h[gi * n + q] = pair_args<T>(ops[q], gi, X[q], Y[q], ncols, ld);
     KFC_HIP(op0->ctx, hipMemcpyAsync(d, h, sizeof(PairArgs<T>) * ng * n, hipMemcpyHostToDevice, st));
+    std::vector<int> ntmax(ng, 0);
+    int64_t slabs = 0;
     for (size_t gi = 0; gi < ng; ++gi) {
-        const PairArgs<T>& a = h[gi * n];
-        const int Pm = std::max(a.Mp, a.Nq);
-        int ntmax = 0;
-        for (int q = 0; q < n; ++q) ntmax = std::max(ntmax, h[gi * n + q].nterm);
-        const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>(a.nslab, std::max(1, 4096 / n))));
-        const dim3 grid(gx, unsigned(n));
-        const PairArgs<T>* tab = d + gi * n;
-        if (Pm <= 48 && ntmax <= 2) {
-            if (Pm <= 16) {
-                if (ntmax <= 1) hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 16, 1>), grid, dim3(64), 0, st, tab);
-                else hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 16, 2>), grid, dim3(64), 0, st, tab);
-            } else if (Pm <= 32) {
-                if (ntmax <= 1) hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 32, 1>), grid, dim3(128), 0, st, tab);
-                else hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 32, 2>), grid, dim3(128), 0, st, tab);
-            } else {
-                if (ntmax <= 1) hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 48, 1>), grid, dim3(192), 0, st, tab);
-                else hipLaunchKernelGGL((k_dense_pair_reg_batch<T, 48, 2>), grid, dim3(192), 0, st, tab);
-            }
-        } else if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair_batch<T, 16>), grid, dim3(256), 0, st, tab);
-        else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair_batch<T, 32>), grid, dim3(256), 0, st, tab);
-        else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair_batch<T, 48>), grid, dim3(256), 0, st, tab);
-        else hipLaunchKernelGGL((k_dense_pair_batch<T, 64>), grid, dim3(256), 0, st, tab);
-        KFC_HIP(op0->ctx, hipGetLastError());
+        for (int q = 0; q < n; ++q) ntmax[gi] = std::max(ntmax[gi], h[gi * n + q].nterm);
+        slabs += h[gi * n].nslab * n;
     }
-    return KFC_OK;
+    const int cus = op0->ctx->num_cus;
+    auto all = [&](const PairShape& sh) {
+        for (size_t gi = 0; gi < ng; ++gi) {
+            const int rc = launch_group<T>(sh, h[gi * n], ntmax[gi], d + gi * n, n, cus, st);
+            if (rc != KFC_OK) return rc;
+        }
+        return int(KFC_OK);
+    };
+    return run_pair_groups(op0->ctx, pair_tune_key(op0, sizeof(T), slabs), st, all);
 }
 
 template int launch_pairs_batch<double>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t,
@@ -588,29 +889,23 @@ size_t pair_args_bytes(int prec) { return prec == 0 ? sizeof(PairArgs<double>) :
 
 template <class T>
 int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st) {
-    for (size_t gi = 0; gi < op->pairs.groups.size(); ++gi) {
-        const PairArgs<T> a = pair_args<T>(op, gi, X, Y, ncols, ld);
-        const int Pm = std::max(a.Mp, a.Nq);
-        const unsigned grid = unsigned(std::min<int64_t>(a.nslab, 4096));
-        if (Pm <= 48 && a.nterm <= 2) {
-            if (Pm <= 16) {
-                if (a.nterm <= 1) hipLaunchKernelGGL((k_dense_pair_reg<T, 16, 1>), dim3(grid), dim3(64), 0, st, a);
-                else hipLaunchKernelGGL((k_dense_pair_reg<T, 16, 2>), dim3(grid), dim3(64), 0, st, a);
-            } else if (Pm <= 32) {
-                if (a.nterm <= 1) hipLaunchKernelGGL((k_dense_pair_reg<T, 32, 1>), dim3(grid), dim3(128), 0, st, a);
-                else hipLaunchKernelGGL((k_dense_pair_reg<T, 32, 2>), dim3(grid), dim3(128), 0, st, a);
-            } else {
-                if (a.nterm <= 1) hipLaunchKernelGGL((k_dense_pair_reg<T, 48, 1>), dim3(grid), dim3(192), 0, st, a);
-                else hipLaunchKernelGGL((k_dense_pair_reg<T, 48, 2>), dim3(grid), dim3(192), 0, st, a);
-            }
-        } else if (Pm <= 16) hipLaunchKernelGGL((k_dense_pair<T, 16>), dim3(grid), dim3(256), 0, st, a);
-        else if (Pm <= 32) hipLaunchKernelGGL((k_dense_pair<T, 32>), dim3(grid), dim3(256), 0, st, a);
-        else if (Pm <= 48) hipLaunchKernelGGL((k_dense_pair<T, 48>), dim3(grid), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_dense_pair<T, 64>), dim3(grid), dim3(256), 0, st, a);
-        KFC_HIP(op->ctx, hipGetLastError());
-    }
     if (op->pairs.groups.empty()) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "dense pair path without groups");
-    return KFC_OK;
+    const size_t ng = op->pairs.groups.size();
+    std::vector<PairArgs<T>> args(ng);
+    int64_t slabs = 0;
+    for (size_t gi = 0; gi < ng; ++gi) {
+        args[gi] = pair_args<T>(op, gi, X, Y, ncols, ld);
+        slabs += args[gi].nslab;
+    }
+    const int cus = op->ctx->num_cus;
+    auto all = [&](const PairShape& sh) {
+        for (size_t gi = 0; gi < ng; ++gi) {
+            const int rc = launch_group<T>(sh, args[gi], args[gi].nterm, nullptr, 1, cus, st);
+            if (rc != KFC_OK) return rc;
+        }
+        return int(KFC_OK);
+    };
+    return run_pair_groups(op->ctx, pair_tune_key(op, sizeof(T), slabs), st, all);
 }
 
 template int launch_pairs<double>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t);
