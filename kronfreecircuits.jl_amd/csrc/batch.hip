@@ -277,7 +277,30 @@ struct Batch {
     DevBuf dN, dxo, dwo, dW, dact, bX, bY, bZ, bH, dG, dC, dcoef, dth, dr;
     BatchDev<S> dev;
     std::vector<int> act;
+    // The filter of one iteration: every step's coefficients in one table (pinned host
+    // copy, device copy), and its launches captured as a graph that is replayed while
+    // the active set, the degree and the table stay the same.
+    double* hcoefs = nullptr;
+    size_t ncoefs = 0;
+    DevBuf dcoefs;
+    hipGraphExec_t fexec = nullptr;
+    std::vector<int> fact;
+    int fdeg = 0;
+    const void* fcoefs = nullptr;
+    Batch() = default;
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    ~Batch() {
+        if (fexec) (void)hipGraphExecDestroy(fexec);
+        if (hcoefs) (void)hipHostFree(hcoefs);
+    }
 };
+
+// KFC_GRAPHS=0 runs the filter steps as separate launches instead of one graph.
+bool graphs_enabled() {
+    const char* e = getenv("KFC_GRAPHS");
+    return !(e && e[0] == '0');
+}
 
 template <class S>
 int batch_init(Batch<S>& b, kfc_ctx* ctx, const std::vector<Band1D>& bands, int nb) {
@@ -363,6 +386,67 @@ int bapply(Batch<S>& b, const S* X, const S* Z, S* Y, const std::vector<double>&
     hipLaunchKernelGGL(k_bapply<S>, dim3(b.nb, unsigned(b.act.size())), dim3(256), 0, b.ctx->stream, b.dev, X, Z, Y,
                        b.dcoef.template as<double>());
     KFC_HIP(b.ctx, hipGetLastError());
+    return KFC_OK;
+}
+
+// The degree steps of the filter on every active block (the three-term recurrence
+// through X, Y, Z; step k's coefficients at row k of the device table), the result
+// copied to X.
+template <class S>
+int filter_launches(Batch<S>& b, S* X, S* Y, S* Z, int degree) {
+    const double* tab = b.dcoefs.template as<double>();
+    const size_t stride = 3 * size_t(b.n);
+    const dim3 grid(b.nb, unsigned(b.act.size()));
+    hipLaunchKernelGGL(k_bapply<S>, grid, dim3(256), 0, b.ctx->stream, b.dev, X, nullptr, Y, tab);
+    S* x0 = X;
+    S* y0 = Y;
+    S* z0 = Z;
+    for (int k = 2; k <= degree; ++k) {   // z = a H y + b y + g x
+        hipLaunchKernelGGL(k_bapply<S>, grid, dim3(256), 0, b.ctx->stream, b.dev, y0, x0, z0,
+                           tab + size_t(k - 1) * stride);
+        S* t = x0;   // (x, y, z) <- (y, z, x)
+        x0 = y0;
+        y0 = z0;
+        z0 = t;
+    }
+    if (y0 != X) hipLaunchKernelGGL(k_bcopy<S>, grid, dim3(256), 0, b.ctx->stream, b.dev, y0, X);
+    KFC_HIP(b.ctx, hipGetLastError());
+    return KFC_OK;
+}
+
+// The filter of one iteration, replayed from a graph captured on its first run with this
+// active set and degree (the launches read their coefficients from the device table,
+// which the caller refreshed).
+template <class S>
+int run_filter(Batch<S>& b, S* X, S* Y, S* Z, int degree) {
+    if (b.act.empty()) return KFC_OK;
+    if (!graphs_enabled()) return filter_launches<S>(b, X, Y, Z, degree);
+    hipStream_t st = b.ctx->stream;
+    if (!b.fexec || b.fact != b.act || b.fdeg != degree || b.fcoefs != b.dcoefs.p) {
+        if (b.fexec) {
+            KFC_HIP(b.ctx, hipGraphExecDestroy(b.fexec));
+            b.fexec = nullptr;
+        }
+        KFC_HIP(b.ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = filter_launches<S>(b, X, Y, Z, degree);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc != KFC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return kfc_fail(b.ctx, KFC_EDEVICE, std::string("filter graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&b.fexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b.fexec = nullptr;
+            return kfc_fail(b.ctx, KFC_EDEVICE, std::string("filter graph instantiate: ") + hipGetErrorString(ei));
+        }
+        b.fact = b.act;
+        b.fdeg = degree;
+        b.fcoefs = b.dcoefs.p;
+    }
+    KFC_HIP(b.ctx, hipGraphLaunch(b.fexec, st));
     return KFC_OK;
 }
 
@@ -536,36 +620,37 @@ int bphase(Batch<S>& b, std::vector<ProbState>& ps, int m, int degree, int64_t m
             sig[p] = e[p] / (a0 - c[p]);
             tau[p] = 2.0 / sig[p];
         }
-        // Y = (H X - c X) sigma / e
-        std::vector<double> coef(3 * size_t(b.n), 0.0);
-        for (int p : act) {
-            coef[3 * p] = sig[p] / e[p];
-            coef[3 * p + 1] = -c[p] * sig[p] / e[p];
+        // every step's coefficients: step 0 Y = (H X - c X) sigma / e, then the
+        // three-term recurrence z = a H y + b y + g x (Zhou-Saad scaling)
+        const size_t stride = 3 * size_t(b.n), need = stride * size_t(degree);
+        if (b.ncoefs < need) {
+            if (b.hcoefs) KFC_HIP(b.ctx, hipHostFree(b.hcoefs));
+            b.hcoefs = nullptr;
+            b.ncoefs = 0;
+            KFC_HIP(b.ctx, hipHostMalloc(reinterpret_cast<void**>(&b.hcoefs), sizeof(double) * need, hipHostMallocDefault));
+            b.ncoefs = need;
+            KFC_HIP(b.ctx, b.dcoefs.ensure(sizeof(double) * need));
         }
-        rc = bapply<S>(b, X, nullptr, Y, coef);
-        if (rc != KFC_OK) return rc;
-        S* x0 = X;
-        S* y0 = Y;
-        S* z0 = Z;
+        KFC_HIP(b.ctx, hipStreamSynchronize(b.ctx->stream));   // the previous iteration's table is read
+        std::fill(b.hcoefs, b.hcoefs + need, 0.0);
+        for (int p : act) {
+            b.hcoefs[3 * p] = sig[p] / e[p];
+            b.hcoefs[3 * p + 1] = -c[p] * sig[p] / e[p];
+        }
         for (int k = 2; k <= degree; ++k) {
+            double* row = b.hcoefs + size_t(k - 1) * stride;
             for (int p : act) {
                 const double sn = 1.0 / (tau[p] - sig[p]);
-                coef[3 * p] = 2.0 * sn / e[p];
-                coef[3 * p + 1] = -2.0 * c[p] * sn / e[p];
-                coef[3 * p + 2] = -sig[p] * sn;
+                row[3 * p] = 2.0 * sn / e[p];
+                row[3 * p + 1] = -2.0 * c[p] * sn / e[p];
+                row[3 * p + 2] = -sig[p] * sn;
                 sig[p] = sn;
             }
-            rc = bapply<S>(b, y0, x0, z0, coef);   // z = a H y + b y + g x
-            if (rc != KFC_OK) return rc;
-            S* t = x0;   // (x, y, z) <- (y, z, x)
-            x0 = y0;
-            y0 = z0;
-            z0 = t;
         }
-        if (y0 != X) {
-            rc = bcopy<S>(b, y0, X);
-            if (rc != KFC_OK) return rc;
-        }
+        KFC_HIP(b.ctx, hipMemcpyAsync(b.dcoefs.p, b.hcoefs, sizeof(double) * need, hipMemcpyHostToDevice,
+                                      b.ctx->stream));
+        rc = run_filter<S>(b, X, Y, Z, degree);
+        if (rc != KFC_OK) return rc;
         for (int p : act) {
             ps[p].matvec += int64_t(degree) * nb;
             ps[p].iters += 1;
