@@ -78,6 +78,19 @@ def cpu_baseline(theta, N, n_matvec_gpu, m):
             "seconds_per_apply": round(t, 4)}
 
 
+def measured_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE runs, each scaled by what the counter reads for a copy of known size),
+    with the file and the commit it was measured at: PMC counters cannot run inside
+    the timed solve, so this figure is not from this run."""
+    tf = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(tf):
+        return None, None
+    with open(tf) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch"), f"profiles/{name} (measured at commit {d.get('commit', 'unrecorded')})"
+
+
 def calibrate_copy(gib=1, reps=5):
     """y = x + 0 over `gib` GiB of FP64, `reps` times: each launch reads and writes
     exactly gib GiB, the yardstick that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -92,105 +105,116 @@ def calibrate_copy(gib=1, reps=5):
                       "bytes_written_per_launch": n * 8, "launches": reps}))
 
 
-def sweep_bench(args):
-    """Zero-pi parameter sweep (parent dims [40, 40, 30], LOBPCG lowest 16, tol 1e-8):
-    plain FP64 point after point against batched solves with and without the
-    learned warm start.  Counted pairs have an FP64 true residual <= 1e-8."""
+def zero_pi_runs(args):
+    """The zero-pi sweep (parent dims [40, 40, 30], LOBPCG lowest 16, tol 1e-8) at
+    args.test_points seeded points, each path timed end to end (operators, seeds,
+    solve, FP64 acceptance residuals), counting pairs whose FP64 true residual is
+    <= 1e-8:
+
+    * plain_fp64_cold: point after point, product seed (the reference's materialize);
+    * plain_fp64_warm: point after point, each warm-started from the previous point's
+      vectors (the reference's sweep convention), the points walked in
+      nearest-neighbour order;
+    * lockstep_fp64: every point's LOBPCG step in one launch per kernel;
+    * lockstep_mixed_learned: lockstep, FP32 bulk and FP64 refinement, started from
+      the warm-start network's prediction (trained here on args.train_points
+      points' plain-solver eigenpairs).
+
+    Returns the record and the lockstep run's outputs."""
     import numpy as np
 
     from kfc import configs
-    from kfc.sweeps import WarmStart, ZeroPiSweep, subspace_capture
+    from kfc.sweeps import WarmStart, ZeroPiSweep, path_order, subspace_capture
 
-    sw = ZeroPiSweep(contexts=args.contexts)
+    sw = ZeroPiSweep(contexts=args.contexts)   # lockstep runs on the first; the training solves use them all
     m = sw.m
     test = [configs.zero_pi_theta(50_000 + i) for i in range(args.test_points)]
     train = [configs.zero_pi_theta(i) for i in range(args.train_points)]
 
     def counted(out):
-        return sum(int(np.sum(np.asarray(o[2]["max_residual"]) <= 1e-8)) * m for o in out)
+        return sum(m for o in out if o[2]["max_residual"] <= 1e-8)
 
-    lock = not args.contexts_only
-    # warm-up: one point through every path (compiles, allocates, tunes)
+    def summary(out, wall):
+        return {"seconds": round(wall, 3), "eigenpairs_per_s": round(counted(out) / wall, 2),
+                "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in out])), 1),
+                "matvec_columns": int(sum(o[2]["n_matvec"] for o in out)),
+                "max_residual": max(o[2]["max_residual"] for o in out)}
+
+    # warm-up: every path once at the sizes it runs at (compiles, tunes launch shapes)
     sw.run_plain(test[:1])
-    sw.run_batched(test[:2], mode=args.sweep_mode, lockstep=lock)
-    plain, t_plain = sw.run_plain(test[:args.plain_points])
-    rate_plain = counted(plain) / t_plain
-    log(f"[sweep] plain f64: {args.plain_points} points {t_plain:.2f} s -> {rate_plain:.1f} pairs/s, "
-        f"iters {[o[2]['iterations'] for o in plain]}")
-    t0 = time.perf_counter()
-    # training data: the lowest m + g pairs (the guards too) from the plain solver
-    nev = sw.width(40 * 40 * 30)
-    gen, t_gen = sw.run_batched(train, mode="f64", nev=nev, lockstep=lock)
-    frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
-    log(f"[sweep] training data: {len(train)} points ({len(train) * nev} eigenpairs) in {t_gen:.1f} s")
-    ws = WarmStart(sw, nbasis=args.nbasis, seed=1)
-    t1 = time.perf_counter()
-    loss = ws.train(train, frames, steps=args.train_steps, lr=1e-3)
-    t_train = time.perf_counter() - t1
-    log(f"[sweep] network trained in {t_train:.1f} s, loss {loss[0]:.4g} -> {loss[-1]:.4g}")
+    sw.run_batched(test, mode="f64", lockstep=True)
+    sw.run_batched(test, mode="mixed", lockstep=True)
     res = {}
-    for name, mode, x0 in (("batched_f64", "f64", None), ("batched_mixed", "mixed", None),
-                           ("batched_f64_learned", "f64", ws.x0_fn), ("batched_mixed_learned", "mixed", ws.x0_fn)):
-        out, wall = sw.run_batched(test, mode=mode, x0_fn=x0, lockstep=lock)
-        res[name] = {"seconds": round(wall, 3), "eigenpairs_per_s": round(counted(out) / wall, 2),
-                     "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in out])), 1),
-                     "max_residual": max(o[2]["max_residual"] for o in out),
-                     "matvec_columns": int(sum(o[2]["n_matvec"] for o in out)),
-                     "seconds_apply_gpu": round(float(sum(o[2]["seconds_apply_gpu"] for o in out)), 4)}
-        log(f"[sweep] {name}: {wall:.2f} s, {res[name]['eigenpairs_per_s']} pairs/s, "
-            f"mean iters {res[name]['mean_iterations']}")
-        if name == "batched_f64":
-            ref = out
-    # the same sweep through independent solves on several contexts, for comparison
-    out_ctx, wall_ctx = sw.run_batched(test, mode="f64")
-    res["contexts_f64"] = {"seconds": round(wall_ctx, 3), "eigenpairs_per_s": round(counted(out_ctx) / wall_ctx, 2),
-                           "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in out_ctx])), 1),
-                           "max_residual": max(o[2]["max_residual"] for o in out_ctx)}
-    log(f"[sweep] contexts_f64 ({args.contexts} contexts): {wall_ctx:.2f} s, {res['contexts_f64']['eigenpairs_per_s']} pairs/s")
-    # agreement with the plain path and what the warm start captures
-    dE = max(float(np.max(np.abs(a[0] - b[0]) / np.abs(b[0]))) for a, b in zip(ref[:args.plain_points], plain))
-    Xf = ws.predict_frames(test[:8])
-    caps = []
-    for (E, V, st, kids), xf in zip(ref[:8], Xf):
-        X0 = sw.from_frame(xf, sw.frame_index(kids), V.shape[0])
-        caps.append(float(np.mean(subspace_capture(X0, V[:, :m]))))
-    best = max(res, key=lambda k: res[k]["eigenpairs_per_s"])
-    # the parent apply's algorithmic work (dense kept-basis pairs on MFMA)
+    cold, t = sw.run_plain(test)
+    res["plain_fp64_cold"] = summary(cold, t)
+    log(f"[zero-pi] plain FP64 cold: {t:.2f} s, {res['plain_fp64_cold']['eigenpairs_per_s']} pairs/s")
+    order = path_order(test)
+    warm, t = sw.run_plain_warm([test[i] for i in order])
+    res["plain_fp64_warm"] = summary(warm, t)
+    log(f"[zero-pi] plain FP64 warm from the previous point: {t:.2f} s, "
+        f"{res['plain_fp64_warm']['eigenpairs_per_s']} pairs/s")
+    lock, t = sw.run_batched(test, mode="f64", lockstep=True)
+    res["lockstep_fp64"] = summary(lock, t)
+    res["lockstep_fp64"]["seconds_apply_gpu"] = round(float(sum(o[2]["seconds_apply_gpu"] for o in lock)), 4)
+    log(f"[zero-pi] lockstep FP64: {t:.2f} s, {res['lockstep_fp64']['eigenpairs_per_s']} pairs/s")
+    # the warm-start network, trained on the plain solver's lowest m + g pairs
+    nev = sw.width(40 * 40 * 30)
+    t0 = time.perf_counter()
+    gen, _ = sw.run_batched(train, mode="f64", nev=nev, lockstep=True)
+    frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
+    ws = WarmStart(sw, nbasis=args.nbasis, seed=1)
+    loss = ws.train(train, frames, steps=args.train_steps, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    log(f"[zero-pi] warm start: {len(train) * nev} training pairs, loss {loss[0]:.4g} -> {loss[1]:.4g}, {t_train:.1f} s")
+    # the two effects apart: FP64 from the prediction, mixed from the product seed
+    out, t = sw.run_batched(test, mode="f64", x0_fn=ws.x0_fn, lockstep=True)
+    res["lockstep_fp64_learned"] = summary(out, t)
+    out, t = sw.run_batched(test, mode="mixed", lockstep=True)
+    res["lockstep_mixed"] = summary(out, t)
+    learned, t = sw.run_batched(test, mode="mixed", x0_fn=ws.x0_fn, lockstep=True)
+    res["lockstep_mixed_learned"] = summary(learned, t)
+    for k in ("lockstep_fp64_learned", "lockstep_mixed", "lockstep_mixed_learned"):
+        log(f"[zero-pi] {k}: {res[k]['seconds']:.2f} s, {res[k]['eigenpairs_per_s']} pairs/s, "
+            f"mean iterations {res[k]['mean_iterations']}")
+    base = res["plain_fp64_cold"]["eigenpairs_per_s"]
+    # the mixed path's error against the plain FP64 solve, per eigenvalue (max over points)
+    rel = np.array([np.abs(a[0] - b[0]) / np.abs(b[0]) for a, b in zip(learned, cold)])
+    caps = [float(np.mean(subspace_capture(sw.from_frame(xf, sw.frame_index(kids), V.shape[0]), V[:, :m])))
+            for xf, (E, V, st, kids) in zip(ws.predict_frames(test[:8]), lock[:8])]
     H0, _ = sw.parent(test[0])
-    bytes_col, flops_col = H0.work()
-    # the batched FP64 sweep's apply: HIP-event time of its apply launches (lockstep:
-    # one launch per axis-pair group for all points) against the columns applied
-    bcols, bsec = res["batched_f64"]["matvec_columns"], res["batched_f64"]["seconds_apply_gpu"]
-    apply_rate = flops_col * bcols / bsec / 1e12 if bsec > 0 else None
-    plain_cols = sum(o[2]["n_matvec"] for o in plain)
-    plain_apply = sum(o[2]["seconds_apply_gpu"] for o in plain)
-    plain_rate = flops_col * plain_cols / plain_apply / 1e12 if plain_apply > 0 else None
-    out = {"metric": "eigenpairs/s at resid<=1e-8 (zero-pi sweep, parent [40,40,30], LOBPCG lowest 16)",
-           "value": res["batched_mixed_learned"]["eigenpairs_per_s"], "unit": "eigenpairs/s", "n_gpus": 1,
-           "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * res["batched_mixed_learned"]["seconds"], 1),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-           "data": "synthetic (seeded circuit parameters)",
-           "config": {"workload": "zero_pi_sweep", "dims": [40, 40, 30], "m": m, "tol": 1e-8,
-                      "test_points": args.test_points, "train_points": args.train_points, "nbasis": args.nbasis,
-                      "contexts": args.contexts,
-                      "batched": "lockstep LOBPCG (kfc_lobpcg_lockstep)" if lock else "independent solves on contexts"},
-           "plain_fp64": {"points": args.plain_points, "seconds": round(t_plain, 3),
-                          "eigenpairs_per_s": round(rate_plain, 2),
-                          "mean_iterations": round(float(np.mean([o[2]["iterations"] for o in plain])), 1)},
+    _, flops_col = H0.work()
+    bsec, bcols = res["lockstep_fp64"]["seconds_apply_gpu"], res["lockstep_fp64"]["matvec_columns"]
+    rate = flops_col * bcols / bsec / 1e12 if bsec > 0 else None
+    rec = {"workload": "zero_pi_sweep", "dims": [40, 40, 30], "m": m, "tol": 1e-8, "points": len(test),
            **res,
-           "speedup_vs_plain": {k: round(v["eigenpairs_per_s"] / rate_plain, 2) for k, v in res.items()},
-           "best": best, "max_rel_dE_batched_vs_plain": dE,
-           "roofline": {"bound": "mfma", "kernel": "k_dense_pair_reg_batch" if lock else "k_dense_pair_reg",
-                        "flops_per_column": flops_col,
-                        "achieved": round(apply_rate, 4) if apply_rate else None, "peak": 78.6, "unit": "TFLOP/s",
-                        "frac": round(apply_rate / 78.6, 5) if apply_rate else None,
-                        "plain_solve_rate": round(plain_rate, 4) if plain_rate else None,
-                        "note": "FP64 MFMA; HIP-event time of the batched FP64 sweep's applies (plain_solve_rate: "
-                                "one solve at a time, launch-bound at 48 000 states)"},
-           "warm_start": {"train_seconds": round(t_train, 2), "data_seconds": round(t_gen, 2),
-                          "initial_loss": float(loss[0]), "final_loss": float(loss[-1]),
-                          "outputs": f"{ws.nb} x {ws.m}", "training_eigenpairs": len(train) * nev,
-                          "mean_subspace_capture": round(float(np.mean(caps)), 4)}}
+           "speedup_vs_plain_fp64_cold": {k: round(v["eigenpairs_per_s"] / base, 2) for k, v in res.items()},
+           "best": max(res, key=lambda k: res[k]["eigenpairs_per_s"]),
+           "target": "lockstep_mixed_learned >= 5x plain_fp64_cold",
+           "target_met": bool(res["lockstep_mixed_learned"]["eigenpairs_per_s"] >= 5 * base),
+           "mixed_rel_dE_per_eigenvalue_vs_fp64": [float(f"{x:.3e}") for x in rel.max(axis=0)],
+           "mixed_max_rel_dE_vs_fp64": float(rel.max()),
+           "warm_start": {"training_points": len(train), "training_eigenpairs": len(train) * nev,
+                          "outputs": f"{ws.nb} x {ws.m}", "initial_loss": float(loss[0]), "final_loss": float(loss[1]),
+                          "seconds_data_and_training": round(t_train, 2),
+                          "mean_subspace_capture": round(float(np.mean(caps)), 4)},
+           "roofline_dense_pairs": {"bound": "mfma", "kernel": "dense kept-basis pairs (lockstep FP64 applies)",
+                                    "achieved": round(rate, 3) if rate else None, "peak": 78.6, "unit": "TFLOP/s",
+                                    "frac": round(rate / 78.6, 4) if rate else None,
+                                    "flops_per_column": flops_col}}
+    return rec, lock
+
+
+def sweep_bench(args):
+    """The zero-pi sweep on its own line: the value is lockstep mixed + learned."""
+    rec, _ = zero_pi_runs(args)
+    r = rec["lockstep_mixed_learned"]
+    out = {"metric": "eigenpairs/s at resid<=1e-8 (zero-pi sweep, parent [40,40,30], LOBPCG lowest 16)",
+           "value": r["eigenpairs_per_s"], "unit": "eigenpairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(1e3 * r["seconds"], 1), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded circuit parameters)",
+           "config": {"workload": "zero_pi_sweep", "dims": [40, 40, 30], "m": 16, "tol": 1e-8,
+                      "test_points": args.test_points, "train_points": args.train_points},
+           "roofline": rec["roofline_dense_pairs"], "zero_pi_sweep": rec}
     print(json.dumps(out))
 
 
@@ -460,6 +484,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-sweep", action="store_true", help="leave the zero-pi sweep sub-record out")
     ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many cycles")
     ap.add_argument("--workload", default="three_fluxonia",
                     choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports", "seeds", "drive",
@@ -547,13 +572,10 @@ def main():
     s_bytes = 8 if args.mode == "f64" else 4
     bytes_per_launch = 3 * (args.N ** 3) * s_bytes
     achieved = bytes_per_launch / avg_col / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"stencil_traffic_{args.N}.json")
-    if os.path.exists(tf) and args.mode == "f64":
-        with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    traffic, traffic_src = measured_traffic(f"stencil_traffic_{args.N}.json") if args.mode == "f64" else (None, None)
     roof_apply = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_stencil_p",
+                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                  "kernel": "k_stencil_p",
                   "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_col * 1e6, 1),
                   "seconds": round(st["seconds_apply_gpu"], 3)}
     # roofline of the block passes (k_tsmm: Gram-only first Gram-Schmidt pass,
@@ -565,13 +587,10 @@ def main():
         no = st["n_ortho_calls"]
         b_o, t_o = st["bytes_ortho"] / no, st["seconds_ortho_gpu"] / no
         a_o = b_o / t_o / 1e9
-        traffic_o = None
-        tf = os.path.join(ROOT, "profiles", f"tsmm_traffic_{args.N}.json")
-        if os.path.exists(tf) and args.mode == "f64":
-            with open(tf) as f:
-                traffic_o = json.load(f).get("bytes_per_launch")
+        traffic_o, src_o = measured_traffic(f"tsmm_traffic_{args.N}.json") if args.mode == "f64" else (None, None)
         roof_ortho = {"bound": "hbm", "achieved": round(a_o, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": round(a_o / HBM_PEAK_GBS, 4), "traffic": traffic_o, "kernel": "k_tsmm",
+                      "frac": round(a_o / HBM_PEAK_GBS, 4), "traffic": traffic_o, "traffic_source": src_o,
+                      "kernel": "k_tsmm",
                       "algorithmic_bytes_per_launch": round(b_o), "avg_launch_us": round(t_o * 1e6, 1),
                       "launches": no, "seconds": round(st["seconds_ortho_gpu"], 3)}
         # the line's roofline is the kernel that holds the GPU longer in the solve
@@ -592,6 +611,8 @@ def main():
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(theta, args.N, st["n_matvec"], m)
+    if not args.skip_sweep:
+        extra["zero_pi_sweep"], _ = zero_pi_runs(args)
     out = {"metric": "eigenpairs/s at resid<=1e-8 (three fluxonia 512^3, Lanczos lowest 10)",
            "value": round(value, 5), "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 2), "higher_is_better": True,

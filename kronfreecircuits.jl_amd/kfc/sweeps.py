@@ -109,6 +109,21 @@ class ZeroPiSweep:
             out.append((E, V, st, kids))
         return out, time.perf_counter() - t0
 
+    def run_plain_warm(self, thetas, tol=1e-8):
+        """One point after another, FP64, each warm-started from the previous point's
+        kept vectors as the reference's sweeps do (``materialize(...; previous)`` ->
+        ``_initial_block``, src/materialize.jl:656-666, bench/common.jl:135-163): the
+        raw coefficients of the previous parent solution, m columns, the guards from
+        the product seed; the first point starts cold."""
+        out, t0, prev = [], time.perf_counter(), None
+        for th in thetas:
+            H, kids = self.parent(th)
+            X0 = prev if prev is not None and prev.shape[0] == H.N else None
+            E, V, st = solve_stats(LOBPCG(tol=tol, mode="f64"), H, self.m, X0=X0)
+            out.append((E, V, st, kids))
+            prev = V[:, :self.m]
+        return out, time.perf_counter() - t0
+
     def width(self, N):
         """Columns LOBPCG converges on a parent of N states: m plus the guards."""
         return self.m + max(0, min(self.guard, (N - 5) // 3 - self.m))
@@ -191,6 +206,23 @@ class WarmStart:
                 X0[:, bad] = np.random.default_rng(0).standard_normal((H.N, int(bad.sum())))
             out.append(X0)
         return out
+
+
+def path_order(thetas):
+    """A walk through the points, each next point the nearest unvisited one in
+    (E_CS, n_g) scaled to their ranges, from the smallest E_CS: the order that gives a
+    sweep warm-started from the previous point its closest neighbours."""
+    P = np.array([[(t["E_CS"] - 0.02) / 0.18, t["n_g"]] for t in thetas])
+    left = list(range(len(thetas)))
+    cur = int(np.argmin(P[:, 0]))
+    order = [cur]
+    left.remove(cur)
+    while left:
+        d = np.linalg.norm(P[left] - P[cur], axis=1)
+        cur = left[int(np.argmin(d))]
+        order.append(cur)
+        left.remove(cur)
+    return order
 
 
 def subspace_capture(X0, V):
