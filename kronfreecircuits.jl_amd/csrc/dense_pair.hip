@@ -514,6 +514,206 @@ __global__ __launch_bounds__(64 * (P / 16)) void k_dense_pair_chain_batch(const 
     dense_pair_slabs_chain<T, P, NT>(a, blockIdx.x, gridDim.x);
 }
 
+// Wave-per-slab variant for groups of at most NT terms with extents up to P <= 48: the
+// block stages A_t and c_t B_t in LDS once, then each of its W waves works through whole
+// slabs on its own (no block barrier after the staging), so a wave's loads and stores
+// overlap the other waves' matrix-core work.  Per slab and per 16-wide tile tc of the
+// p axis (the axis of the result's lanes):
+//   U_t^T[:, tc] = X_s^T A_t^T[:, tc]        (GEMM1: X_s^T fragments in registers, loaded
+//                                             once per slab; A_t from LDS)
+//   Y_s^T[:, tc] += (c_t B_t) U_t^T[:, tc]   (GEMM2: c_t B_t from LDS)
+// so Y_s = A_t X_s (c_t B_t)^T.  GEMM1's accumulator tiles are GEMM2's B operands as
+// they stand (the same chaining as above), and the result tile holds 16 consecutive
+// p-axis entries per register: with a unit-strided p axis the epilogue's loads and
+// stores are whole 128-byte lines.  P / 4 contraction steps (P = 40: ten, not twelve).
+// Buffer access to one slab: the descriptor's range ends where the slab's last column
+// ends, so a lane of a padded tile past dq reads zero and its store is dropped, and a
+// lane past dp sets bit 31 of its offset (past every range) for the same effect; no lane
+// masks.  dq * sq >= (dq - 1) * sq + dp * sp puts every column index >= dq past the range.
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t slab_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
+}
+__device__ inline double bload(double, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline float bload(float, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline void bstore(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, 0);
+}
+__device__ inline void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+// bit 31 when index >= extent
+__device__ inline uint32_t past(int index, int extent) { return uint32_t(extent - 1 - index) & 0x80000000u; }
+
+// LDS layout of a staged d x d port matrix on a P-wide side: column-major with leading
+// dimension ld_of and cols_of columns.  FP64 packs it at P (a 16-row tile past P reads
+// the next column's entries, which only reach result entries outside the slab, and the
+// pad keeps the last one in the array); FP32 pads to whole tiles, since its contraction
+// steps reach 12 rows past their first.
+template <class T, int P>
+constexpr int lds_ld() { return sizeof(T) == 8 ? P : 16 * ((P + 15) / 16); }
+template <class T, int P>
+constexpr int lds_size() { return lds_ld<T, P>() * lds_ld<T, P>(); }
+
+// PI, PJ: compile-time extents (16, 32, 40 or 48) bounding dp and dq; every loop bound is
+// a constant, so the compiler pipelines the LDS reads under the matrix-core work, and
+// rows or columns past dp, dq cost only padded work: their loads read zero and their
+// stores are dropped (buffer ranges above).
+//
+// Per slab and per 16-wide tile tp of the q axis:
+//   U[:, tp] = X_s (c_t B_t)^T[:, tp]      (GEMM1: X_s fragments in registers, loaded once
+//                                          per slab, 16 consecutive p-axis entries a row;
+//                                          c_t B_t from LDS)
+//   Y_s^T[tp, :] += U[:, tp]^T A_t^T       (GEMM2: A_t from LDS)
+// GEMM1's accumulator tile, lane (r, k) holding rows out_row(k, q) of column r, is
+// GEMM2's A operand (row r, contraction index out_row(k, q)) as it stands, and GEMM2's
+// result holds 16 consecutive p-axis entries per register: the loads of X and of the
+// epilogue and the stores all move whole 128-byte rows.
+template <class T, int PI, int PJ, int NT, int W>
+__device__ inline void dense_pair_slabs_wave(const PairArgs<T>& a, int64_t w0, int64_t wstep) {
+    using v4 = typename V4<T>::type;
+    constexpr int NTI = (PI + 15) / 16, NTJ = (PJ + 15) / 16, KSJ = PJ / 4;
+    constexpr int LDA = lds_ld<T, PI>(), LDB = lds_ld<T, PJ>();
+    constexpr int MSA = lds_size<T, PI>(), MSB = lds_size<T, PJ>();
+    constexpr int PADA = 16 * NTI - LDA + 1, PADB = 16 * NTJ - LDB + 1;
+    __shared__ T As[NT * MSA + PADA];   // A_t, zero outside dp x dp
+    __shared__ T Bs[NT * MSB + PADB];   // c_t B_t, zero outside dq x dq
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int dp = a.dp, dq = a.dq, nterm = a.nterm;
+    for (int e = tid; e < NT * MSA + PADA; e += 64 * W) {
+        const int t = e / MSA, f = e - t * MSA, i = f % LDA, k = f / LDA;
+        As[e] = (t < nterm && i < dp && k < dp) ? a.A[t][i + (int64_t)k * dp] : T(0);
+    }
+    for (int e = tid; e < NT * MSB + PADB; e += 64 * W) {
+        const int t = e / MSB, f = e - t * MSB, i = f % LDB, k = f / LDB;
+        Bs[e] = (t < nterm && i < dq && k < dq) ? a.c[t] * a.B[t][i + (int64_t)k * dq] : T(0);
+    }
+    __syncthreads();
+    const int64_t perc = a.L * a.M * a.R;
+    // in-slab byte offsets fit 31 bits (the launcher checks it)
+    const uint32_t bsp0 = uint32_t(a.sp * sizeof(T)), bsq0 = uint32_t(a.sq * sizeof(T));
+    const uint32_t extent = uint32_t((dq - 1) * bsq0 + dp * bsp0);
+    const int64_t wave = w0 + __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int64_t s = wave; s < a.nslab; s += wstep) {
+        const int64_t col = s / perc;
+        int64_t rem = s - col * perc;
+        const int64_t l = rem % a.L;
+        rem /= a.L;
+        const int64_t m = rem % a.M;
+        const int64_t r = rem / a.M;
+        const int64_t sbase = l + m * a.L * a.dp + r * a.sr;
+        const __amdgpu_buffer_rsrc_t xr = slab_rsrc(a.X + col * a.ld + sbase, extent);
+        const __amdgpu_buffer_rsrc_t yr = slab_rsrc(a.Y + col * a.ld + sbase, extent);
+        const __amdgpu_buffer_rsrc_t dr = slab_rsrc(a.D ? a.D + sbase : a.X, extent);
+        // the strides, opaque to the compiler per slab: offsets are formed where they are
+        // used rather than held across the loop
+        uint32_t bsp = __builtin_amdgcn_readfirstlane(bsp0), bsq = __builtin_amdgcn_readfirstlane(bsq0);
+        asm volatile("" : "+s"(bsp), "+s"(bsq));
+        // p-axis offset of tile ti's lane (bit 31 past dp; the q-axis part, below 2^31,
+        // cannot carry it back into range)
+        auto poff = [&](int ti) { return (uint32_t(16 * ti + r16) * bsp) | past(16 * ti + r16, dp); };
+        // GEMM1's A operand: xa[ti][ks] = X_s[16 ti + r16, 4 ks + k4]
+        T xa[NTI][KSJ];
+#pragma unroll
+        for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+            for (int ks = 0; ks < KSJ; ++ks) xa[ti][ks] = bload(T(0), xr, poff(ti) + uint32_t(4 * ks + k4) * bsq);
+#pragma unroll
+        for (int tp = 0; tp < NTJ; ++tp) {
+            // c_t B_t and A_t are read again from LDS for every tile: kept in registers
+            // across them they would take far more per lane
+            asm volatile("" ::: "memory");
+            // the epilogue's HBM operand for this tile, in flight during its GEMMs: X (the
+            // first group, for D .* X; D is shared by every column and stays in L2, so it
+            // is read at the end) or Y (groups accumulating into it), at
+            // Y_s[16 tc + r16, 16 tp + out_row(k4, q)]
+            const bool diag = a.D != nullptr;
+            T pv[NTI][4];
+#pragma unroll
+            for (int tc = 0; tc < NTI; ++tc)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    pv[tc][q] = bload(T(0), diag ? xr : yr, poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq);
+            v4 acc[NTI];
+#pragma unroll
+            for (int tc = 0; tc < NTI; ++tc) acc[tc] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                if (t < nterm) {   // uniform
+                    const T* At = As + t * MSA;
+                    const T* Bt = Bs + t * MSB;
+                    // one 16-row tile of U at a time: GEMM1 into it, then its share of GEMM2
+#pragma unroll
+                    for (int ti = 0; ti < NTI; ++ti) {
+                        v4 u = v4{0, 0, 0, 0};
+#pragma unroll
+                        for (int ks = 0; ks < KSJ; ++ks)
+                            u = mfma(xa[ti][ks], Bt[(16 * tp + r16) + (4 * ks + k4) * LDB], u);   // c_t B_t[j, j']
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            // steps whose contraction indices all lie past PI are padding
+                            if (16 * ti + out_row(T(0), 0, q) < PI)
+#pragma unroll
+                                for (int tc = 0; tc < NTI; ++tc)
+                                    acc[tc] = mfma(u[q], At[(16 * tc + r16) + (16 * ti + out_row(T(0), k4, q)) * LDA],
+                                                   acc[tc]);   // A_t[i, i']
+                        }
+                    }
+                }
+            }
+            T dv[NTI][4];
+#pragma unroll
+            for (int tc = 0; tc < NTI; ++tc)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    dv[tc][q] = diag ? bload(T(0), dr, poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq) : T(1);
+#pragma unroll
+            for (int tc = 0; tc < NTI; ++tc)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    bstore(T(dv[tc][q] * pv[tc][q] + acc[tc][q]), yr,
+                           poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq);
+        }
+    }
+}
+
+// Four-wave blocks; the register budget allows two waves per SIMD (at three the compiler
+// spills, and FP64 with two terms at 40 x 40 fits only three blocks a unit in LDS).
+constexpr int WAVE_W = 4;
+
+// Blocks go to the eight XCDs round-robin, so blocks b, b + 8, ... of a row of the grid share
+// one XCD and its L2.  The logical block of b numbers those classes one after another:
+// consecutive slabs then run on one XCD, and a cache line shared by neighbouring slabs
+// (a strided p axis interleaves them) is fetched and written back by one L2, not four.
+__device__ inline int64_t xcd_block(int b, int g) {
+    const int r = b % 8, k = b / 8;
+    int off = 0;
+    for (int c = 0; c < r; ++c) off += (g - c + 7) / 8;
+    return int64_t(off) + k;
+}
+
+template <class T, int PI, int PJ, int NT>
+__global__ __launch_bounds__(64 * WAVE_W) __attribute__((amdgpu_waves_per_eu(2)))
+void k_dense_pair_wave(const PairArgs<T> a) {
+    const int64_t lb = xcd_block(int(blockIdx.x), int(gridDim.x));
+    dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
+}
+
+template <class T, int PI, int PJ, int NT>
+__global__ __launch_bounds__(64 * WAVE_W) __attribute__((amdgpu_waves_per_eu(2)))
+void k_dense_pair_wave_batch(const PairArgs<T>* __restrict__ tab) {
+    const PairArgs<T> a = tab[blockIdx.y];
+    // (gridDim.x * blockIdx.y + b) % 8 is constant over a residue class of b, so the same
+    // numbering groups one XCD's blocks in every row
+    const int64_t lb = xcd_block(int(blockIdx.x), int(gridDim.x));
+    dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
+}
+
 // Blocks of kernel K that fit on the device at once (occupancy x compute units): a
 // chain-kernel grid is capped there, so that every block works through many slabs and
 // its setup (A_t into LDS, the B_t fragments) is paid once per block, not per slab.
@@ -715,15 +915,16 @@ bool pairs_same_structure(const kfc_op* a, const kfc_op* b) {
 
 // Launch shapes of the register-resident kernels (groups of at most two terms, extents
 // up to 48): the kernel family (0: register-chained, 1: register fragments with the
-// intermediate through LDS) and the grid (0: one block per slab, at most 4096 blocks in
-// all; k > 0: k times the blocks that fit on the device at once, each looping over
-// slabs).  Which is fastest depends on the work in flight (a single operator's columns
+// intermediate through LDS, 2: wave per slab) and the
+// grid (0: one block per slab, at most 4096 blocks in all; k > 0: k times the blocks
+// that fit on the device at once, each looping over slabs; the wave-per-slab kernels
+// always take the latter, counting a wave per slab).  Which is fastest depends on the work in flight (a single operator's columns
 // or a whole lockstep sweep), so the launcher times them on the first application of
 // each shape and keeps the winner.
 struct PairShape {
     int family, grid;
 };
-constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2}};
+constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2}, {2, 1}, {2, 2}};
 constexpr int kNumPairShapes = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
 
 template <class T, int P, int NT>
@@ -755,12 +956,63 @@ int launch_reg_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T
     return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
 }
 
+template <class T, int PI, int PJ, int NT>
+int launch_wave_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus,
+                      hipStream_t st) {
+    constexpr int W = WAVE_W, threads = 64 * W;
+    const int mult = std::max(1, sh.grid);
+    auto grid_for = [&](auto kernel) -> unsigned {
+        const int64_t cap = resident_blocks(kernel, threads, cus) * mult;
+        const int64_t need = (a.nslab + W - 1) / W;   // blocks that give every slab its own wave
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>(need, (cap + n - 1) / n)));
+    };
+    if (tab) {
+        auto k = k_dense_pair_wave_batch<T, PI, PJ, NT>;
+        hipLaunchKernelGGL(k, dim3(grid_for(k), unsigned(n)), dim3(threads), 0, st, tab);
+    } else {
+        auto k = k_dense_pair_wave<T, PI, PJ, NT>;
+        hipLaunchKernelGGL(k, dim3(grid_for(k)), dim3(threads), 0, st, a);
+    }
+    return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
+}
+
+// the smallest of 16, 32, 40, 48 bounding d (0: none)
+inline int wave_extent(int d) { return d <= 16 ? 16 : d <= 32 ? 32 : d <= 40 ? 40 : d <= 48 ? 48 : 0; }
+
+template <class T, int NT, int PI>
+int launch_wave_j(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus, hipStream_t st) {
+    switch (wave_extent(a.dq)) {
+        case 16: return launch_wave_group<T, PI, 16, NT>(sh, a, tab, n, cus, st);
+        case 32: return launch_wave_group<T, PI, 32, NT>(sh, a, tab, n, cus, st);
+        case 40: return launch_wave_group<T, PI, 40, NT>(sh, a, tab, n, cus, st);
+        default: return launch_wave_group<T, PI, 48, NT>(sh, a, tab, n, cus, st);
+    }
+}
+
+template <class T, int NT>
+int launch_wave_sized(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus,
+                      hipStream_t st) {
+    switch (wave_extent(a.dp)) {
+        case 16: return launch_wave_j<T, NT, 16>(sh, a, tab, n, cus, st);
+        case 32: return launch_wave_j<T, NT, 32>(sh, a, tab, n, cus, st);
+        case 40: return launch_wave_j<T, NT, 40>(sh, a, tab, n, cus, st);
+        default: return launch_wave_j<T, NT, 48>(sh, a, tab, n, cus, st);
+    }
+}
+
 // One group on n problems (tab: their device records, or null with n = 1 and a passed
 // by value).  Groups of at most two terms with extents up to 48 take shape sh.
 template <class T>
 int launch_group(const PairShape& sh, const PairArgs<T>& a, int ntmax, const PairArgs<T>* tab, int n, int cus,
                  hipStream_t st) {
     const int Pm = std::max(a.Mp, a.Nq);
+    // the wave-per-slab kernels address a slab with byte offsets below 2^31, padded
+    // tiles included
+    const bool wave_ok = int64_t(Pm) * (a.sp + a.sq) * int64_t(sizeof(T)) < (int64_t(1) << 31);
+    if (Pm <= 48 && ntmax <= 2 && sh.family >= 2 && wave_ok) {
+        return ntmax <= 1 ? launch_wave_sized<T, 1>(sh, a, tab, n, cus, st)
+                          : launch_wave_sized<T, 2>(sh, a, tab, n, cus, st);
+    }
     if (Pm <= 48 && ntmax <= 2) {
         if (Pm <= 16) return ntmax <= 1 ? launch_reg_group<T, 16, 1>(sh, a, tab, n, cus, st)
                                         : launch_reg_group<T, 16, 2>(sh, a, tab, n, cus, st);
@@ -785,13 +1037,13 @@ int launch_group(const PairShape& sh, const PairArgs<T>& a, int ntmax, const Pai
 }
 
 // The tuner's cache: (dims, each group's axes, extents and term count, scalar size,
-// total slabs rounded up to a power of two) -> index into kPairShapes.
+// total slabs rounded up to a power of two) -> index into kPairShapes for each group.
 struct PairTuneKey {
     std::vector<int64_t> sig;
     bool operator<(const PairTuneKey& o) const { return sig < o.sig; }
 };
 std::mutex g_pair_tune_mu;
-std::map<PairTuneKey, int> g_pair_tune;
+std::map<PairTuneKey, std::vector<int>> g_pair_tune;
 
 PairTuneKey pair_tune_key(const kfc_op* op, size_t scalar, int64_t slabs_total) {
     PairTuneKey k;
@@ -810,36 +1062,56 @@ PairTuneKey pair_tune_key(const kfc_op* op, size_t scalar, int64_t slabs_total) 
     return k;
 }
 
-// Applies every group with the tuned shape: on the first application of a key each
-// candidate runs the whole apply (idempotent: the first group writes Y, the rest add to
-// it) once to warm up and once timed with HIP events on the stream.
-int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, hipStream_t st, const std::function<int(const PairShape&)>& all) {
-    int pick = -1;
+// Applies the ng groups in order (the first writes Y, the rest add to it), each with its
+// tuned shape.  On the first application of a key every group times every candidate
+// (once to warm up, once with HIP events on the stream); a group run out of order leaves
+// Y wrong, so the whole apply runs again with the picks afterwards.
+int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, int ng, hipStream_t st,
+                    const std::function<int(const PairShape&, int)>& one) {
+    auto apply = [&](const std::vector<int>& pick) {
+        for (int g = 0; g < ng; ++g) {
+            const int rc = one(kPairShapes[pick[g]], g);
+            if (rc != KFC_OK) return rc;
+        }
+        return int(KFC_OK);
+    };
+    // KFC_PAIR_SHAPE=k pins shape k of kPairShapes for every group (read at each call:
+    // tests and measurements switch it within one process)
+    if (const char* env = std::getenv("KFC_PAIR_SHAPE")) {
+        const int k = std::atoi(env);
+        if (env[0] != '\0' && k >= 0 && k < kNumPairShapes) return apply(std::vector<int>(ng, k));
+    }
+    std::vector<int> pick;
     {
         std::lock_guard<std::mutex> g(g_pair_tune_mu);
         auto it = g_pair_tune.find(key);
         if (it != g_pair_tune.end()) pick = it->second;
     }
-    if (pick >= 0) return all(kPairShapes[pick]);
+    if (int(pick.size()) == ng) return apply(pick);
     hipEvent_t e0, e1;
     KFC_HIP(ctx, hipEventCreate(&e0));
     KFC_HIP(ctx, hipEventCreate(&e1));
-    float best = 1e30f;
-    pick = 0;
+    pick.assign(ng, 0);
     int rc = KFC_OK;
-    for (int c = 0; c < kNumPairShapes && rc == KFC_OK; ++c) {
-        rc = all(kPairShapes[c]);
-        if (rc != KFC_OK) break;
-        KFC_HIP(ctx, hipEventRecord(e0, st));
-        rc = all(kPairShapes[c]);
-        if (rc != KFC_OK) break;
-        KFC_HIP(ctx, hipEventRecord(e1, st));
-        KFC_HIP(ctx, hipEventSynchronize(e1));
-        float ms = 0;
-        KFC_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            pick = c;
+    for (int g = 0; g < ng && rc == KFC_OK; ++g) {
+        float best = 1e30f;
+        for (int c = 0; c < kNumPairShapes && rc == KFC_OK; ++c) {
+            if ((rc = one(kPairShapes[c], g)) != KFC_OK) break;
+            if ((rc = hipEventRecord(e0, st) == hipSuccess ? KFC_OK : KFC_EDEVICE) != KFC_OK) break;
+            if ((rc = one(kPairShapes[c], g)) != KFC_OK) break;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                rc = KFC_EDEVICE;
+                break;
+            }
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = KFC_EDEVICE;
+                break;
+            }
+            if (ms < best) {
+                best = ms;
+                pick[g] = c;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -849,7 +1121,7 @@ int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, hipStream_t st, const 
         std::lock_guard<std::mutex> g(g_pair_tune_mu);
         g_pair_tune[key] = pick;
     }
-    return all(kPairShapes[pick]);   // Y from the chosen shape
+    return apply(pick);   // Y from the chosen shapes
 }
 
 template <class T>
@@ -870,14 +1142,8 @@ int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* co
         slabs += h[gi * n].nslab * n;
     }
     const int cus = op0->ctx->num_cus;
-    auto all = [&](const PairShape& sh) {
-        for (size_t gi = 0; gi < ng; ++gi) {
-            const int rc = launch_group<T>(sh, h[gi * n], ntmax[gi], d + gi * n, n, cus, st);
-            if (rc != KFC_OK) return rc;
-        }
-        return int(KFC_OK);
-    };
-    return run_pair_groups(op0->ctx, pair_tune_key(op0, sizeof(T), slabs), st, all);
+    auto one = [&](const PairShape& sh, int gi) { return launch_group<T>(sh, h[gi * n], ntmax[gi], d + gi * n, n, cus, st); };
+    return run_pair_groups(op0->ctx, pair_tune_key(op0, sizeof(T), slabs), int(ng), st, one);
 }
 
 template int launch_pairs_batch<double>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t,
@@ -898,14 +1164,8 @@ int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, 
         slabs += args[gi].nslab;
     }
     const int cus = op->ctx->num_cus;
-    auto all = [&](const PairShape& sh) {
-        for (size_t gi = 0; gi < ng; ++gi) {
-            const int rc = launch_group<T>(sh, args[gi], args[gi].nterm, nullptr, 1, cus, st);
-            if (rc != KFC_OK) return rc;
-        }
-        return int(KFC_OK);
-    };
-    return run_pair_groups(op->ctx, pair_tune_key(op, sizeof(T), slabs), st, all);
+    auto one = [&](const PairShape& sh, int gi) { return launch_group<T>(sh, args[gi], args[gi].nterm, nullptr, 1, cus, st); };
+    return run_pair_groups(op->ctx, pair_tune_key(op, sizeof(T), slabs), int(ng), st, one);
 }
 
 template int launch_pairs<double>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t);
