@@ -714,6 +714,193 @@ void k_dense_pair_wave_batch(const PairArgs<T>* __restrict__ tab) {
     dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
 }
 
+// Groups whose p axis is strided (L > 1: axes before p) interleave neighbouring slabs
+// element by element, so a wave that gathers one slab moves 8-byte pieces.  Here a block
+// of WS waves takes WS neighbouring slabs l0 .. l0 + WS - 1 of one (m, r, column):
+//   load      the block reads X[l0 .. l0 + WS - 1, i, j], WS consecutive entries per
+//             (i, j) (32 bytes in FP64, 16 in FP32, with WS = 4), into one LDS tile
+//             per slab (rows padded to whole tiles, zero outside dp x dq);
+//   compute   wave w applies slab l0 + w with the register-chained GEMMs above, its X
+//             fragments read from its tile, and writes the result back over the tile;
+//   epilogue  the block forms Y = D .* X (or Y) + result in the same segments.
+template <class T, int PI, int PJ, int NT, int WS>
+__device__ inline void dense_pair_chunks_staged(const PairArgs<T>& a, int64_t c0, int64_t cstep) {
+    using v4 = typename V4<T>::type;
+    constexpr int NTI = (PI + 15) / 16, NTJ = (PJ + 15) / 16, KSJ = PJ / 4;
+    constexpr int LDA = lds_ld<T, PI>(), LDB = lds_ld<T, PJ>();
+    constexpr int MSA = lds_size<T, PI>(), MSB = lds_size<T, PJ>();
+    constexpr int PADA = 16 * NTI - LDA + 1, PADB = 16 * NTJ - LDB + 1;
+    constexpr int LDX = 16 * NTI, MSX = LDX * PJ;          // a slab's tile: LDX x PJ
+    constexpr int NTH = 64 * WS, NPASS = (PI * PJ + 63) / 64;   // 64 (i, j) pairs a pass
+    // passes a batch: every load of a phase in flight at once up to 24 (one round trip
+    // each way at 40 x 40)
+    constexpr int NB = NPASS < 24 ? NPASS : 24;
+    __shared__ T As[NT * MSA + PADA];
+    __shared__ T Bs[NT * MSB + PADB];
+    __shared__ T Xs[WS * MSX];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int dp = a.dp, dq = a.dq, nterm = a.nterm;
+    for (int e = tid; e < NT * MSA + PADA; e += NTH) {
+        const int t = e / MSA, f = e - t * MSA, i = f % LDA, k = f / LDA;
+        As[e] = (t < nterm && i < dp && k < dp) ? a.A[t][i + (int64_t)k * dp] : T(0);
+    }
+    for (int e = tid; e < NT * MSB + PADB; e += NTH) {
+        const int t = e / MSB, f = e - t * MSB, i = f % LDB, k = f / LDB;
+        Bs[e] = (t < nterm && i < dq && k < dq) ? a.c[t] * a.B[t][i + (int64_t)k * dq] : T(0);
+    }
+    for (int e = tid; e < WS * MSX; e += NTH) Xs[e] = T(0);   // the padding stays zero
+    __syncthreads();
+    // chunk counts fit 32 bits (the launcher checks N * sizeof(T) < 2^31)
+    const uint32_t L = uint32_t(a.L), M = uint32_t(a.M), R = uint32_t(a.R);
+    const uint32_t nlc = (L + WS - 1) / WS;                   // chunks of WS slabs along l
+    const uint32_t nchunk = nlc * M * R * uint32_t(a.nslab / (a.L * a.M * a.R));
+    // the cooperative phases: lane group of WS threads <-> l, pairs (i, j) over the rest
+    const int wl = tid % WS, p0 = tid / WS, npair = dp * dq;
+    const int di = 64 % dp, dj = 64 / dp;
+    for (uint32_t c = uint32_t(c0); c < nchunk; c += uint32_t(cstep)) {
+        const uint32_t lc = c % nlc, rest = c / nlc;
+        const uint32_t m = rest % M, rr = (rest / M) % R, col = rest / (M * R);
+        const uint32_t l0 = lc * WS;
+        const int64_t cbase = int64_t(col) * a.ld + l0 + int64_t(m) * L * a.dp + int64_t(rr) * a.sr;   // slab l0's (0, 0)
+        const bool lok = l0 + wl < L;
+        // offsets below 2^31 (the launcher checks N * sizeof(T)); pass u of a batch covers
+        // pair p0 + 64 (ub + u)
+        const T* __restrict__ xc = a.X + cbase;
+        T* __restrict__ yc = a.Y + cbase;
+        const T* __restrict__ dc = a.D ? a.D + (cbase - col * a.ld) : nullptr;
+        const uint32_t sp = uint32_t(a.sp), sq = uint32_t(a.sq);
+        // load: X into the tiles
+        for (int ub = 0; ub < NPASS; ub += NB) {
+            T v[NB];
+            int i = (p0 + 64 * ub) % dp, j = (p0 + 64 * ub) / dp;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                v[u] = (lok && p0 + 64 * (ub + u) < npair) ? xc[wl + uint32_t(i) * sp + uint32_t(j) * sq] : T(0);
+                i += di;
+                j += dj;
+                if (i >= dp) {
+                    i -= dp;
+                    ++j;
+                }
+            }
+            i = (p0 + 64 * ub) % dp;
+            j = (p0 + 64 * ub) / dp;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (p0 + 64 * (ub + u) < npair) Xs[wl * MSX + i + j * LDX] = v[u];
+                i += di;
+                j += dj;
+                if (i >= dp) {
+                    i -= dp;
+                    ++j;
+                }
+            }
+        }
+        __syncthreads();
+        // compute: wave w, slab l0 + w
+        if (l0 + w < L) {
+            T* xs = Xs + w * MSX;
+            T xa[NTI][KSJ];
+#pragma unroll
+            for (int ti = 0; ti < NTI; ++ti)
+#pragma unroll
+                for (int ks = 0; ks < KSJ; ++ks) xa[ti][ks] = xs[(16 * ti + r16) + (4 * ks + k4) * LDX];
+            // every wave has its fragments before any result lands on its own tile
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int tp = 0; tp < NTJ; ++tp) {
+                v4 acc[NTI];
+#pragma unroll
+                for (int tc = 0; tc < NTI; ++tc) acc[tc] = v4{0, 0, 0, 0};
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if (t < nterm) {
+                        const T* At = As + t * MSA;
+                        const T* Bt = Bs + t * MSB;
+#pragma unroll
+                        for (int ti = 0; ti < NTI; ++ti) {
+                            v4 u = v4{0, 0, 0, 0};
+#pragma unroll
+                            for (int ks = 0; ks < KSJ; ++ks)
+                                u = mfma(xa[ti][ks], Bt[(16 * tp + r16) + (4 * ks + k4) * LDB], u);
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                if (16 * ti + out_row(T(0), 0, q) < PI)
+#pragma unroll
+                                    for (int tc = 0; tc < NTI; ++tc)
+                                        acc[tc] = mfma(u[q], At[(16 * tc + r16) + (16 * ti + out_row(T(0), k4, q)) * LDA],
+                                                       acc[tc]);
+                            }
+                        }
+                    }
+                }
+                // the result tile back over the slab's tile (entries inside dp x dq only)
+#pragma unroll
+                for (int tc = 0; tc < NTI; ++tc)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int i = 16 * tc + r16, j = 16 * tp + out_row(T(0), k4, q);
+                        if (i < dp && j < dq) xs[i + j * LDX] = acc[tc][q];
+                    }
+            }
+        }
+        __syncthreads();
+        // epilogue: Y = D .* X (or Y) + result, a batch's loads before its stores
+        for (int ub = 0; ub < NPASS; ub += NB) {
+            T v[NB];
+            int i = (p0 + 64 * ub) % dp, j = (p0 + 64 * ub) / dp;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (lok && p0 + 64 * (ub + u) < npair) {
+                    const uint32_t off = wl + uint32_t(i) * sp + uint32_t(j) * sq;
+                    v[u] = dc ? dc[off] * xc[off] : yc[off];
+                }
+                i += di;
+                j += dj;
+                if (i >= dp) {
+                    i -= dp;
+                    ++j;
+                }
+            }
+            i = (p0 + 64 * ub) % dp;
+            j = (p0 + 64 * ub) / dp;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (lok && p0 + 64 * (ub + u) < npair)
+                    yc[wl + uint32_t(i) * sp + uint32_t(j) * sq] = v[u] + Xs[wl * MSX + i + j * LDX];
+                i += di;
+                j += dj;
+                if (i >= dp) {
+                    i -= dp;
+                    ++j;
+                }
+            }
+        }
+        __syncthreads();   // the tiles are free for the next chunk
+    }
+}
+
+// slabs (and waves) a block: 4, two blocks a compute unit at 40 x 32 (8 spills FP32
+// registers at the four waves per SIMD it would need)
+template <class T> constexpr int staged_ws() { return 4; }
+template <class T> constexpr int staged_wpe() { return 2; }
+
+template <class T, int PI, int PJ, int NT>
+__global__ __launch_bounds__(64 * staged_ws<T>()) __attribute__((amdgpu_waves_per_eu(staged_wpe<T>())))
+void k_dense_pair_staged(const PairArgs<T> a) {
+    dense_pair_chunks_staged<T, PI, PJ, NT, staged_ws<T>()>(a, xcd_block(int(blockIdx.x), int(gridDim.x)), gridDim.x);
+}
+
+template <class T, int PI, int PJ, int NT>
+__global__ __launch_bounds__(64 * staged_ws<T>()) __attribute__((amdgpu_waves_per_eu(staged_wpe<T>())))
+void k_dense_pair_staged_batch(const PairArgs<T>* __restrict__ tab) {
+    const PairArgs<T> a = tab[blockIdx.y];
+    dense_pair_chunks_staged<T, PI, PJ, NT, staged_ws<T>()>(a, xcd_block(int(blockIdx.x), int(gridDim.x)), gridDim.x);
+}
+
 // Blocks of kernel K that fit on the device at once (occupancy x compute units): a
 // chain-kernel grid is capped there, so that every block works through many slabs and
 // its setup (A_t into LDS, the B_t fragments) is paid once per block, not per slab.
@@ -915,16 +1102,19 @@ bool pairs_same_structure(const kfc_op* a, const kfc_op* b) {
 
 // Launch shapes of the register-resident kernels (groups of at most two terms, extents
 // up to 48): the kernel family (0: register-chained, 1: register fragments with the
-// intermediate through LDS, 2: wave per slab) and the
+// intermediate through LDS, 2: wave per slab, 3: neighbouring slabs of a strided p axis
+// staged through LDS by the block, else as 2) and the
 // grid (0: one block per slab, at most 4096 blocks in all; k > 0: k times the blocks
 // that fit on the device at once, each looping over slabs; the wave-per-slab kernels
-// always take the latter, counting a wave per slab).  Which is fastest depends on the work in flight (a single operator's columns
-// or a whole lockstep sweep), so the launcher times them on the first application of
-// each shape and keeps the winner.
+// always take the latter, counting a wave per slab).  Which is fastest depends on the
+// work in flight (a single operator's columns or a whole lockstep sweep), so the
+// launcher times them per group on the first application of each structure and keeps
+// the winner.
 struct PairShape {
     int family, grid;
 };
-constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2}, {2, 1}, {2, 2}};
+constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2},
+                                     {2, 1}, {2, 2}, {3, 1}, {3, 2}};
 constexpr int kNumPairShapes = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
 
 template <class T, int P, int NT>
@@ -957,8 +1147,31 @@ int launch_reg_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T
 }
 
 template <class T, int PI, int PJ, int NT>
+int launch_staged_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus,
+                        hipStream_t st) {
+    constexpr int WS = staged_ws<T>(), threads = 64 * WS;
+    const int mult = std::max(1, sh.grid);
+    const int64_t ncols = a.nslab / (a.L * a.M * a.R);
+    const int64_t nchunk = (a.L + WS - 1) / WS * a.M * a.R * ncols;
+    auto grid_for = [&](auto kernel) -> unsigned {
+        const int64_t cap = resident_blocks(kernel, threads, cus) * mult;
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>(nchunk, (cap + n - 1) / n)));
+    };
+    if (tab) {
+        auto k = k_dense_pair_staged_batch<T, PI, PJ, NT>;
+        hipLaunchKernelGGL(k, dim3(grid_for(k), unsigned(n)), dim3(threads), 0, st, tab);
+    } else {
+        auto k = k_dense_pair_staged<T, PI, PJ, NT>;
+        hipLaunchKernelGGL(k, dim3(grid_for(k)), dim3(threads), 0, st, a);
+    }
+    return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
+}
+
+template <class T, int PI, int PJ, int NT>
 int launch_wave_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus,
                       hipStream_t st) {
+    // family 3 stages neighbouring slabs of a strided p axis through LDS
+    if (sh.family == 3 && a.L > 1) return launch_staged_group<T, PI, PJ, NT>(sh, a, tab, n, cus, st);
     constexpr int W = WAVE_W, threads = 64 * W;
     const int mult = std::max(1, sh.grid);
     auto grid_for = [&](auto kernel) -> unsigned {
@@ -1064,7 +1277,8 @@ PairTuneKey pair_tune_key(const kfc_op* op, size_t scalar, int64_t slabs_total) 
 
 // Applies the ng groups in order (the first writes Y, the rest add to it), each with its
 // tuned shape.  On the first application of a key every group times every candidate
-// (once to warm up, once with HIP events on the stream); a group run out of order leaves
+// (once to warm up, then the better of two runs timed with HIP events on the stream); a
+// group run out of order leaves
 // Y wrong, so the whole apply runs again with the picks afterwards.
 int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, int ng, hipStream_t st,
                     const std::function<int(const PairShape&, int)>& one) {
@@ -1097,20 +1311,22 @@ int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, int ng, hipStream_t st
         float best = 1e30f;
         for (int c = 0; c < kNumPairShapes && rc == KFC_OK; ++c) {
             if ((rc = one(kPairShapes[c], g)) != KFC_OK) break;
-            if ((rc = hipEventRecord(e0, st) == hipSuccess ? KFC_OK : KFC_EDEVICE) != KFC_OK) break;
-            if ((rc = one(kPairShapes[c], g)) != KFC_OK) break;
-            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                rc = KFC_EDEVICE;
-                break;
-            }
-            float ms = 0;
-            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                rc = KFC_EDEVICE;
-                break;
-            }
-            if (ms < best) {
-                best = ms;
-                pick[g] = c;
+            for (int rep = 0; rep < 2 && rc == KFC_OK; ++rep) {   // the better of two timed runs
+                if ((rc = hipEventRecord(e0, st) == hipSuccess ? KFC_OK : KFC_EDEVICE) != KFC_OK) break;
+                if ((rc = one(kPairShapes[c], g)) != KFC_OK) break;
+                if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = KFC_EDEVICE;
+                    break;
+                }
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = KFC_EDEVICE;
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    pick[g] = c;
+                }
             }
         }
     }

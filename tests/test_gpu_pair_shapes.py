@@ -6,7 +6,7 @@ and grids, chosen per operator structure by timing on first use.  Each shape is
 pinned here with KFC_PAIR_SHAPE and checked on its own: the single-operator apply
 against the oracle's per-term chain (FP64 and FP32), and the batched apply of the
 lockstep sweep against the one-point solver.  The operators cover unit and
-strided p axes, extents that are not multiples of 4 or 16, one and two terms per
+strided p axes (the strided ones also through the LDS-staged kernel), extents that are not multiples of 4 or 16, one and two terms per
 group (shared factors merged on the host), and every tile size up to 48."""
 import numpy as np
 import pytest
@@ -17,7 +17,7 @@ from tests.helpers import to_kfc, to_oracle
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-N_SHAPES = 9           # kPairShapes in dense_pair.hip
+N_SHAPES = 11          # kPairShapes in dense_pair.hip
 TOL64 = 1e-13
 TOL32 = 4e-5
 
@@ -51,7 +51,7 @@ def _pair_operator(dims, pairs, seed):
     return to_kfc(dims, terms, constant=0.25), to_oracle(dims, terms, constant=0.25)
 
 
-CASES = [([40, 40, 30], [(0, 1), (0, 2)]),      # the zero-pi parent's structure
+CASES = [([40, 40, 30], [(0, 1), (1, 2)]),      # the zero-pi parent's structure (strided p in (1, 2))
          ([3, 37, 29], [(1, 2)]),               # strided p axis, ragged extents
          ([45, 6, 33], [(0, 2), (1, 2)]),        # 48-wide tiles
          ([13, 10, 7, 2], [(0, 1), (2, 3)]),     # 16-wide tiles, four axes
@@ -73,7 +73,7 @@ def test_pair_shape_apply(monkeypatch, shape, case):
         assert _rel(_run(H, X, "f32"), Yo) < TOL32, (shape, dims, ncols)
 
 
-@pytest.mark.parametrize("shape", [0, 7, 8])
+@pytest.mark.parametrize("shape", [0, 7, 8, 9, 10])
 def test_pair_shape_lockstep(monkeypatch, shape):
     """The batched (one launch for every point) form of the shape in a lockstep sweep:
     each point's pairs match the one-point solver and meet the tolerance."""
