@@ -105,6 +105,92 @@ def calibrate_copy(gib=1, reps=5):
                       "bytes_written_per_launch": n * 8, "launches": reps}))
 
 
+def summarize_pmc(args):
+    """HBM bytes per launch of one kernel from two rocprofv3 --pmc runs of the same
+    command (FETCH_SIZE, then WRITE_SIZE; counter_collection CSVs, KiB per dispatch),
+    each scaled by what the counter reads for the elementwise copy of known size
+    (the same two counters over `bench.py --calibrate-copy`).  Written to a profiles
+    JSON stamped with the commit of this tree (the code that was measured)."""
+    import csv
+    import subprocess
+
+    def kib(path, counter, want):
+        tot, n = 0.0, 0
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter and want(r["Kernel_Name"]):
+                tot += float(r["Counter_Value"])
+                n += 1
+        return tot, n
+
+    is_copy = lambda k: "add" in k.lower() and "fill" not in k.lower()   # torch.add(x, 0, out=y)
+    is_kernel = lambda k: args.pmc_kernel in k
+    cf, ncf = kib(args.pmc_copy_fetch, "FETCH_SIZE", is_copy)
+    cw, ncw = kib(args.pmc_copy_write, "WRITE_SIZE", is_copy)
+    copy_bytes = (1 << 30)
+    # the copy reads and writes exactly 1 GiB per launch: bytes per KiB of counter
+    fscale = copy_bytes / (cf / ncf * 1024.0)
+    wscale = copy_bytes / (cw / ncw * 1024.0)
+    f, nf = kib(args.pmc_fetch, "FETCH_SIZE", is_kernel)
+    w, nw = kib(args.pmc_write, "WRITE_SIZE", is_kernel)
+    fetch = f / nf * 1024.0 * fscale
+    write = w / nw * 1024.0 * wscale
+    try:
+        commit = subprocess.run(["git", "-C", ROOT, "rev-parse", "--short", "HEAD"], capture_output=True,
+                                text=True).stdout.strip() or "unknown"
+    except OSError:
+        commit = "unknown"
+    out = {"kernel": args.pmc_kernel, "workload": args.pmc_workload, "command": args.pmc_command,
+           "bytes_per_launch": int(round(fetch + write)), "fetch_bytes": int(round(fetch)),
+           "write_bytes": int(round(write)), "launches_counted": nf, "fetch_scale": round(fscale, 4),
+           "write_scale": round(wscale, 4), "commit": commit}
+    with open(args.pmc_out, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
+def pairs_bench(args):
+    """The dense kept-basis apply of the zero-pi parent [40, 40, 30] on 64 columns,
+    FP64 and FP32 (the tuned launch shape per axis-pair group), timed with HIP events
+    on the library's stream: the MFMA roofline of the lockstep sweep's kernels."""
+    import torch
+
+    from kfc import configs
+    from kfc.sweeps import ZeroPiSweep
+    sw = ZeroPiSweep(contexts=1)
+    H, _ = sw.parent(configs.zero_pi_theta(50_000))
+    _, flops_col = H.work()
+    ncol, reps = 64, max(1, args.steps) * 20
+    st = H.ctx.torch_stream()
+    res = {}
+    for prec, dt, peak in (("f64", torch.float64, 78.6), ("f32", torch.float32, 157.3)):
+        X = torch.randn(ncol, H.N, dtype=dt, device="cuda:0")
+        Y = torch.empty_like(X)
+        for _ in range(max(1, args.warmup) * 3):
+            H.mul(Y, X, prec)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            H.mul(Y, X, prec)
+        e1.record(st)
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / reps / 1e3
+        rate = flops_col * ncol / t / 1e12
+        res[prec] = {"us_per_apply": round(t * 1e6, 1), "tflops": round(rate, 2), "peak": peak,
+                     "frac": round(rate / peak, 4)}
+        log(f"[pairs] {prec}: {t * 1e6:.1f} us per apply of {ncol} columns, {rate:.2f} TFLOP/s")
+    out = {"metric": "dense kept-basis apply TFLOP/s (zero-pi parent [40,40,30], 64 columns)",
+           "value": res["f64"]["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+           "ms_per_step": round(res["f64"]["us_per_apply"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded circuit parameters, random block)",
+           "config": {"workload": "zero_pi_parent_pairs", "dims": [40, 40, 30], "columns": ncol,
+                      "flops_per_column": flops_col},
+           "roofline": {"bound": "mfma", "achieved": res["f64"]["tflops"], "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": res["f64"]["frac"], "traffic": None},
+           "f32": res["f32"]}
+    print(json.dumps(out))
+
+
 def zero_pi_runs(args):
     """The zero-pi sweep (parent dims [40, 40, 30], LOBPCG lowest 16, tol 1e-8) at
     args.test_points seeded points, each path timed end to end (operators, seeds,
@@ -488,7 +574,7 @@ def main():
     ap.add_argument("--verbosity", type=int, default=20, help="Lanczos progress line every this many cycles")
     ap.add_argument("--workload", default="three_fluxonia",
                     choices=["three_fluxonia", "zero_pi_sweep", "transmon_sweep", "ports", "seeds", "drive",
-                             "sternheimer"])
+                             "sternheimer", "pairs"])
     ap.add_argument("--seed-n", type=int, default=256, help="seeds, drive and sternheimer workloads: points per mode")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -504,7 +590,19 @@ def main():
     ap.add_argument("--sweep-mode", default="mixed", choices=["f64", "mixed"])
     ap.add_argument("--leaf-mode", default="mixed", choices=["f64", "mixed"], help="transmon sweep leaf solves")
     ap.add_argument("--leaf-method", default="chebyshev", choices=["chebyshev", "lanczos"])
+    ap.add_argument("--summarize-pmc", action="store_true",
+                    help="write a profiles JSON of one kernel's HBM bytes per launch from rocprofv3 CSVs (no GPU)")
+    ap.add_argument("--pmc-kernel", default="k_tsmm")
+    ap.add_argument("--pmc-workload", default="")
+    ap.add_argument("--pmc-command", default="")
+    ap.add_argument("--pmc-fetch")
+    ap.add_argument("--pmc-write")
+    ap.add_argument("--pmc-copy-fetch")
+    ap.add_argument("--pmc-copy-write")
+    ap.add_argument("--pmc-out")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        return summarize_pmc(args)
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
@@ -522,6 +620,8 @@ def main():
         return sternheimer_bench(args)
     if args.workload == "transmon_sweep":
         return transmon_bench(args)
+    if args.workload == "pairs":
+        return pairs_bench(args)
     import numpy as np
     import torch
 
