@@ -114,13 +114,13 @@ def summarize_pmc(args):
     import csv
     import subprocess
 
-    def kib(path, counter, want):
-        tot, n = 0.0, 0
-        for r in csv.DictReader(open(path)):
-            if r["Counter_Name"] == counter and want(r["Kernel_Name"]):
-                tot += float(r["Counter_Value"])
-                n += 1
-        return tot, n
+    def kib(path, counter, want, stat="mean"):
+        vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(path))
+                if r["Counter_Name"] == counter and want(r["Kernel_Name"])]
+        if stat == "median":   # the typical launch: wider launches of the same kernel left out
+            vals.sort()
+            return vals[len(vals) // 2] * len(vals), len(vals)
+        return sum(vals), len(vals)
 
     is_copy = lambda k: "add" in k.lower() and "fill" not in k.lower()   # torch.add(x, 0, out=y)
     is_kernel = lambda k: args.pmc_kernel in k
@@ -130,8 +130,8 @@ def summarize_pmc(args):
     # the copy reads and writes exactly 1 GiB per launch: bytes per KiB of counter
     fscale = copy_bytes / (cf / ncf * 1024.0)
     wscale = copy_bytes / (cw / ncw * 1024.0)
-    f, nf = kib(args.pmc_fetch, "FETCH_SIZE", is_kernel)
-    w, nw = kib(args.pmc_write, "WRITE_SIZE", is_kernel)
+    f, nf = kib(args.pmc_fetch, "FETCH_SIZE", is_kernel, args.pmc_stat)
+    w, nw = kib(args.pmc_write, "WRITE_SIZE", is_kernel, args.pmc_stat)
     fetch = f / nf * 1024.0 * fscale
     write = w / nw * 1024.0 * wscale
     try:
@@ -142,7 +142,7 @@ def summarize_pmc(args):
     out = {"kernel": args.pmc_kernel, "workload": args.pmc_workload, "command": args.pmc_command,
            "bytes_per_launch": int(round(fetch + write)), "fetch_bytes": int(round(fetch)),
            "write_bytes": int(round(write)), "launches_counted": nf, "fetch_scale": round(fscale, 4),
-           "write_scale": round(wscale, 4), "commit": commit}
+           "write_scale": round(wscale, 4), "statistic": args.pmc_stat + " launch", "commit": commit}
     with open(args.pmc_out, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out))
@@ -600,6 +600,7 @@ def main():
     ap.add_argument("--pmc-copy-fetch")
     ap.add_argument("--pmc-copy-write")
     ap.add_argument("--pmc-out")
+    ap.add_argument("--pmc-stat", default="mean", choices=["mean", "median"])
     args = ap.parse_args()
     if args.summarize_pmc:
         return summarize_pmc(args)

@@ -573,7 +573,7 @@ constexpr int lds_size() { return lds_ld<T, P>() * lds_ld<T, P>(); }
 // GEMM2's A operand (row r, contraction index out_row(k, q)) as it stands, and GEMM2's
 // result holds 16 consecutive p-axis entries per register: the loads of X and of the
 // epilogue and the stores all move whole 128-byte rows.
-template <class T, int PI, int PJ, int NT, int W>
+template <class T, int PI, int PJ, int NT, int W, bool DIAG>
 __device__ inline void dense_pair_slabs_wave(const PairArgs<T>& a, int64_t w0, int64_t wstep) {
     using v4 = typename V4<T>::type;
     constexpr int NTI = (PI + 15) / 16, NTJ = (PJ + 15) / 16, KSJ = PJ / 4;
@@ -632,13 +632,18 @@ __device__ inline void dense_pair_slabs_wave(const PairArgs<T>& a, int64_t w0, i
             // first group, for D .* X; D is shared by every column and stays in L2, so it
             // is read at the end) or Y (groups accumulating into it), at
             // Y_s[16 tc + r16, 16 tp + out_row(k4, q)]
-            const bool diag = a.D != nullptr;
+            // In FP64 the entry X_s[16 tc + r16, 16 tp + k4 + 4 q] of the result's layout is
+            // the GEMM1 fragment xa[tc][4 tp + q] this lane already holds, so D .* X needs
+            // no second read of X (FP32 interleaves its rows differently).
+            constexpr bool XREG = DIAG && sizeof(T) == 8;
             T pv[NTI][4];
+            if constexpr (!XREG) {
 #pragma unroll
-            for (int tc = 0; tc < NTI; ++tc)
+                for (int tc = 0; tc < NTI; ++tc)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    pv[tc][q] = bload(T(0), diag ? xr : yr, poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq);
+                    for (int q = 0; q < 4; ++q)
+                        pv[tc][q] = bload(T(0), DIAG ? xr : yr, poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq);
+            }
             v4 acc[NTI];
 #pragma unroll
             for (int tc = 0; tc < NTI; ++tc) acc[tc] = v4{0, 0, 0, 0};
@@ -666,12 +671,18 @@ __device__ inline void dense_pair_slabs_wave(const PairArgs<T>& a, int64_t w0, i
                     }
                 }
             }
+            if constexpr (XREG) {
+#pragma unroll
+                for (int tc = 0; tc < NTI; ++tc)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) pv[tc][q] = (4 * tp + q < KSJ) ? xa[tc][(4 * tp + q) % KSJ] : T(0);
+            }
             T dv[NTI][4];
 #pragma unroll
             for (int tc = 0; tc < NTI; ++tc)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    dv[tc][q] = diag ? bload(T(0), dr, poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq) : T(1);
+                    dv[tc][q] = DIAG ? bload(T(0), dr, poff(tc) + uint32_t(16 * tp + out_row(T(0), k4, q)) * bsq) : T(1);
 #pragma unroll
             for (int tc = 0; tc < NTI; ++tc)
 #pragma unroll
@@ -697,21 +708,23 @@ __device__ inline int64_t xcd_block(int b, int g) {
     return int64_t(off) + k;
 }
 
-template <class T, int PI, int PJ, int NT>
+// DIAG: the group writes Y = D .* X + ... (the first group of an operator); otherwise it
+// adds to Y
+template <class T, int PI, int PJ, int NT, bool DIAG>
 __global__ __launch_bounds__(64 * WAVE_W) __attribute__((amdgpu_waves_per_eu(2)))
 void k_dense_pair_wave(const PairArgs<T> a) {
     const int64_t lb = xcd_block(int(blockIdx.x), int(gridDim.x));
-    dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
+    dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W, DIAG>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
 }
 
-template <class T, int PI, int PJ, int NT>
+template <class T, int PI, int PJ, int NT, bool DIAG>
 __global__ __launch_bounds__(64 * WAVE_W) __attribute__((amdgpu_waves_per_eu(2)))
 void k_dense_pair_wave_batch(const PairArgs<T>* __restrict__ tab) {
     const PairArgs<T> a = tab[blockIdx.y];
     // (gridDim.x * blockIdx.y + b) % 8 is constant over a residue class of b, so the same
     // numbering groups one XCD's blocks in every row
     const int64_t lb = xcd_block(int(blockIdx.x), int(gridDim.x));
-    dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
+    dense_pair_slabs_wave<T, PI, PJ, NT, WAVE_W, DIAG>(a, lb * WAVE_W, int64_t(gridDim.x) * WAVE_W);
 }
 
 // Groups whose p axis is strided (L > 1: axes before p) interleave neighbouring slabs
@@ -1179,11 +1192,12 @@ int launch_wave_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<
         const int64_t need = (a.nslab + W - 1) / W;   // blocks that give every slab its own wave
         return unsigned(std::max<int64_t>(1, std::min<int64_t>(need, (cap + n - 1) / n)));
     };
+    // every problem of a batch is the same group, so a.D decides for all of them
     if (tab) {
-        auto k = k_dense_pair_wave_batch<T, PI, PJ, NT>;
+        auto k = a.D ? k_dense_pair_wave_batch<T, PI, PJ, NT, true> : k_dense_pair_wave_batch<T, PI, PJ, NT, false>;
         hipLaunchKernelGGL(k, dim3(grid_for(k), unsigned(n)), dim3(threads), 0, st, tab);
     } else {
-        auto k = k_dense_pair_wave<T, PI, PJ, NT>;
+        auto k = a.D ? k_dense_pair_wave<T, PI, PJ, NT, true> : k_dense_pair_wave<T, PI, PJ, NT, false>;
         hipLaunchKernelGGL(k, dim3(grid_for(k)), dim3(threads), 0, st, a);
     }
     return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
