@@ -204,13 +204,17 @@ def zero_pi_runs(args):
     * lockstep_fp64: every point's LOBPCG step in one launch per kernel;
     * lockstep_mixed_learned: lockstep, FP32 bulk and FP64 refinement, started from
       the warm-start network's prediction (trained here on args.train_points
-      points' plain-solver eigenpairs).
+      points' plain-solver eigenpairs);
+    * lockstep_fp64_subspace / lockstep_mixed_subspace: lockstep, every point
+      started from the lowest Ritz vectors of its operator on the learned subspace
+      (the leading args.subspace_basis principal directions of the same training
+      eigenvectors), the Rayleigh-Ritz step on the device, timed with the solve.
 
     Returns the record and the lockstep run's outputs."""
     import numpy as np
 
     from kfc import configs
-    from kfc.sweeps import WarmStart, ZeroPiSweep, path_order, subspace_capture
+    from kfc.sweeps import LearnedSubspace, WarmStart, ZeroPiSweep, path_order, subspace_capture
 
     sw = ZeroPiSweep(contexts=args.contexts)   # lockstep runs on the first; the training solves use them all
     m = sw.m
@@ -243,15 +247,26 @@ def zero_pi_runs(args):
     res["lockstep_fp64"] = summary(lock, t)
     res["lockstep_fp64"]["seconds_apply_gpu"] = round(float(sum(o[2]["seconds_apply_gpu"] for o in lock)), 4)
     log(f"[zero-pi] lockstep FP64: {t:.2f} s, {res['lockstep_fp64']['eigenpairs_per_s']} pairs/s")
-    # the warm-start network, trained on the plain solver's lowest m + g pairs
+    # the warm-start network, trained on the plain solver's lowest m + g pairs (the
+    # lockstep solver with one guard column, so that m + g pairs fit its block)
     nev = sw.width(40 * 40 * 30)
     t0 = time.perf_counter()
-    gen, _ = sw.run_batched(train, mode="f64", nev=nev, lockstep=True)
+    gen, _ = sw.run_batched(train, mode="f64", nev=nev, lockstep=True, guard=1)
+    t_gen = time.perf_counter() - t0
     frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
     ws = WarmStart(sw, nbasis=args.nbasis, seed=1)
     loss = ws.train(train, frames, steps=args.train_steps, lr=1e-3)
     t_train = time.perf_counter() - t0
-    log(f"[zero-pi] warm start: {len(train) * nev} training pairs, loss {loss[0]:.4g} -> {loss[1]:.4g}, {t_train:.1f} s")
+    log(f"[zero-pi] warm start: {len(train) * nev} training pairs ({t_gen:.1f} s), loss {loss[0]:.4g} -> {loss[1]:.4g}, "
+        f"{t_train:.1f} s")
+    t0 = time.perf_counter()
+    sub = LearnedSubspace(sw, nbasis=args.subspace_basis).fit(frames)
+    t_sub = time.perf_counter() - t0
+    sw.run_batched(test[:2], mode="f64", subspace=sub)    # warm-up at the subspace width
+    out, t = sw.run_batched(test, mode="f64", subspace=sub)
+    res["lockstep_fp64_subspace"] = summary(out, t)
+    out, t = sw.run_batched(test, mode="mixed", subspace=sub)
+    res["lockstep_mixed_subspace"] = summary(out, t)
     # the two effects apart: FP64 from the prediction, mixed from the product seed
     out, t = sw.run_batched(test, mode="f64", x0_fn=ws.x0_fn, lockstep=True)
     res["lockstep_fp64_learned"] = summary(out, t)
@@ -259,7 +274,8 @@ def zero_pi_runs(args):
     res["lockstep_mixed"] = summary(out, t)
     learned, t = sw.run_batched(test, mode="mixed", x0_fn=ws.x0_fn, lockstep=True)
     res["lockstep_mixed_learned"] = summary(learned, t)
-    for k in ("lockstep_fp64_learned", "lockstep_mixed", "lockstep_mixed_learned"):
+    for k in ("lockstep_fp64_learned", "lockstep_mixed", "lockstep_mixed_learned", "lockstep_fp64_subspace",
+              "lockstep_mixed_subspace"):
         log(f"[zero-pi] {k}: {res[k]['seconds']:.2f} s, {res[k]['eigenpairs_per_s']} pairs/s, "
             f"mean iterations {res[k]['mean_iterations']}")
     base = res["plain_fp64_cold"]["eigenpairs_per_s"]
@@ -283,6 +299,9 @@ def zero_pi_runs(args):
                           "outputs": f"{ws.nb} x {ws.m}", "initial_loss": float(loss[0]), "final_loss": float(loss[1]),
                           "seconds_data_and_training": round(t_train, 2),
                           "mean_subspace_capture": round(float(np.mean(caps)), 4)},
+           "learned_subspace": {"basis": args.subspace_basis, "frame_rows": int(sub.B.shape[0]),
+                                "training_eigenpairs": len(train) * nev, "seconds_fit": round(t_sub, 2),
+                                "singular_value_ratio_last": float(sub.singular_values[-1] / sub.singular_values[0])},
            "roofline_dense_pairs": {"bound": "mfma", "kernel": "dense kept-basis pairs (lockstep FP64 applies)",
                                     "achieved": round(rate, 3) if rate else None, "peak": 78.6, "unit": "TFLOP/s",
                                     "frac": round(rate / 78.6, 4) if rate else None,
@@ -584,9 +603,10 @@ def main():
                     help="zero-pi sweep: batched solves as independent solves on contexts instead of lockstep")
     ap.add_argument("--test-points", type=int, default=64)
     ap.add_argument("--plain-points", type=int, default=8)
-    ap.add_argument("--train-points", type=int, default=256)
+    ap.add_argument("--train-points", type=int, default=1000)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--nbasis", type=int, default=64, help="frame basis size of the warm start")
+    ap.add_argument("--subspace-basis", type=int, default=256, help="directions of the learned subspace start")
     ap.add_argument("--sweep-mode", default="mixed", choices=["f64", "mixed"])
     ap.add_argument("--leaf-mode", default="mixed", choices=["f64", "mixed"], help="transmon sweep leaf solves")
     ap.add_argument("--leaf-method", default="chebyshev", choices=["chebyshev", "lanczos"])

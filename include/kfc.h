@@ -314,6 +314,20 @@ int kfc_lobpcg_lockstep(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard,
                         int32_t precond, double eta, int32_t seed_kind, const double* const* X0s, uint64_t seed,
                         int32_t mode, double* E, double* const* Vs, kfc_stats* stats);
 
+/* The lockstep sweep started from a learned subspace.  No counterpart in the
+ * reference: it supplies the X0 of src/lobpcg.jl:217 (src/materialize.jl:656-666)
+ * for every sweep point at once.  Point i starts from the lowest m + g Ritz
+ * vectors of H_i on span(P_i B C_i): B (DEVICE, nf x nb, column-major) spans a
+ * parameter-independent frame of the parent basis, P_i puts frame row f at
+ * parent row rowmaps[i][f] (dropped when -1; host arrays of nf entries, no two
+ * rows alike), C_i (host, nb x ns) selects ns directions, or C = NULL for all nb
+ * (ns is then nb).  The start is perturbed as a warm start; everything else as
+ * kfc_lobpcg_lockstep. */
+int kfc_lobpcg_lockstep_subspace(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard, double tol,
+                                 int64_t maxiter, int32_t precond, double eta, const double* B, int64_t nf,
+                                 int32_t nb, const int64_t* const* rowmaps, const double* const* C, int32_t ns,
+                                 uint64_t seed, int32_t mode, double* E, double* const* Vs, kfc_stats* stats);
+
 /* Replaces solve(::ChebyshevFilter, H, m) (src/chebyshev.jl:310-406).
  * ortho: 0 = :svqb, 1 = :cholqr; bound: a finite number, or NaN for :gershgorin. */
 int kfc_chebyshev(kfc_op* op, int64_t m, int32_t degree, int64_t extra, int64_t maxiter,

@@ -959,6 +959,136 @@ struct Phase {
     }
 };
 
+// The start blocks from a learned subspace (the subspace seed of
+// kfc_lobpcg_lockstep_subspace): point i's block spans S_i = P_i B C_i, where B
+// (nf x nb) is a basis of a parameter-independent frame of the parent basis, P_i
+// places frame row f at parent row rowmap_i[f] (none when -1) and C_i (nb x ns)
+// picks ns directions (the identity when absent).  The start block is the lowest
+// w Ritz vectors of H_i on span(S_i): K = S^T H S and G = S^T S (FP64 Grams on
+// the device), the projected problem on host threads (the generalized form only
+// when G is not the identity, i.e. when frame rows were dropped).
+struct SubspaceSeed {
+    const double* B = nullptr;   // device, nf x nb column-major
+    int64_t nf = 0;
+    int nb = 0, ns = 0;
+    const int64_t* const* rowmaps = nullptr;   // host, n arrays of nf entries
+    const double* const* C = nullptr;          // host, n arrays nb x ns, or null
+};
+
+__global__ __launch_bounds__(256) void k_frame_scatter(const double* __restrict__ BC, int64_t ldb, int64_t nf, int ns,
+                                                       const int64_t* __restrict__ rowmap, double* __restrict__ X,
+                                                       int64_t ldx) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nf * ns; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = e % nf, j = e / nf;
+        const int64_t r = rowmap[f];
+        if (r >= 0) X[r + j * ldx] = BC[f + j * ldb];
+    }
+}
+
+// S_i into X (N x ns, leading dimension N): zero, then the frame rows scattered
+int subspace_block(Work& w, const SubspaceSeed& sub, int i, int64_t N, DevBuf& BC, DevBuf& map, double* X) {
+    kfc_ctx* ctx = w.ctx;
+    hipStream_t st = ctx->stream;
+    const double* src = sub.B;
+    if (sub.C && sub.C[i]) {
+        std::vector<cd> M(size_t(sub.nb) * sub.ns);
+        for (size_t k = 0; k < M.size(); ++k) M[k] = sub.C[i][k];
+        KFC_HIP(ctx, BC.ensure(sizeof(double) * size_t(sub.nf) * sub.ns));
+        int rc = blas_gemm<double>(w, sub.B, sub.nf, sub.nb, M, sub.ns, BC.as<double>(), sub.nf, sub.nf, 0.0);
+        if (rc != KFC_OK) return rc;
+        src = BC.as<double>();
+    }
+    KFC_HIP(ctx, map.ensure(sizeof(int64_t) * size_t(sub.nf)));
+    KFC_HIP(ctx, hipMemcpyAsync(map.p, sub.rowmaps[i], sizeof(int64_t) * size_t(sub.nf), hipMemcpyHostToDevice, st));
+    KFC_HIP(ctx, hipMemsetAsync(X, 0, sizeof(double) * size_t(N) * sub.ns, st));
+    const int64_t tot = sub.nf * sub.ns;
+    const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>((tot + 255) / 256, 8192)));
+    hipLaunchKernelGGL(k_frame_scatter, dim3(gx), dim3(256), 0, st, src, sub.nf, sub.nf, sub.ns, map.as<int64_t>(), X, N);
+    KFC_HIP(ctx, hipGetLastError());
+    return KFC_OK;   // the caller's rowmap outlives the call, hence the copy
+}
+
+// The lowest W Ritz vectors of every point on its subspace, written as its start block
+// (the X slot of S64, point stride 3 W N).
+int subspace_starts(kfc_op* const* ops, int n, int W, const SubspaceSeed& sub, double* S0, Team& team) {
+    kfc_ctx* ctx = ops[0]->ctx;
+    const int64_t N = ops[0]->N;
+    const int ns = sub.ns;
+    const bool trace = std::getenv("KFC_TRACE") != nullptr;
+    auto t0 = clk::now();
+    Work w;
+    w.ctx = ctx;
+    DevBuf X, HX, BC, map;
+    KFC_HIP(ctx, X.ensure(sizeof(double) * size_t(N) * ns));
+    KFC_HIP(ctx, HX.ensure(sizeof(double) * size_t(N) * ns));
+    std::vector<std::vector<cd>> G(n), K(n);
+    // S^T S = (B C)^T (B C) when no frame row is dropped: formed once for such points
+    std::vector<char> full(n, 1);
+    for (int i = 0; i < n; ++i)
+        for (int64_t f = 0; f < sub.nf && full[i]; ++f) full[i] = sub.rowmaps[i][f] >= 0;
+    std::vector<cd> GB;
+    bool have_gb = false;
+    for (int i = 0; i < n; ++i) {
+        int rc = subspace_block(w, sub, i, N, BC, map, X.as<double>());
+        if (rc == KFC_OK) rc = op_apply(ops[i], X.p, HX.p, ns, N, KFC_PREC_F64, ctx->stream);
+        if (rc == KFC_OK && !full[i]) rc = blas_gram<double>(w, X.as<double>(), N, ns, X.as<double>(), N, ns, N, G[i]);
+        if (rc == KFC_OK) rc = blas_gram<double>(w, X.as<double>(), N, ns, HX.as<double>(), N, ns, N, K[i]);
+        if (rc != KFC_OK) return kfc_fail(ctx, rc, "subspace start of point " + std::to_string(i) + ": " + ops[i]->ctx->err);
+        if (full[i] && !sub.C && !have_gb) {   // B^T B, the same for every such point
+            rc = blas_gram<double>(w, sub.B, sub.nf, ns, sub.B, sub.nf, ns, sub.nf, GB);
+            if (rc != KFC_OK) return kfc_fail(ctx, rc, "subspace start: " + ctx->err);
+            have_gb = true;
+        }
+        if (full[i]) {
+            if (sub.C) {   // (B C_i)^T (B C_i), its own Gram
+                rc = blas_gram<double>(w, X.as<double>(), N, ns, X.as<double>(), N, ns, N, G[i]);
+                if (rc != KFC_OK) return kfc_fail(ctx, rc, "subspace start: " + ctx->err);
+            } else {
+                G[i] = GB;
+            }
+        }
+    }
+    const double t_dev = secs(t0);
+    std::vector<std::vector<double>> Y(n);
+    std::vector<char> ok(n, 1);
+    team.run(n, [&](int i) {
+        std::vector<double> g(size_t(ns) * ns), k(size_t(ns) * ns);
+        double off = 0;
+        for (int b = 0; b < ns; ++b)
+            for (int a = 0; a < ns; ++a) {
+                g[a + size_t(b) * ns] = 0.5 * (G[i][a + size_t(b) * ns].real() + G[i][b + size_t(a) * ns].real());
+                k[a + size_t(b) * ns] = 0.5 * (K[i][a + size_t(b) * ns].real() + K[i][b + size_t(a) * ns].real());
+                off = std::max(off, std::fabs(g[a + size_t(b) * ns] - (a == b ? 1.0 : 0.0)));
+            }
+        std::vector<double> lam, th;
+        if (off <= 1e-12) {   // orthonormal S: the standard problem
+            sym_eig_lowest(ns, W, k, lam, Y[i]);
+        } else {
+            std::vector<int> U(ns);
+            for (int a = 0; a < ns; ++a) U[a] = a;
+            ok[i] = ls_rr_solve(U, ns, W, false, g, k, th, Y[i]);
+        }
+    });
+    const double t_rr = secs(t0) - t_dev;
+    for (int i = 0; i < n; ++i)
+        if (!ok[i])
+            return kfc_fail(ctx, KFC_EBREAKDOWN, "subspace start of point " + std::to_string(i) +
+                                                     ": the subspace holds fewer than " + std::to_string(W) +
+                                                     " independent directions");
+    for (int i = 0; i < n; ++i) {
+        int rc = subspace_block(w, sub, i, N, BC, map, X.as<double>());
+        std::vector<cd> M(Y[i].size());
+        for (size_t k = 0; k < M.size(); ++k) M[k] = Y[i][k];
+        if (rc == KFC_OK)
+            rc = blas_gemm<double>(w, X.as<double>(), N, ns, M, W, S0 + size_t(i) * 3 * W * N, N, N, 0.0);
+        if (rc != KFC_OK) return kfc_fail(ctx, rc, "subspace start of point " + std::to_string(i) + ": " + ops[i]->ctx->err);
+    }
+    if (trace)
+        std::fprintf(stderr, "[lockstep] subspace starts: projections %.3f s, Rayleigh-Ritz %.3f s, blocks %.3f s\n", t_dev,
+                     t_rr, secs(t0) - t_dev - t_rr);
+    return KFC_OK;
+}
+
 template <class T>
 int alloc_phase_buffers(kfc_ctx* ctx, int n, int64_t N, int W, DevBuf* S, DevBuf* A) {
     const size_t bytes = sizeof(T) * size_t(n) * 3 * W * N;
@@ -971,7 +1101,7 @@ int alloc_phase_buffers(kfc_ctx* ctx, int n, int64_t N, int W, DevBuf* S, DevBuf
 
 int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol, int64_t maxiter, int32_t precond,
                  double eta, int32_t seed_kind, const double* const* X0s, uint64_t seed, int32_t mode, double* E,
-                 double* const* Vs, kfc_stats* stats) {
+                 double* const* Vs, kfc_stats* stats, const SubspaceSeed* sub = nullptr) {
     auto t0 = clk::now();
     // KFC_TRACE set: per-phase host timings on stderr
     const bool trace = std::getenv("KFC_TRACE") != nullptr;
@@ -998,7 +1128,7 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
         Team team0(std::max(1, std::min(16, int(std::thread::hardware_concurrency()))));
         // warm starts checked and host-made blocks (random, or a full warm start) formed on
         // host threads; product seeds are formed on the device below
-        team0.run(n, [&](int i) {
+        team0.run(sub ? 0 : n, [&](int i) {
             kfc_op* op = ops[i];
             const bool warm = X0s && X0s[i];
             if (warm) {
@@ -1019,7 +1149,9 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
             kfc_op* op = ops[i];
             pts[i].op = op;
             double* Xd = S64[0].as<double>() + size_t(i) * 3 * W * N;
-            if (devseed[i]) {
+            if (sub) {
+                npert[i] = W;   // formed below, perturbed as a warm start with its guards
+            } else if (devseed[i]) {
                 rc = lobpcg_seed_dev<double>(op, int(m), W, X0hs[i], (X0s && X0s[i]) ? nx0 : 0, true, false, seed + i, Xd,
                                              npert[i]);
                 if (rc != KFC_OK) return kfc_fail(ctx, rc, "instance " + std::to_string(i) + ": " + op->ctx->err);
@@ -1034,6 +1166,12 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                 if (rc == KFC_OK) rc = op_shift_inverse_dev(op, Di, dmin - 1, dD.as<double>() + size_t(n + i) * N);
                 if (rc != KFC_OK) return kfc_fail(ctx, rc, "instance " + std::to_string(i) + ": " + op->ctx->err);
             }
+        }
+        if (sub) {
+            const auto ts = clk::now();
+            rc = subspace_starts(ops, n, W, *sub, S64[0].as<double>(), team0);
+            if (rc != KFC_OK) return rc;
+            if (trace) std::fprintf(stderr, "[lockstep] subspace starts (%d directions): %.3f s\n", sub->ns, secs(ts));
         }
         // the seeded columns of every point perturbed on the device, one launch
         DevBuf dnp;
@@ -1200,7 +1338,7 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                     s.n_apply_calls = 0;
                     s.max_residual = fresh[p];
                     s.n_converged = int(m);
-                    s.warm = (X0s && X0s[p]) ? 1 : 0;
+                    s.warm = ((X0s && X0s[p]) || sub) ? 1 : 0;
                 }
             }
             KFC_HIP(ctx, hipStreamSynchronize(st));
@@ -1261,4 +1399,71 @@ extern "C" int kfc_lobpcg_lockstep(int32_t n, kfc_op* const* ops, int64_t m, int
         return kfc_fail(op0->ctx, KFC_EUNSUPPORTED, "kfc_lobpcg_lockstep: m + guard above 21 is not served");
     (void)hipSetDevice(op0->ctx->device);
     return lockstep_run(n, ops, m, guard, tol, maxiter, precond, eta, seed_kind, X0s, seed, mode, E, Vs, stats);
+}
+
+// Lockstep LOBPCG of a sweep started from a learned subspace (no counterpart in the
+// reference: it supplies the X0 of src/lobpcg.jl:217 for every sweep point).
+extern "C" int kfc_lobpcg_lockstep_subspace(int32_t n, kfc_op* const* ops, int64_t m, int64_t guard, double tol,
+                                            int64_t maxiter, int32_t precond, double eta, const double* B, int64_t nf,
+                                            int32_t nb, const int64_t* const* rowmaps, const double* const* C,
+                                            int32_t ns, uint64_t seed, int32_t mode, double* E, double* const* Vs,
+                                            kfc_stats* stats) {
+    if (n < 1 || !ops || !ops[0])
+        return kfc_fail(nullptr, KFC_EARG, "kfc_lobpcg_lockstep_subspace: bad arguments");
+    kfc_op* op0 = ops[0];
+    if (!B || nf < 1 || nb < 1 || !rowmaps)
+        return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: the subspace needs B, nf >= 1, nb >= 1 and rowmaps");
+    if (!C) ns = nb;
+    if (ns < 1 || ns > nb)
+        return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: ns must lie in 1:nb, got " + std::to_string(ns));
+    const int64_t N = op0->N;
+    const int g = int(std::max<int64_t>(0, std::min<int64_t>(guard, (N - 5) / 3 - m)));
+    if (ns < m + g)
+        return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: the subspace has " + std::to_string(ns) +
+                                                " directions for a block of " + std::to_string(m + g));
+    for (int i = 0; i < n; ++i) {
+        if (!rowmaps[i]) return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: null rowmap");
+        std::vector<char> hit;
+        for (int64_t f = 0; f < nf; ++f) {
+            const int64_t r = rowmaps[i][f];
+            if (r < -1 || r >= N)
+                return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: rowmap entry " + std::to_string(r) +
+                                                        " outside -1:" + std::to_string(N - 1));
+            if (r >= 0) {
+                if (hit.empty()) hit.assign(size_t(N), 0);
+                if (hit[size_t(r)]) return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: two frame rows map to row " + std::to_string(r));
+                hit[size_t(r)] = 1;
+            }
+        }
+    }
+    SubspaceSeed sub;
+    sub.B = B;
+    sub.nf = nf;
+    sub.nb = nb;
+    sub.ns = ns;
+    sub.rowmaps = rowmaps;
+    sub.C = C;
+    // the checks of kfc_lobpcg_lockstep, then the run with the subspace starts
+    if (!E || !Vs) return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: bad arguments");
+    for (int i = 0; i < n; ++i) {
+        if (!ops[i] || !Vs[i]) return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: null operator or output");
+        if (ops[i]->ctx != op0->ctx)
+            return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: every operator must live on one context");
+        if (ops[i]->N != N) return kfc_fail(op0->ctx, KFC_EARG, "kfc_lobpcg_lockstep_subspace: every operator must have one size");
+        if (ops[i]->cplx) return kfc_fail(op0->ctx, KFC_EUNSUPPORTED, "kfc_lobpcg_lockstep_subspace: complex operators are not served");
+    }
+    if (m < 1) return kfc_fail(op0->ctx, KFC_EARG, "LOBPCG requires m >= 1, got " + std::to_string(m));
+    if (N < 3 * m + 5)
+        return kfc_fail(op0->ctx, KFC_EARG, "LOBPCG needs N >= 3m + 5, got N = " + std::to_string(N) + " for m = " +
+                                                std::to_string(m) + "; the block eigensolver is unreliable on so small a problem, use Dense()");
+    if (!(tol > 0) || !std::isfinite(tol)) return kfc_fail(op0->ctx, KFC_EARG, "LOBPCG: tol must be positive and finite");
+    if (maxiter < 1) return kfc_fail(op0->ctx, KFC_EARG, "LOBPCG: maxiter must be at least 1");
+    if (precond == KFC_PRECOND_RITZ && !(eta > 0 && std::isfinite(eta)))
+        return kfc_fail(op0->ctx, KFC_EARG, "eta must be positive and finite");
+    if (guard < 0) return kfc_fail(op0->ctx, KFC_EARG, "LOBPCG: guard must be nonnegative");
+    if (3 * (m + g) > LS_CMAX)
+        return kfc_fail(op0->ctx, KFC_EUNSUPPORTED, "kfc_lobpcg_lockstep: m + guard above 21 is not served");
+    (void)hipSetDevice(op0->ctx->device);
+    return lockstep_run(n, ops, m, guard, tol, maxiter, precond, eta, KFC_SEED_PRODUCT, nullptr, seed, mode, E, Vs,
+                        stats, &sub);
 }

@@ -135,6 +135,10 @@ SIGNATURES = {
     "kfc_lobpcg_lockstep": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int64, C.c_double, C.c_int64,
                                       C.c_int32, C.c_double, C.c_int32, C.POINTER(_D), C.c_uint64, C.c_int32, _D,
                                       C.POINTER(_D), C.POINTER(Stats)]),
+    "kfc_lobpcg_lockstep_subspace": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int64, C.c_double, C.c_int64,
+                                               C.c_int32, C.c_double, C.c_void_p, C.c_int64, C.c_int32,
+                                               C.POINTER(C.POINTER(C.c_int64)), C.POINTER(_D), C.c_int32, C.c_uint64,
+                                               C.c_int32, _D, C.POINTER(_D), C.POINTER(Stats)]),
     "kfc_chebyshev": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_double,
                                 C.c_int32, C.c_double, C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
     "kfc_dense": (C.c_int, [_P, C.c_int64, _D, _D, C.POINTER(Stats)]),

@@ -298,6 +298,57 @@ def solve_batch(s, Hs, m, X0s=None, lockstep=False):
     return out, [x.as_dict() for x in st]
 
 
+def solve_batch_subspace(s, Hs, m, B, rowmaps, coeffs=None):
+    """A lockstep LOBPCG sweep (``kfc_lobpcg_lockstep_subspace``) whose points start
+    from a learned subspace instead of the product seed: point i starts from the
+    lowest m + g Ritz vectors of H_i on span(P_i B C_i).  ``B`` is a device float64
+    tensor (nf, nb) whose columns span a parameter-independent frame of the parent
+    basis; ``rowmaps[i]`` (nf int64) places frame row f at parent row rowmaps[i][f]
+    (-1: not kept at that point); ``coeffs`` is None (all nb directions) or a list of
+    (nb, ns) host arrays.  The start stands in for X0 (src/lobpcg.jl:217); the
+    acceptance and outputs are those of ``solve_batch(..., lockstep=True)``."""
+    import torch
+    if not isinstance(s, LOBPCG):
+        raise L.ArgumentError("solve_batch_subspace serves LOBPCG sweeps")
+    if s.callback is not None:
+        raise L.UnsupportedError("solve_batch_subspace runs many solves at once and calls no per-iteration callback")
+    n = len(Hs)
+    if len(rowmaps) != n or (coeffs is not None and len(coeffs) != n):
+        raise L.DimensionMismatch(f"{n} operators, {len(rowmaps)} row maps"
+                                  + ("" if coeffs is None else f" and {len(coeffs)} coefficient blocks"))
+    if not (isinstance(B, torch.Tensor) and B.is_cuda and B.dtype == torch.float64 and B.dim() == 2):
+        raise L.ArgumentError("solve_batch_subspace: B must be a 2-D float64 device tensor")
+    nf, nb = B.shape
+    Bc = B.t().contiguous()            # column-major (nf x nb): row j of the transpose is column j
+    maps = [np.ascontiguousarray(np.asarray(r, dtype=np.int64).reshape(-1)) for r in rowmaps]
+    for r in maps:
+        if r.shape[0] != nf:
+            raise L.DimensionMismatch(f"a row map has {r.shape[0]} entries for a frame of {nf} rows")
+    mp = (C.POINTER(C.c_int64) * n)(*[r.ctypes.data_as(C.POINTER(C.c_int64)) for r in maps])
+    ns, cp, keep = nb, None, []
+    if coeffs is not None:
+        blocks = [np.asfortranarray(np.asarray(c, dtype=np.float64)) for c in coeffs]
+        ns = blocks[0].shape[1]
+        for b in blocks:
+            if b.shape != (nb, ns):
+                raise L.DimensionMismatch(f"coefficient block of size {b.shape}, expected ({nb}, {ns})")
+        keep = [b.reshape(-1, order="F") for b in blocks]
+        cp = (L._D * n)(*[k.ctypes.data_as(L._D) for k in keep])
+    ops = (C.c_void_p * n)(*[H.handle.value for H in Hs])
+    E = np.empty(n * m)
+    Vb = [np.empty(H.N * m) for H in Hs]
+    Vp = (L._D * n)(*[v.ctypes.data_as(L._D) for v in Vb])
+    st = (L.Stats * n)()
+    torch.cuda.synchronize()            # B is read on the library's stream
+    L.check(L.lib().kfc_lobpcg_lockstep_subspace(
+        n, ops, int(m), int(s.guard), float(s.tol), int(s.maxiter), L.KFC_PRECOND[s.preconditioner], float(s.eta),
+        C.c_void_p(Bc.data_ptr()), int(nf), int(nb), mp, cp, int(ns), int(s.rng_seed), _mode(s.mode),
+        E.ctypes.data_as(L._D), Vp, st), Hs[0].ctx.handle)
+    del Bc
+    out = [(E[i * m:(i + 1) * m].copy(), unpack_block(Vb[i], Hs[i].N, m, Hs[i].dtype)) for i in range(n)]
+    return out, [x.as_dict() for x in st]
+
+
 def solve_chebyshev_batch(Hs, m, extra=8, degree=60, tol=1e-9, maxiter=200, mode="mixed", seed=0x4346):
     """Batched ChebyshevFilter solves of one-axis operators on one context
     (``kfc_chebyshev_batch``): every filter step of the whole batch is one launch.

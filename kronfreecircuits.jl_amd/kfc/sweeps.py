@@ -10,16 +10,21 @@ the lowest 16 pairs.  Three ways to run a sweep:
   device contexts (kfc_lobpcg_batch), FP32 bulk iterations with FP64
   Rayleigh-Ritz and refinement (mode="mixed");
 * learned: batched, plus an initial block X0 predicted from the parameters by
-  the warm-start network.
+  the warm-start network;
+* learned subspace: batched, each point started from the lowest Ritz vectors of
+  its operator on a learned subspace (LearnedSubspace).
 
-The warm start works in a canonical frame of the parent basis that does not
+Both learned starts work in a canonical frame of the parent basis that does not
 move with the parameters: the theta leaf's kept states are charge states
 |n>, whose energy order changes with n_g, so a parent vector is re-indexed by
-(charge n, phi level, zeta level) over a fixed window.  A fixed 64-vector basis
-B of that frame is taken from the training eigenvectors (their leading
-principal directions); the network maps the six circuit parameters to the 64 x
-16 coefficients of the lowest 16 parent eigenvectors on B, and
-X0 = B C mapped back to the parent basis of the point."""
+(charge n, phi level, zeta level).  The window holds every charge the theta
+leaf keeps at any n_g in [0, 1] and every phi and zeta level, so the lowest
+parent states lie inside it.  A fixed basis B of that frame is taken from the
+training eigenvectors (their leading principal directions).  The network maps
+the six circuit parameters to the coefficients of the lowest m + g parent
+eigenvectors on B (X0 = B C mapped back to the parent basis of the point); the
+learned subspace instead hands span(B) itself to a Rayleigh-Ritz step on the
+device, whose lowest m + g Ritz vectors start LOBPCG."""
 from __future__ import annotations
 
 import math
@@ -29,12 +34,29 @@ import numpy as np
 
 from . import _lib as L
 from . import configs
-from .solvers import LOBPCG, solve_batch, solve_stats
+from .solvers import LOBPCG, solve_batch, solve_batch_subspace, solve_stats
 from .tree import materialize_child, operator
 
-WINDOW = (-12, 14)       # charges n in [-12, 14) of the theta leaf
-PHI_LEVELS = 20
-ZETA_LEVELS = 12
+WINDOW = (-19, 20)       # charges n in [-19, 20): kept by the theta leaf (40 states) at every n_g in [0, 1]
+PHI_LEVELS = 40
+ZETA_LEVELS = 30
+
+
+def principal_directions(Z, k, seed=0):
+    """The leading k left singular vectors of Z (frame rows x training vectors), an
+    orthonormal frame basis (randomized range finder with power iterations on the
+    device when one is present; the spectra here fall off by orders of magnitude)."""
+    import torch
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    Zt = torch.as_tensor(np.ascontiguousarray(Z), dtype=torch.float64, device=dev)
+    k = min(k, min(Zt.shape))
+    g = torch.Generator(device=dev).manual_seed(seed)
+    Q = Zt @ torch.randn(Zt.shape[1], min(Zt.shape[1], k + 32), dtype=torch.float64, device=dev, generator=g)
+    for _ in range(4):
+        Q, _ = torch.linalg.qr(Q)
+        Q, _ = torch.linalg.qr(Zt @ (Zt.T @ Q))
+    U, S, _ = torch.linalg.svd(Q.T @ Zt, full_matrices=False)
+    return (Q @ U[:, :k]).cpu().numpy(), S[:k].cpu().numpy()
 
 
 def features(theta):
@@ -77,15 +99,16 @@ class ZeroPiSweep:
         window, or -1 when that charge is not among the kept theta states."""
         Vt = kids[0].vectors
         charge = np.argmax(np.abs(Vt), axis=0) - self.ncut          # kept index -> charge n
-        pos = {int(n): k for k, n in enumerate(charge)}
         d0, d1, d2 = Vt.shape[1], kids[1].vectors.shape[1], kids[2].vectors.shape[1]
-        idx = []
-        for iz in range(min(ZETA_LEVELS, d2)):
-            for ip in range(min(PHI_LEVELS, d1)):
-                for n in range(*WINDOW):
-                    k = pos.get(n, -1)
-                    idx.append(-1 if k < 0 else k + d0 * (ip + d1 * iz))
-        return np.array(idx)
+        lo, hi = WINDOW
+        kept = np.full(hi - lo, -1, dtype=np.int64)                 # charge n -> kept index
+        inside = (charge >= lo) & (charge < hi)
+        kept[charge[inside] - lo] = np.flatnonzero(inside)
+        nz, nphi = min(ZETA_LEVELS, d2), min(PHI_LEVELS, d1)
+        # frame order: charge fastest, then phi level, then zeta level
+        k = np.broadcast_to(kept[None, None, :], (nz, nphi, hi - lo))
+        plane = d0 * (np.arange(nphi)[None, :, None] + d1 * np.arange(nz)[:, None, None])
+        return np.where(k >= 0, k + plane, -1).reshape(-1)
 
     def to_frame(self, V, idx):
         out = np.zeros((len(idx), V.shape[1]))
@@ -128,13 +151,21 @@ class ZeroPiSweep:
         """Columns LOBPCG converges on a parent of N states: m plus the guards."""
         return self.m + max(0, min(self.guard, (N - 5) // 3 - self.m))
 
-    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, nev=None, lockstep=False):
+    def run_batched(self, thetas, mode="mixed", tol=1e-8, x0_fn=None, nev=None, lockstep=False, subspace=None,
+                    guard=None):
         """All points' parent solves in flight: on the contexts round-robin, or
         (lockstep) on one context advancing together; ``nev`` pairs per point
-        (default m).  Lockstep serves blocks of up to 21 columns (m + guards); a
-        wider request runs on the contexts."""
+        (default m), ``guard`` guard columns (default the sweep's).  Lockstep serves
+        blocks of up to 21 columns (m + guards); a wider request runs on the
+        contexts.  ``subspace`` (a fitted LearnedSubspace): lockstep, every point
+        started from the lowest Ritz vectors on the learned subspace."""
         n_ev = nev or self.m
-        if lockstep and 3 * (n_ev + self.guard) > 64:
+        gd = self.guard if guard is None else guard
+        if subspace is not None:
+            lockstep = True
+        if lockstep and 3 * (n_ev + gd) > 64:
+            if subspace is not None:
+                raise L.UnsupportedError("a learned-subspace sweep runs in lockstep: m + guard must stay below 22")
             lockstep = False
         t0 = time.perf_counter()
         Hs, kidss = [], []
@@ -145,7 +176,11 @@ class ZeroPiSweep:
         X0s = None
         if x0_fn is not None:
             X0s = x0_fn(thetas, Hs, kidss)
-        res, st = solve_batch(LOBPCG(tol=tol, mode=mode, guard=self.guard), Hs, nev or self.m, X0s, lockstep=lockstep)
+        solver = LOBPCG(tol=tol, mode=mode, guard=gd)
+        if subspace is not None:
+            res, st = solve_batch_subspace(solver, Hs, n_ev, subspace.B, [self.frame_index(k) for k in kidss])
+        else:
+            res, st = solve_batch(solver, Hs, n_ev, X0s, lockstep=lockstep)
         wall = time.perf_counter() - t0
         return [(E, V, s, k) for (E, V), s, k in zip(res, st, kidss)], wall
 
@@ -167,13 +202,7 @@ class WarmStart:
 
     def fit_basis(self, frames):
         """B: the leading principal directions of the stacked training eigenvectors."""
-        Z = np.concatenate([f[:, :self.m] for f in frames], axis=1)
-        G = Z.T @ Z
-        w, U = np.linalg.eigh(G)
-        top = np.argsort(w)[::-1][:self.nb]
-        B = Z @ U[:, top] / np.sqrt(np.maximum(w[top], 1e-300))
-        Q, _ = np.linalg.qr(B)
-        self.B = Q
+        self.B, _ = principal_directions(np.concatenate([f[:, :self.m] for f in frames], axis=1), self.nb)
 
     def coeffs(self, Vf):
         C = self.B.T @ Vf[:, :self.m]           # nb x m
@@ -206,6 +235,32 @@ class WarmStart:
                 X0[:, bad] = np.random.default_rng(0).standard_normal((H.N, int(bad.sum())))
             out.append(X0)
         return out
+
+
+class LearnedSubspace:
+    """The learned subspace: the leading ``nbasis`` principal directions of the
+    training eigenvectors in the canonical frame (kept on the device).  A sweep
+    started from it (``run_batched(..., subspace=...)``, kfc_lobpcg_lockstep_subspace)
+    begins each point at the lowest m + g Ritz vectors of that point's operator on
+    span(B): the Rayleigh-Ritz step resolves the order and mixing of neighbouring
+    states that a regression of individual eigenvectors blurs, so the start is
+    accurate in exactly the directions (the next states up) that set LOBPCG's
+    convergence of the last block columns."""
+
+    def __init__(self, sweep, nbasis=256):
+        self.sw, self.nb = sweep, nbasis
+        self.B = None
+        self.singular_values = None
+
+    def fit(self, frames, ncols=None):
+        """B from the training eigenvectors in the frame (their first ``ncols``
+        columns each, default all)."""
+        import torch
+        Z = np.concatenate([f if ncols is None else f[:, :ncols] for f in frames], axis=1)
+        U, S = principal_directions(Z, self.nb)
+        self.B = torch.tensor(U, dtype=torch.float64, device="cuda")
+        self.singular_values = S
+        return self
 
 
 def path_order(thetas):

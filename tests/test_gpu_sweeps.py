@@ -150,3 +150,59 @@ def test_transmon_batch_chebyshev_vs_oracle():
         leaves = [O.Node([(f"varphi_{i}", O.PeriodicGrid(inst[f"N{i}"], 8))], keep=8, real=False) for i in (1, 2)]
         ref = O.materialize(O.Node(leaves, keep=12, real=False), spec)
         assert np.max(np.abs(top.energies - ref.energies) / np.abs(ref.energies)) < 1e-9
+
+
+def test_learned_subspace_start():
+    """The learned-subspace start (kfc_lobpcg_lockstep_subspace): each point begins at
+    the lowest Ritz vectors of its operator on span(P B C), B the leading principal
+    directions of training eigenvectors in the canonical frame.  Same energies and
+    residual bar as the cold lockstep sweep, fewer iterations; a coefficient block
+    selecting the first ns columns of B gives the solve of B[:, :ns] bit for bit."""
+    import torch
+
+    import kfc
+    from kfc import configs
+    from kfc.sweeps import LearnedSubspace
+    sw = _small_sweep()
+    W = sw.width(12 * 12 * 10)
+    train = [configs.zero_pi_theta(i) for i in range(48)]
+    test = [configs.zero_pi_theta(9100 + i) for i in range(6)]
+    gen, _ = sw.run_batched(train, mode="f64", nev=W)
+    frames = [sw.to_frame(V, sw.frame_index(kids)) for E, V, st, kids in gen]
+    sub = LearnedSubspace(sw, nbasis=48).fit(frames)
+    assert sub.B.shape == (frames[0].shape[0], 48)
+    B = sub.B.cpu().numpy()
+    assert np.allclose(B.T @ B, np.eye(48), atol=1e-12)
+    cold, _ = sw.run_batched(test, mode="f64", lockstep=True)
+    warm, _ = sw.run_batched(test, mode="f64", subspace=sub)
+    for (Ec, Vc, sc, _), (Ew, Vw, sw_, _) in zip(cold, warm):
+        assert sc["max_residual"] <= 1e-8 and sw_["max_residual"] <= 1e-8
+        assert np.max(np.abs(Ec - Ew) / np.abs(Ec)) < 1e-9
+        assert np.allclose(Vw.T @ Vw, np.eye(sw.m), atol=1e-10)
+        assert sw_["warm"] == 1
+    it_cold = np.mean([o[2]["iterations"] for o in cold])
+    it_warm = np.mean([o[2]["iterations"] for o in warm])
+    assert it_warm < 0.8 * it_cold, (it_warm, it_cold)
+    # C selecting B's first ns columns == the subspace of those columns
+    ns = 24
+    Hs, kidss = zip(*[sw.parent(th, ctx=sw.ctxs[0]) for th in test])
+    maps = [sw.frame_index(k) for k in kidss]
+    s = kfc.LOBPCG(tol=1e-8, mode="f64", guard=sw.guard)
+    a, sa = kfc.solve_batch_subspace(s, list(Hs), sw.m, sub.B[:, :ns].contiguous(), maps)
+    Cs = [np.eye(48)[:, :ns] for _ in test]
+    b, sb = kfc.solve_batch_subspace(s, list(Hs), sw.m, sub.B, maps, Cs)
+    for (Ea, Va), (Eb, Vb), x, y in zip(a, b, sa, sb):
+        assert np.array_equal(Ea, Eb) and np.array_equal(Va, Vb)
+        assert x["iterations"] == y["iterations"]
+    # refusals: a subspace narrower than the block, two frame rows on one parent row
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve_batch_subspace(s, list(Hs), sw.m, sub.B[:, :W - 1].contiguous(), maps)
+    bad = [m_.copy() for m_ in maps]
+    hit = np.flatnonzero(bad[0] >= 0)
+    bad[0][hit[1]] = bad[0][hit[0]]
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve_batch_subspace(s, list(Hs), sw.m, sub.B, bad)
+    with pytest.raises(kfc.DimensionMismatch):
+        kfc.solve_batch_subspace(s, list(Hs), sw.m, sub.B, [m_[:-1] for m_ in maps])
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve_batch_subspace(s, list(Hs), sw.m, torch.zeros(4, 4), maps)
