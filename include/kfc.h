@@ -98,7 +98,8 @@ typedef struct {
     int64_t n_apply_calls;   /* block apply launches */
     double seconds_total;
     double seconds_apply;     /* host wall time around the applies */
-    double seconds_apply_gpu; /* HIP-event time of the applies on the context stream */
+    double seconds_apply_gpu; /* HIP-event time of the applies on the context stream (-1: the
+                                 applies ran inside captured step graphs, untimed) */
     double max_residual;     /* worst fresh FP64 true residual of the returned pairs */
     int32_t n_converged;
     int32_t warm;            /* 1 when an X0 was used */

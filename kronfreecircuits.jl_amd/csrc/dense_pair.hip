@@ -1355,15 +1355,29 @@ int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, int ng, hipStream_t st
 }
 
 template <class T>
+void pairs_fill_table(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
+                      void* host_tab) {
+    const size_t ng = ops[0]->pairs.groups.size();
+    PairArgs<T>* h = static_cast<PairArgs<T>*>(host_tab);
+    for (size_t gi = 0; gi < ng; ++gi)
+        for (int q = 0; q < n; ++q) h[gi * n + q] = pair_args<T>(ops[q], gi, X[q], Y[q], ncols, ld);
+}
+
+template <class T>
 int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
                        hipStream_t st, void* host_tab, void* dev_tab) {
+    if (n == 0) return KFC_OK;
+    pairs_fill_table<T>(ops, n, X, Y, ncols, ld, host_tab);
+    return launch_pairs_filled<T>(ops, n, st, host_tab, dev_tab);
+}
+
+template <class T>
+int launch_pairs_filled(kfc_op* const* ops, int n, hipStream_t st, void* host_tab, void* dev_tab) {
     if (n == 0) return KFC_OK;
     const kfc_op* op0 = ops[0];
     const size_t ng = op0->pairs.groups.size();
     PairArgs<T>* h = static_cast<PairArgs<T>*>(host_tab);
     PairArgs<T>* d = static_cast<PairArgs<T>*>(dev_tab);
-    for (size_t gi = 0; gi < ng; ++gi)
-        for (int q = 0; q < n; ++q) h[gi * n + q] = pair_args<T>(ops[q], gi, X[q], Y[q], ncols, ld);
     KFC_HIP(op0->ctx, hipMemcpyAsync(d, h, sizeof(PairArgs<T>) * ng * n, hipMemcpyHostToDevice, st));
     std::vector<int> ntmax(ng, 0);
     int64_t slabs = 0;
@@ -1376,6 +1390,10 @@ int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* co
     return run_pair_groups(op0->ctx, pair_tune_key(op0, sizeof(T), slabs), int(ng), st, one);
 }
 
+template void pairs_fill_table<double>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t, void*);
+template void pairs_fill_table<float>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t, void*);
+template int launch_pairs_filled<double>(kfc_op* const*, int, hipStream_t, void*, void*);
+template int launch_pairs_filled<float>(kfc_op* const*, int, hipStream_t, void*, void*);
 template int launch_pairs_batch<double>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t,
                                         hipStream_t, void*, void*);
 template int launch_pairs_batch<float>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t,

@@ -239,6 +239,14 @@ size_t pair_args_bytes(int prec);
 template <class T>
 int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
                        hipStream_t st, void* host_tab, void* dev_tab);
+// The same in two parts: the host table filled, then its upload and the launches (a
+// graph captures the second part once and replays it after each refill of host_tab;
+// the shapes must have been tuned by an earlier launch of the same key).
+template <class T>
+void pairs_fill_table(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
+                      void* host_tab);
+template <class T>
+int launch_pairs_filled(kfc_op* const* ops, int n, hipStream_t st, void* host_tab, void* dev_tab);
 int op_host_diag(kfc_op* op);
 
 // ---- derived quantities (host: kfc_api.hip, apply.hip; device: seeds.hip) ----------------

@@ -37,7 +37,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 
 #include "blas.h"
@@ -685,6 +687,24 @@ struct Phase {
     double t_wait = 0, t_rr = 0;   // host time blocked on the stream / in the Rayleigh-Ritz steps
     int64_t n_iter = 0, n_pointiter = 0, n_collapsed = 0;
     int nchunk0 = 0;               // row chunks per point with every point active (the partials' capacity)
+    // The device work of a step (the uploads of theta, the slots and the pair table,
+    // the residual, the apply, the Grams, their sums and the records' download) as one
+    // graph per active count: captured the second time that count occurs (its first
+    // run tunes the pair launch shapes) and replayed while it lasts.  The host refills
+    // the pinned tables before each replay.  KFC_GRAPHS=0: separate launches.
+    std::map<int, hipGraphExec_t> gexec = {};
+    std::set<int> gseen = {};
+    int64_t n_graph_steps = 0, n_captures = 0;
+    bool apply_untimed = false;    // some applies ran inside graphs (no events there)
+
+    ~Phase() {
+        for (auto& kv : gexec) (void)hipGraphExecDestroy(kv.second);
+    }
+
+    static bool graphs_on() {
+        const char* e = std::getenv("KFC_GRAPHS");
+        return !(e && e[0] == '0');
+    }
 
     // Row chunks per point for a launch over `nact` points: with fewer active points
     // each point's rows are split finer (up to 16x), so that the tail of a sweep, where
@@ -718,10 +738,16 @@ struct Phase {
     // Slot tables: half 0 for the steps of an iteration, half 1 for the rotation
     // (its upload may still be in flight when the next iteration fills half 0).
     int upload_slots(const std::vector<LsSlot>& sl, int half = 0) {
+        fill_slots(sl, half);
+        return copy_slots(int(sl.size()), half);
+    }
+    void fill_slots(const std::vector<LsSlot>& sl, int half) {
+        std::copy(sl.begin(), sl.end(), hslot.as<LsSlot>() + size_t(half) * pts.size());
+    }
+    int copy_slots(int count, int half) {
         LsSlot* h = hslot.as<LsSlot>() + size_t(half) * pts.size();
         LsSlot* d = dslot.as<LsSlot>() + size_t(half) * pts.size();
-        std::copy(sl.begin(), sl.end(), h);
-        KFC_HIP(ctx, hipMemcpyAsync(d, h, sizeof(LsSlot) * sl.size(), hipMemcpyHostToDevice, st));
+        KFC_HIP(ctx, hipMemcpyAsync(d, h, sizeof(LsSlot) * count, hipMemcpyHostToDevice, st));
         return KFC_OK;
     }
 
@@ -775,10 +801,97 @@ struct Phase {
     }
 
     int upload_theta() {
+        fill_theta();
+        return copy_theta();
+    }
+    void fill_theta() {
         double* h = htheta.as<double>();
         for (size_t p = 0; p < pts.size(); ++p)
             for (int j = 0; j < cfg.W; ++j) h[p * cfg.W + j] = pts[p].theta.empty() ? 0.0 : pts[p].theta[j];
+    }
+    int copy_theta() {
         KFC_HIP(ctx, hipMemcpyAsync(dtheta.p, htheta.p, sizeof(double) * pts.size() * cfg.W, hipMemcpyHostToDevice, st));
+        return KFC_OK;
+    }
+
+    // The pair table of H applied to the R columns (W of them at column W) of each
+    // listed point, written to the R columns of A[par]; the matvecs counted.
+    void fill_apply_r(const std::vector<int>& list) {
+        const int n = int(list.size());
+        std::vector<kfc_op*> ops(n);
+        std::vector<const void*> X(n);
+        std::vector<void*> Y(n);
+        for (int q = 0; q < n; ++q) {
+            LsPoint& pt = pts[list[q]];
+            ops[q] = pt.op;
+            const int64_t off = int64_t(list[q]) * 3 * cfg.W * N + int64_t(cfg.W) * N;
+            X[q] = S[pt.par] + off;
+            Y[q] = A[pt.par] + off;
+            pt.matvec += cfg.W;
+        }
+        pairs_fill_table<T>(ops.data(), n, X.data(), Y.data(), cfg.W, N, htab.p);
+    }
+
+    // The device work of one step over nact points (see gexec): uploads, residual,
+    // apply (from the filled pair table), Grams, sums, records to the host.
+    int enqueue_step(const std::vector<int>& act, const double* pth, const double* D, int pmode) {
+        const int nact = int(act.size());
+        int rc = copy_theta();
+        if (rc == KFC_OK) rc = copy_slots(nact, 0);
+        if (rc != KFC_OK) return rc;
+        const LsArgs<T> a = args(nact);
+        hipLaunchKernelGGL(k_ls_resid<T>, dim3(nchunk, nact), dim3(256), 0, st, a, cfg.W, dtheta.as<double>(), pth, D,
+                           pmode, cfg.eta, 0);
+        KFC_HIP(ctx, hipGetLastError());
+        std::vector<kfc_op*> ops(nact);
+        for (int q = 0; q < nact; ++q) ops[q] = pts[act[q]].op;
+        rc = launch_pairs_filled<T>(ops.data(), nact, st, htab.p, dtab.p);
+        if (rc != KFC_OK) return rc;
+        hipLaunchKernelGGL(k_ls_gram<T>, dim3(nchunk, nact), dim3(256), 0, st, a);
+        KFC_HIP(ctx, hipGetLastError());
+        const int ne = LS_NORM + cfg.W;
+        hipLaunchKernelGGL(k_ls_sum, dim3((ne + 255) / 256, nact), dim3(256), 0, st, dpart.as<double>(), nchunk, ne,
+                           dred.as<double>());
+        KFC_HIP(ctx, hipGetLastError());
+        KFC_HIP(ctx, hipMemcpyAsync(hred.p, dred.p, sizeof(double) * LS_E * nact, hipMemcpyDeviceToHost, st));
+        return KFC_OK;
+    }
+
+    // One step through its graph (captured or replayed); false in *done when the step
+    // has to run as launches (the first time its active count occurs).
+    int graph_step(const std::vector<int>& act, const double* pth, const double* D, int pmode, bool* done) {
+        const int nact = int(act.size());
+        *done = false;
+        auto it = gexec.find(nact);
+        if (it == gexec.end()) {
+            if (!gseen.count(nact)) {
+                gseen.insert(nact);
+                return KFC_OK;
+            }
+            KFC_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_step(act, pth, D, pmode);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc != KFC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return kfc_fail(ctx, KFC_EDEVICE, std::string("lockstep step capture: ") + hipGetErrorString(e));
+            hipGraphExec_t x = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess)
+                return kfc_fail(ctx, KFC_EDEVICE, std::string("lockstep step instantiate: ") + hipGetErrorString(ei));
+            it = gexec.emplace(nact, x).first;
+            ++n_captures;
+        }
+        KFC_HIP(ctx, hipGraphLaunch(it->second, st));
+        const auto t0 = clk::now();
+        KFC_HIP(ctx, hipStreamSynchronize(st));
+        t_wait += secs(t0);
+        ++n_graph_steps;
+        apply_untimed = true;
+        *done = true;
         return KFC_OK;
     }
 
@@ -887,13 +1000,26 @@ struct Phase {
                 sl.push_back(LsSlot{p, pts[p].par, c, 0});
                 cs.push_back(c);
             }
+            // the RitzJacobi shifts: diagonal expectations on the first step, then the Ritz values
+            const double* pth = dtheta.as<double>();
+            int pmode = cfg.precond == KFC_PRECOND_RITZ ? 1 : cfg.precond == KFC_PRECOND_JACOBI ? 2 : 0;
+            const double* D = pmode == 0 ? nullptr : pmode == 2 ? Dall + pts.size() * N : Dall;
+            bool graphed = false;
+            if (pair_batch && graphs_on() && !(pmode == 1 && first_prep)) {
+                fill_theta();
+                fill_slots(sl, 0);
+                fill_apply_r(act);
+                rc = graph_step(act, pth, D, pmode, &graphed);
+                if (rc != KFC_OK) return rc;
+                if (!graphed) {   // this count's first step: launches (the table is filled)
+                    for (int p : act) pts[p].matvec -= W;
+                }
+            }
+            if (!graphed) {
             rc = upload_theta();
             if (rc == KFC_OK) rc = upload_slots(sl);
             if (rc != KFC_OK) return rc;
             const LsArgs<T> a = args(nact);
-            // the RitzJacobi shifts: diagonal expectations on the first step, then the Ritz values
-            const double* pth = dtheta.as<double>();
-            int pmode = cfg.precond == KFC_PRECOND_RITZ ? 1 : cfg.precond == KFC_PRECOND_JACOBI ? 2 : 0;
             if (pmode == 1 && first_prep) {
                 hipLaunchKernelGGL(k_ls_dexp<T>, dim3(nchunk, nact), dim3(256), 0, st, a, Dall);
                 KFC_HIP(ctx, hipGetLastError());
@@ -906,7 +1032,6 @@ struct Phase {
                 pth = dpth.as<double>();
                 first_prep = false;
             }
-            const double* D = pmode == 0 ? nullptr : pmode == 2 ? Dall + pts.size() * N : Dall;
             hipLaunchKernelGGL(k_ls_resid<T>, dim3(nchunk, nact), dim3(256), 0, st, a, W, dtheta.as<double>(), pth, D,
                                pmode, cfg.eta, 0);
             KFC_HIP(ctx, hipGetLastError());
@@ -918,6 +1043,7 @@ struct Phase {
             if (rc != KFC_OK) return rc;
             rc = read_apply_time();
             if (rc != KFC_OK) return rc;
+            }
             // convergence of every column on the residuals of the current X
             std::vector<int> go;
             std::vector<int> gcs;
@@ -1226,6 +1352,7 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                      (long long)N, W, secs(t0), nthreads, int(pair_batch));
     LsCfg cfg{int(m), W, tol, maxiter, precond, eta};
     double apply_gpu = 0;
+    bool untimed = false;   // applies ran inside step graphs: no per-apply device time
     int64_t iters32 = 0, iters64 = 0;
     std::vector<int64_t> it32(n, 0), mv32(n, 0);
     const double* Dall = pmode ? dD.as<double>() : nullptr;
@@ -1253,9 +1380,11 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                         hred, hslot, htheta, hM, htab, Dall, team, sw, &apply_gpu, e0, e1};
         rc = ph.run(&iters32);
         if (rc != KFC_OK) return rc;
+        untimed |= ph.apply_untimed;
         if (trace)
-            std::fprintf(stderr, "[lockstep] fp32 phase: %lld steps, %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, %.3f s\n",
-                         (long long)ph.n_iter, (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, secs(t0));
+            std::fprintf(stderr, "[lockstep] fp32 phase: %lld steps (%lld from %lld graphs), %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, %.3f s\n",
+                         (long long)ph.n_iter, (long long)ph.n_graph_steps, (long long)ph.n_captures,
+                         (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, secs(t0));
         // FP64 continuation from each point's FP32 X
         for (int i = 0; i < n; ++i) {
             it32[i] = pts[i].it;
@@ -1279,9 +1408,11 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
                          hred, hslot, htheta, hM, htab, Dall, team, 0.0, &apply_gpu, e0, e1};
         rc = ph.run(&iters64);
         if (rc != KFC_OK) return rc;
+        untimed |= ph.apply_untimed;
         if (trace)
-            std::fprintf(stderr, "[lockstep] fp64 phase: %lld steps, %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, %.3f s\n",
-                         (long long)ph.n_iter, (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, secs(t0));
+            std::fprintf(stderr, "[lockstep] fp64 phase: %lld steps (%lld from %lld graphs), %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, %.3f s\n",
+                         (long long)ph.n_iter, (long long)ph.n_graph_steps, (long long)ph.n_captures,
+                         (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, secs(t0));
         // fresh FP64 residuals of the returned m pairs: A[par ^ 1] X slot = H X
         std::vector<int> fin;
         for (int i = 0; i < n; ++i)
@@ -1356,8 +1487,8 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
         for (int i = 0; i < n; ++i) {
             if (pts[i].failed) continue;
             stats[i].seconds_total = tot / n;
-            stats[i].seconds_apply_gpu = apply_gpu / n;
-            stats[i].seconds_apply = apply_gpu / n;
+            stats[i].seconds_apply_gpu = untimed ? -1.0 : apply_gpu / n;
+            stats[i].seconds_apply = untimed ? -1.0 : apply_gpu / n;
         }
     }
     (void)iters32;

@@ -40,3 +40,50 @@ def test_filter_graph_equals_launches(monkeypatch, mode):
         leaves = [O.Node([(f"varphi_{i}", O.PeriodicGrid(inst[f"N{i}"], 8))], keep=8, real=False) for i in (1, 2)]
         ref = O.materialize(O.Node(leaves, keep=12, real=False), spec)
         assert np.max(np.abs(top.energies - ref.energies) / np.abs(ref.energies)) < 1e-9
+
+
+def _lockstep(monkeypatch, graphs, pts, mode, sub=None, sw=None):
+    monkeypatch.setenv("KFC_GRAPHS", "1" if graphs else "0")
+    return sw.run_batched(pts, mode=mode, lockstep=True, subspace=sub)[0]
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+def test_lockstep_step_graph_equals_launches(monkeypatch, mode):
+    """The lockstep LOBPCG step (uploads, residual, batched dense-pair apply, Grams,
+    sums, records download) as one graph per active count, replayed with the host's
+    Rayleigh-Ritz, rotation and synchronization between replays: bit for bit the
+    separate launches, and the oracle's dense spectrum."""
+    from kfc import configs
+    from kfc.sweeps import ZeroPiSweep
+    from tests.helpers import oracle_from_kfc_terms
+    sw = ZeroPiSweep(ncut=12, Nphi=48, Nzeta=10, keep=12, m=6, contexts=1)
+    pts = [configs.zero_pi_theta(7300 + i) for i in range(7)]
+    eager = _lockstep(monkeypatch, False, pts, mode, sw=sw)
+    graph = _lockstep(monkeypatch, True, pts, mode, sw=sw)
+    again = _lockstep(monkeypatch, True, pts, mode, sw=sw)
+    for e, g, a in zip(eager, graph, again):
+        assert np.array_equal(e[0], g[0]) and np.array_equal(e[1], g[1])
+        assert np.array_equal(g[0], a[0]) and np.array_equal(g[1], a[1])
+        assert e[2]["iterations"] == g[2]["iterations"] == a[2]["iterations"]
+        assert e[2]["n_matvec"] == g[2]["n_matvec"]
+        assert g[2]["max_residual"] <= 1e-8
+    assert eager[0][2]["seconds_apply_gpu"] > 0 and graph[0][2]["seconds_apply_gpu"] == -1.0
+    for th, (E, V, st, kids) in zip(pts[:3], graph[:3]):
+        H, _ = sw.parent(th)
+        w = np.linalg.eigvalsh(O.to_matrix(oracle_from_kfc_terms(H)))[:sw.m]
+        assert np.max(np.abs(E - w) / np.abs(w)) < 1e-9
+
+
+def test_lockstep_subspace_graph_equals_launches(monkeypatch):
+    from kfc import configs
+    from kfc.sweeps import LearnedSubspace, ZeroPiSweep
+    sw = ZeroPiSweep(ncut=12, Nphi=48, Nzeta=10, keep=12, m=6, contexts=2)
+    W = sw.width(12 * 12 * 10)
+    gen, _ = sw.run_batched([configs.zero_pi_theta(i) for i in range(32)], mode="f64", nev=W)
+    sub = LearnedSubspace(sw, nbasis=40).fit([sw.to_frame(V, sw.frame_index(k)) for E, V, st, k in gen])
+    pts = [configs.zero_pi_theta(7400 + i) for i in range(5)]
+    eager = _lockstep(monkeypatch, False, pts, "f64", sub=sub, sw=sw)
+    graph = _lockstep(monkeypatch, True, pts, "f64", sub=sub, sw=sw)
+    for e, g in zip(eager, graph):
+        assert np.array_equal(e[0], g[0]) and np.array_equal(e[1], g[1])
+        assert e[2]["iterations"] == g[2]["iterations"]
