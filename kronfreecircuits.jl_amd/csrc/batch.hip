@@ -669,9 +669,14 @@ int cheb_batch_t(int n, kfc_op* const* ops, int64_t m, int64_t extra, int degree
     kfc_ctx* ctx = ops[0]->ctx;
     auto t0 = clk::now();
     std::vector<Band1D> bands(n);
-    for (int i = 0; i < n; ++i) {
-        int rc = band_of(ops[i], bands[i]);
+    std::vector<int> brc(n, KFC_OK);
+    for (int i = 0; i < n; ++i) {   // the memoized diagonals first (an operator may appear twice)
+        const int rc = ops[i]->ndims == 1 ? op_host_diag(ops[i]) : KFC_OK;
         if (rc != KFC_OK) return rc;
+    }
+    parallel_for(n, [&](int i) { brc[i] = band_of(ops[i], bands[i]); });
+    for (int i = 0; i < n; ++i) {
+        if (brc[i] != KFC_OK) return brc[i];
         if (m + extra > bands[i].N)
             return kfc_fail(ctx, KFC_EARG, "kfc_chebyshev_batch: m + extra exceeds the size of problem " + std::to_string(i));
     }
@@ -684,17 +689,17 @@ int cheb_batch_t(int n, kfc_op* const* ops, int64_t m, int64_t extra, int degree
     }
     int64_t total = 0;
     for (auto& x : bands) total += x.N * nb;
-    // random start (seeded per problem), FP64 on the host
+    // random start (seeded per problem), FP64 on the host, the problems on host threads
     std::vector<cd> X0(static_cast<size_t>(total));
     {
-        int64_t off = 0;
-        for (int p = 0; p < n; ++p) {
+        std::vector<int64_t> offs(n + 1, 0);
+        for (int p = 0; p < n; ++p) offs[p + 1] = offs[p] + bands[p].N * nb;
+        parallel_for(n, [&](int p) {
             std::mt19937_64 g(seed + 0x9e3779b97f4a7c15ull * uint64_t(p + 1));
             std::normal_distribution<double> nd;
             for (int64_t i = 0; i < bands[p].N * nb; ++i)
-                X0[off + i] = IsCx<S>::value ? cd(nd(g), nd(g)) : cd(nd(g), 0.0);
-            off += bands[p].N * nb;
-        }
+                X0[offs[p] + i] = IsCx<S>::value ? cd(nd(g), nd(g)) : cd(nd(g), 0.0);
+        });
     }
     Batch<S> b64;
     int rc = batch_init<S>(b64, ctx, bands, nb);
