@@ -340,6 +340,13 @@ int kfc_chebyshev(kfc_op* op, int64_t m, int32_t degree, int64_t extra, int64_t 
  * m == 0 returns all prod(dims) pairs. */
 int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats);
 
+/* solve(::Dense, H, m) for each of n operators (the tops of a batched sweep, which
+ * the reference solves one after another in materialize): Matrix(H) of each
+ * operator of at most 256 states formed on the host from its terms and
+ * diagonalized on host threads, larger ones as kfc_dense.  1 <= m <= prod(dims)
+ * of every operator; E[n x m]; Vs: n host arrays; stats: NULL or n records. */
+int kfc_dense_batch(int32_t n, kfc_op* const* ops, int64_t m, double* E, double* const* Vs, kfc_stats* stats);
+
 /* Fresh FP64 true residuals ||H v_j - E_j v_j|| of host pairs (E, V[N x m]):
  * _worst_true_residual per column (src/solvers.jl:65-69). */
 int kfc_residuals(kfc_op* op, int64_t m, const double* E, const double* V, double* resid);

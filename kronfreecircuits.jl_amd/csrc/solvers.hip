@@ -2,6 +2,7 @@
 // Block data stay on the device; only small projected matrices cross to the
 // host, where they are diagonalized in FP64 (herm_eig).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <numeric>
@@ -192,18 +193,80 @@ int true_residuals(kfc_op* op, Work& w, const std::vector<double>& E, const std:
 // ------------------------------------------------------------------------------------------
 // Dense: Matrix(H) by identity blocks on the device (src/apply.jl:506-531), host eig.
 
+// Matrix(H) of a small operator formed on the host from its terms (src/apply.jl:506-531
+// forms it by applying H to identity blocks; for a few hundred states the device round
+// trips cost more than the matrix): the fused diagonal plus, per kept term, its
+// coefficient times the Kronecker product of the factors' dense forms.
+void host_matrix(kfc_op* op, std::vector<cd>& M) {
+    const int64_t N = op->N;
+    const int K = op->ndims;
+    std::vector<int64_t> stride(K, 1);
+    for (int a = 1; a < K; ++a) stride[a] = stride[a - 1] * op->dims[a - 1];
+    M.assign(size_t(N) * N, cd(0));
+    for (int64_t i = 0; i < N; ++i) M[i + size_t(i) * N] = op->hdiag[i];
+    for (const Term& t : op->terms) {
+        std::vector<std::vector<cd>> F(K);   // empty: identity on that axis
+        for (const Factor& f : t.factors) {
+            const int64_t d = f.d;
+            std::vector<cd>& A = F[f.axis];
+            A.assign(size_t(d) * d, cd(0));
+            if (f.kind == KFC_DIAG) {
+                for (int64_t i = 0; i < d; ++i) A[i + size_t(i) * d] = f.diag[i];
+            } else if (f.kind == KFC_BANDED) {   // y[i] = sum_b w_b[i] x[i + o_b] on valid rows
+                for (size_t b = 0; b < f.offsets.size(); ++b)
+                    for (int64_t i = 0; i < d; ++i) {
+                        const int64_t j = i + f.offsets[b];
+                        if (j >= 0 && j < d) A[i + size_t(j) * d] += f.weights[i + b * d];
+                    }
+            } else if (f.kind == KFC_SHIFT) {    // y[i + s] = factor x[i]
+                for (int64_t i = 0; i < d; ++i) {
+                    const int64_t r = i + f.shift;
+                    if (r >= 0 && r < d) A[r + size_t(i) * d] += f.factor;
+                }
+            } else {
+                A = f.matrix;
+            }
+        }
+        for (int64_t c = 0; c < N; ++c)
+            for (int64_t r = 0; r < N; ++r) {
+                cd v = t.coeff;
+                for (int a = 0; a < K && v != cd(0); ++a) {
+                    const int64_t d = op->dims[a];
+                    const int64_t ra = (r / stride[a]) % d, ca = (c / stride[a]) % d;
+                    if (F[a].empty()) {
+                        if (ra != ca) v = 0;
+                    } else {
+                        v *= F[a][ra + size_t(ca) * d];
+                    }
+                }
+                M[r + size_t(c) * N] += v;
+            }
+    }
+}
+
+// Operators up to this many states have Matrix(H) formed on the host
+constexpr int64_t DENSE_HOST_MAX = 256;
+
 template <class S>
 int dense_t(kfc_op* op, int64_t m, double* Eout, double* Vout, kfc_stats* st) {
     auto t0 = clk::now();
     const int64_t N = op->N;
     if (N > 8192) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "Dense: prod(dims) above 8192 is not served; use an iterative solver");
     Timer tm;
+    std::vector<cd> M, blk;
+    if (N <= DENSE_HOST_MAX) {
+        const int rc = op_host_diag(op);
+        if (rc != KFC_OK) return rc;
+        host_matrix(op, M);
+    }
     const int B = int(std::min<int64_t>(N, 256));
     DevBuf be, by;
-    KFC_HIP(op->ctx, be.ensure(sizeof(S) * size_t(N) * B));
-    KFC_HIP(op->ctx, by.ensure(sizeof(S) * size_t(N) * B));
-    std::vector<cd> M(size_t(N) * N), blk;
-    for (int64_t c0 = 0; c0 < N; c0 += B) {
+    if (N > DENSE_HOST_MAX) {
+        KFC_HIP(op->ctx, be.ensure(sizeof(S) * size_t(N) * B));
+        KFC_HIP(op->ctx, by.ensure(sizeof(S) * size_t(N) * B));
+        M.assign(size_t(N) * N, cd(0));
+    }
+    for (int64_t c0 = 0; N > DENSE_HOST_MAX && c0 < N; c0 += B) {
         const int w = int(std::min<int64_t>(B, N - c0));
         std::vector<cd> E(size_t(N) * w, cd(0));
         for (int j = 0; j < w; ++j) E[(c0 + j) + size_t(j) * N] = 1;
@@ -228,6 +291,54 @@ int dense_t(kfc_op* op, int64_t m, double* Eout, double* Vout, kfc_stats* st) {
         st->seconds_apply_gpu = tm.apply_gpu;
         st->seconds_total = secs(t0);
         st->n_converged = int(k);
+    }
+    return KFC_OK;
+}
+
+// Dense solves of many small operators (the tops of a batched sweep): each Matrix(H)
+// formed on the host and its eigendecomposition, on host threads; larger operators
+// take dense_t one after another.
+int dense_batch(int n, kfc_op* const* ops, int64_t m, double* E, double* const* Vs, kfc_stats* stats) {
+    std::vector<int> rcs(n, KFC_OK), big;
+    for (int i = 0; i < n; ++i) {
+        if (ops[i]->N > DENSE_HOST_MAX) {
+            big.push_back(i);
+            continue;
+        }
+        rcs[i] = op_host_diag(ops[i]);   // memoized on the operator, formed before the threads
+        if (rcs[i] != KFC_OK) return kfc_fail(ops[0]->ctx, rcs[i], "Dense (batch) of operator " + std::to_string(i) + ": " + ops[i]->ctx->err);
+    }
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int i = next++; i < n; i = next++) {
+            kfc_op* op = ops[i];
+            const int64_t N = op->N;
+            if (N > DENSE_HOST_MAX) continue;
+            auto t0 = clk::now();
+            std::vector<cd> M, Z;
+            std::vector<double> w;
+            host_matrix(op, M);
+            small_eig(int(N), M, w, Z);
+            const int64_t k = (m > 0 && m < N) ? m : N;
+            for (int64_t j = 0; j < k; ++j) E[size_t(i) * m + j] = w[j];
+            Z.resize(size_t(N) * k);
+            write_host_V(op, Z, Vs[i]);
+            if (stats) {
+                stats[i] = kfc_stats{};
+                stats[i].seconds_total = secs(t0);
+                stats[i].n_converged = int(k);
+            }
+        }
+    };
+    const int T = std::max(1, std::min({n, 16, int(std::thread::hardware_concurrency())}));
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    for (int i : big) {
+        const int rc = ops[i]->cplx ? dense_t<Cx<double>>(ops[i], m, E + size_t(i) * m, Vs[i], stats ? stats + i : nullptr)
+                                    : dense_t<double>(ops[i], m, E + size_t(i) * m, Vs[i], stats ? stats + i : nullptr);
+        if (rc != KFC_OK) return kfc_fail(ops[0]->ctx, rc, "Dense (batch) of operator " + std::to_string(i) + ": " + ops[i]->ctx->err);
     }
     return KFC_OK;
 }
@@ -1515,6 +1626,18 @@ int lobpcg_solve(kfc_op* op, int64_t m, int64_t guard, double tol, int64_t maxit
 }
 
 extern "C" {
+
+int kfc_dense_batch(int32_t n, kfc_op* const* ops, int64_t m, double* E, double* const* Vs, kfc_stats* stats) {
+    if (n < 1 || !ops || !E || !Vs) return kfc_fail(nullptr, KFC_EARG, "kfc_dense_batch: empty batch or null argument");
+    for (int i = 0; i < n; ++i) {
+        if (!ops[i] || !Vs[i]) return kfc_fail(ops[0] ? ops[0]->ctx : nullptr, KFC_EARG, "kfc_dense_batch: null operator or output");
+        if (m < 1 || m > ops[i]->N)
+            return kfc_fail(ops[i]->ctx, KFC_EARG, "kfc_dense_batch: m must lie in 1:prod(dims) of every operator");
+        if (ops[i]->N > 8192)
+            return kfc_fail(ops[i]->ctx, KFC_EUNSUPPORTED, "Dense: prod(dims) above 8192 is not served; use an iterative solver");
+    }
+    return dense_batch(n, ops, m, E, Vs, stats);
+}
 
 int kfc_dense(kfc_op* op, int64_t m, double* E, double* V, kfc_stats* stats) {
     if (!op || !E || !V) return kfc_fail(op ? op->ctx : nullptr, KFC_EARG, "kfc_dense: null argument");

@@ -5,7 +5,7 @@ from ._lib import (Context, AbortError, ArgumentError, BreakdownError, Convergen
 from .bases import ChargeBasis, FockBasis, PeriodicGrid, PhaseGrid, fd_weights, localop
 from .localops import BandedOp, DenseOp, DiagonalOp, LocalOp, ShiftOp, dense, imagpart, realpart
 from .operator import AxisTerm, KronFreeOperator
-from .solvers import (LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, residuals, resolve, solve, solve_batch, solve_batch_subspace, solve_chebyshev_batch,
+from .solvers import (LOBPCG, Auto, ChebyshevFilter, Dense, Lanczos, Solver, residuals, resolve, solve, solve_batch, solve_batch_subspace, solve_chebyshev_batch, solve_dense_batch,
                       solve_stats)
 from .tree import (MaterializedNode, Node, NumericSpec, System, circuit_terms, constant_term, materialize, operator,
                    port_matrix, port_operator)
@@ -16,7 +16,7 @@ __all__ = ["Context", "AbortError", "ArgumentError", "BreakdownError", "Converge
            "UnsupportedError", "context", "lib", "missing_symbols", "ChargeBasis", "FockBasis", "PeriodicGrid",
            "PhaseGrid", "fd_weights", "localop", "BandedOp", "DenseOp", "DiagonalOp", "LocalOp", "ShiftOp", "dense",
            "imagpart", "realpart", "AxisTerm", "KronFreeOperator", "LOBPCG", "Auto", "ChebyshevFilter", "Dense",
-           "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_batch", "solve_batch_subspace", "solve_chebyshev_batch", "solve_stats", "MaterializedNode", "Node",
+           "Lanczos", "Solver", "residuals", "resolve", "solve", "solve_batch", "solve_batch_subspace", "solve_chebyshev_batch", "solve_dense_batch", "solve_stats", "MaterializedNode", "Node",
            "NumericSpec", "System", "circuit_terms", "constant_term", "materialize", "operator", "port_matrix",
            "port_operator", "DriveHamiltonian", "InteractionApply", "KronFreeApply", "evolve_rk4",
            "Resolvent", "apply_resolvent", "complement_solve", "resolvent"]

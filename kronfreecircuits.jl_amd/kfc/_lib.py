@@ -142,6 +142,7 @@ SIGNATURES = {
     "kfc_chebyshev": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_double,
                                 C.c_int32, C.c_double, C.c_uint64, C.c_int32, _D, _D, C.POINTER(Stats)]),
     "kfc_dense": (C.c_int, [_P, C.c_int64, _D, _D, C.POINTER(Stats)]),
+    "kfc_dense_batch": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, _D, C.POINTER(_D), C.POINTER(Stats)]),
     "kfc_residuals": (C.c_int, [_P, C.c_int64, _D, _D, _D]),
     "kfc_chebyshev_batch": (C.c_int, [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int64, C.c_int32, C.c_double,
                                       C.c_int64, C.c_uint64, C.c_int32, _D, C.POINTER(_D), C.POINTER(Stats)]),

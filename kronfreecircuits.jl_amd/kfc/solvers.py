@@ -349,6 +349,23 @@ def solve_batch_subspace(s, Hs, m, B, rowmaps, coeffs=None):
     return out, [x.as_dict() for x in st]
 
 
+def solve_dense_batch(Hs, m):
+    """``solve(Dense(), H, m)`` for every operator of a sweep at once
+    (``kfc_dense_batch``): small operators' matrices formed and diagonalized on host
+    threads.  Returns lists of (E, V) and the stats."""
+    n = len(Hs)
+    if n == 0:
+        return [], []
+    ops = (C.c_void_p * n)(*[H.handle.value for H in Hs])
+    E = np.empty(n * m)
+    Vb = [np.empty(H.N * m * (2 if H.dtype == np.complex128 else 1)) for H in Hs]
+    Vp = (L._D * n)(*[v.ctypes.data_as(L._D) for v in Vb])
+    st = (L.Stats * n)()
+    L.check(L.lib().kfc_dense_batch(n, ops, int(m), E.ctypes.data_as(L._D), Vp, st), Hs[0].ctx.handle)
+    out = [(E[i * m:(i + 1) * m].copy(), unpack_block(Vb[i], Hs[i].N, m, Hs[i].dtype)) for i in range(n)]
+    return out, [x.as_dict() for x in st]
+
+
 def solve_chebyshev_batch(Hs, m, extra=8, degree=60, tol=1e-9, maxiter=200, mode="mixed", seed=0x4346):
     """Batched ChebyshevFilter solves of one-axis operators on one context
     (``kfc_chebyshev_batch``): every filter step of the whole batch is one launch.

@@ -34,7 +34,7 @@ from .bases import PeriodicGrid
 from .operator import KronFreeOperator
 from concurrent.futures import ThreadPoolExecutor
 
-from .solvers import ChebyshevFilter, Dense, Lanczos, solve_chebyshev_batch, solve_stats
+from .solvers import ChebyshevFilter, Dense, Lanczos, solve_chebyshev_batch, solve_dense_batch, solve_stats
 from .tree import (MaterializedNode, Node, circuit_terms, constant_term, fix_phases, materialize, node_axisterms,
                    operator, positive_mask, route)
 from .configs import two_transmons_spec
@@ -124,11 +124,9 @@ class TransmonSweep:
         for (E, V, st), H, leaf in zip(results, leaf_ops, leaf_nodes):
             V = fix_phases(np.asarray(V), positive_mask(leaf, H.dims))
             mats.append(MaterializedNode(leaf, list(H.dims), np.asarray(E), V, [None], H, solver, st))
-        tops = []
-        for i, (root, spec) in enumerate(zip(roots, specs)):
-            kids = mats[2 * i:2 * i + 2]
-            Ht, _ = operator(root, spec, kids=kids)
-            E, V, st = solve_stats(Dense(), Ht, self.k_top)
-            tops.append(MaterializedNode(root, list(Ht.dims), np.asarray(E)[:self.k_top],
-                                         np.asarray(V)[:, :self.k_top], kids, Ht, Dense(), st))
+        # every top's operator, then their Dense solves at once (host threads)
+        Hts = [operator(root, spec, kids=mats[2 * i:2 * i + 2])[0] for i, (root, spec) in enumerate(zip(roots, specs))]
+        res, sts = solve_dense_batch(Hts, self.k_top)
+        tops = [MaterializedNode(root, list(Ht.dims), np.asarray(E), np.asarray(V), mats[2 * i:2 * i + 2], Ht, Dense(), st)
+                for i, (root, Ht, (E, V), st) in enumerate(zip(roots, Hts, res, sts))]
         return tops, time.perf_counter() - t0, t_leaves

@@ -200,3 +200,51 @@ def test_lanczos_fused_moments_vs_dense(mode, variant, march):
     assert np.max(np.abs(E - w[:8])) < 1e-9 * max(1.0, np.max(np.abs(w[:8])))
     assert np.max(O.true_residuals(Ho, E, V)) < 1e-8
     assert np.allclose(V.T @ V, np.eye(8), atol=1e-10)
+
+
+def _hermitian_operator(rng, dims, cplx):
+    """Every LocalOp kind in Hermitian combinations: dense Hermitian factors on two
+    axes, a symmetric banded factor, a shift plus its adjoint, a real diagonal, a
+    constant."""
+    import kfc
+
+    def herm(n):
+        A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+        return (A + A.conj().T) / 2
+    d0, d1 = dims[0], dims[1]
+    w = rng.standard_normal(d1)
+    lo, hi = np.roll(w, 1), w.copy()          # y[i] = lo[i] x[i-1] + c[i] x[i] + hi[i] x[i+1], hi[i] = lo[i+1]
+    f = complex(*rng.standard_normal(2)) if cplx else float(rng.standard_normal())
+    terms = [kfc.AxisTerm(0.7, [(0, kfc.DenseOp(herm(d0))), (1, kfc.DenseOp(herm(d1)))]),
+             kfc.AxisTerm(0.3, [(1, kfc.BandedOp(d1, [-1, 0, 1], [lo, rng.standard_normal(d1), hi]))]),
+             kfc.AxisTerm(0.5, [(0, kfc.ShiftOp(d0, 1, f))]),
+             kfc.AxisTerm(0.5, [(0, kfc.ShiftOp(d0, -1, np.conj(f)))]),
+             kfc.AxisTerm(1.0, [(len(dims) - 1, kfc.DiagonalOp(rng.uniform(-1, 1, dims[-1])))])]
+    return kfc.KronFreeOperator(dims, terms, constant=0.4)
+
+
+def test_dense_host_matrix_and_batch():
+    """Dense of small operators forms Matrix(H) on the host from the terms (every
+    LocalOp kind, real and complex, a constant and a fused diagonal); the result
+    matches the oracle's dense matrix, and kfc_dense_batch over several operators (one
+    above the host limit, through the device path) matches kfc_dense one by one."""
+    import kfc
+    rng = np.random.default_rng(77)
+    Hs = [_hermitian_operator(rng, dims, cplx)
+          for dims, cplx in (([4, 5, 3], False), ([6, 7], True), ([3, 4, 2], True), ([17, 16], False))]
+    m = 5
+    singles = []
+    for H in Hs:
+        E, V = kfc.solve(kfc.Dense(), H, m)
+        M = O.to_matrix(oracle_from_kfc_terms(H))
+        assert np.allclose(M, M.conj().T, atol=1e-12)
+        w = np.linalg.eigvalsh(M)[:m]
+        assert np.max(np.abs(E - w)) <= 1e-10 * max(1.0, np.max(np.abs(w)))
+        r = np.linalg.norm(M @ V - V * E, axis=0)
+        assert np.max(r) <= 1e-9 * max(1.0, np.max(np.abs(w)))
+        singles.append((E, V))
+    batch, st = kfc.solve_dense_batch(Hs, m)
+    for (Eb, Vb), (Es, Vs) in zip(batch, singles):
+        assert np.array_equal(Eb, Es) and np.array_equal(Vb, Vs)
+    with pytest.raises(kfc.ArgumentError):
+        kfc.solve_dense_batch(Hs, 0)
