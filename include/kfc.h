@@ -163,8 +163,9 @@ int64_t kfc_op_size(const kfc_op* op);
 int kfc_op_fused(const kfc_op* op);
 /* Fused-stencil launch shape (no counterpart in the reference: the choice of
  * the project's autotuner).  variant 0 = LDS tile, 1 = register queue,
- * 2 = plane sweep, 3 = plane sweep fed by LDS-DMA loads (2 and 3: constant
- * weights only, others fall back to 1); march = planes per block along the
+ * 2 = plane sweep, 3 = plane sweep fed by LDS-DMA loads, 4 = plane sweep
+ * loading two points a lane (2-4: constant weights only, others fall back to
+ * 1; 4 on real, aligned blocks only, else 2); march = planes per block along the
  * last axis.  Setting variant -1 returns the choice to the autotuner, which
  * times the candidates on the first application of each column count. */
 int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march);

@@ -679,15 +679,26 @@ constexpr int SA_D = 2;
 // without barriers while the next one streams in, with one barrier per batch.
 // DOT (real S): the block also writes sum x_i^2 and sum x_i y_i over its output
 // points to a.dots[2 lg], a.dots[2 lg + 1] (FP64), x the input at the output point.
-template <class S, bool DP, bool ASYNC, bool DOT>
+// PAIR (variant 4, real blocks): each lane loads two neighbouring points of a plane
+// row at once (8-byte loads in FP32, 16-byte in FP64), so a plane takes three load
+// instructions a thread instead of five.  The tile then starts one point further left
+// (columns x0 - 2 .. x0 + 65) so that every pair is aligned; the launcher checks the
+// alignment of the block, its leading dimension and its strides.
+template <class S, bool DP, bool ASYNC, bool DOT, bool PAIR = false>
 __device__ inline void stencil_sweep(const StArgs<S>& a) {
     static_assert(!DOT || !IsCx<S>::value, "the fused dots are for real blocks");
+    static_assert(!PAIR || (!IsCx<S>::value && !ASYNC), "paired loads are for the real plane sweep");
+    constexpr int TW = PAIR ? SP_TX + 4 : SP_W;       // tile row width (points)
+    constexpr int TT = TW * SP_H;                      // tile points
+    constexpr int XOFF = PAIR ? 1 : 0;                 // tile column of x0 - 1
+    constexpr int NPR = TW / 2;                        // pairs a tile row (PAIR)
+    constexpr int NLP = PAIR ? (TT / 2 + 255) / 256 : 1;
     constexpr int DW = sizeof(S) / 4;                 // dwords per point
     constexpr int TDW = SP_TILE * DW;                 // dwords per plane tile
     constexpr int NCH = (TDW + 255) / 256;            // dword chunks per thread
     constexpr int PDW = NCH * 256;                    // padded plane (dwords)
     constexpr int NT_SYNC = ASYNC ? 1 : 2;
-    __shared__ S tile[NT_SYNC][ASYNC ? 1 : SP_TILE];
+    __shared__ S tile[NT_SYNC][ASYNC ? 1 : TT];
     __shared__ __attribute__((aligned(16))) uint32_t bufA[ASYNC ? SA_D * PDW : 1];
     __shared__ __attribute__((aligned(16))) uint32_t bufB[ASYNC ? SA_D * PDW : 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -719,6 +730,22 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
         gok[j] = e < SP_TILE && gx >= 0 && gx < a.d0 && gy >= 0 && gy < a.d1;
         goff[j] = gok[j] ? gx + gy * int(a.s1) : 0;
     }
+    // PAIR: pair e = tid + 256 j of the tile (row e / NPR, columns 2 (e % NPR) + 0, 1);
+    // its plane offset and which of its two points lie in the domain
+    int poff[NLP];
+    int pok[NLP];   // bit 0: first point in the domain, bit 1: second, 4: pair within the tile
+    if constexpr (PAIR) {
+#pragma unroll
+        for (int j = 0; j < NLP; ++j) {
+            const int e = tid + 256 * j;
+            const int r = e / NPR, cp = e - r * NPR;
+            const int gx = x0 - 2 + 2 * cp, gy = y0 - 1 + r;
+            const bool yin = e < TT / 2 && gy >= 0 && gy < a.d1;
+            pok[j] = (yin && gx >= 0 && gx < a.d0 ? 1 : 0) | (yin && gx + 1 >= 0 && gx + 1 < a.d0 ? 2 : 0) |
+                     (e < TT / 2 ? 4 : 0);
+            poff[j] = (pok[j] & 3) ? gx + gy * int(a.s1) : 0;
+        }
+    }
     const int z0 = bz * a.L, z1 = min(z0 + a.L, a.d2);
     // LDS-DMA: dword e = (j * 4 + wave) * 64 + lane of a plane tile is chunk j of
     // this thread; its offset in the plane (dwords), or -1 outside the domain
@@ -749,17 +776,41 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
             }
         }
     };
-    S pre[SP_NL];
+    S pre[PAIR ? 2 * NLP : SP_NL];
     auto fetch = [&](int z) {
         const bool zin = z >= 0 && z < a.d2;
         const S* __restrict__ Xz = Xc + (int64_t)z * a.s2;
+        if constexpr (PAIR) {
+            typedef S v2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-        for (int j = 0; j < SP_NL; ++j) pre[j] = (zin && gok[j]) ? Xz[goff[j]] : zero;
+            for (int j = 0; j < NLP; ++j) {
+                if (zin && (pok[j] & 3) == 3) {
+                    const v2 v = *reinterpret_cast<const v2*>(Xz + poff[j]);
+                    pre[2 * j] = v[0];
+                    pre[2 * j + 1] = v[1];
+                } else {
+                    pre[2 * j] = (zin && (pok[j] & 1)) ? Xz[poff[j]] : zero;
+                    pre[2 * j + 1] = (zin && (pok[j] & 2)) ? Xz[poff[j] + 1] : zero;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SP_NL; ++j) pre[j] = (zin && gok[j]) ? Xz[goff[j]] : zero;
+        }
     };
     auto stash = [&](S* t) {
+        if constexpr (PAIR) {
 #pragma unroll
-        for (int j = 0; j < SP_NL; ++j)
-            if (tid + 256 * j < SP_TILE) t[tid + 256 * j] = pre[j];
+            for (int j = 0; j < NLP; ++j)
+                if (pok[j] & 4) {
+                    t[2 * (tid + 256 * j)] = pre[2 * j];
+                    t[2 * (tid + 256 * j) + 1] = pre[2 * j + 1];
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SP_NL; ++j)
+                if (tid + 256 * j < SP_TILE) t[tid + 256 * j] = pre[j];
+        }
     };
     // per-row diagonal pieces that do not depend on axis 2
     constexpr int NDP = DP ? 4 : 1;   // products of per-axis diagonals (none in most operators)
@@ -795,7 +846,7 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
 #pragma unroll
         for (int rr = 0; rr < SP_R + 2; ++rr)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) v[rr][dx] = t[(r0 + rr) * SP_W + lane + dx];
+            for (int dx = 0; dx < 3; ++dx) v[rr][dx] = t[(r0 + rr) * TW + lane + dx + XOFF];
         // plane zp feeds output zp + 1 (dz = 0 weights), zp (dz = 1) and zp - 1 (dz = 2)
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
@@ -923,6 +974,15 @@ __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
 }
 
 template <class S, bool DP, bool DOT>
+__global__ __launch_bounds__(256) void k_stencil_pp(const StArgs<S> a) {
+    if constexpr (IsCx<S>::value) {
+        stencil_sweep<S, DP, false, DOT>(a);
+    } else {
+        stencil_sweep<S, DP, false, DOT, true>(a);
+    }
+}
+
+template <class S, bool DP, bool DOT>
 __global__ __launch_bounds__(256) void k_stencil_a(const StArgs<S> a) {
     stencil_sweep<S, DP, true, DOT>(a);
 }
@@ -1003,7 +1063,11 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     a.gamma = s_from<S>(gamma);
     bool cw = true;
     for (int p = 0; p < 27; ++p) cw &= !(a.u[p] || a.v[p] || a.w[p]);
-    const int variant = (op->stencil_variant >= 2 && !cw) ? 1 : op->stencil_variant;
+    int variant = (op->stencil_variant >= 2 && !cw) ? 1 : op->stencil_variant;
+    // paired loads need every pair aligned: real blocks, an aligned base, even strides
+    if (variant == 4 && (IsCx<S>::value || reinterpret_cast<uintptr_t>(X) % (2 * sizeof(S)) != 0 || ld % 2 != 0 ||
+                         a.s1 % 2 != 0 || a.s2 % 2 != 0))
+        variant = 2;
     const int ty = variant >= 2 ? SP_TY : ST_TY;
     a.nb0 = int((a.d0 + ST_TX - 1) / ST_TX);
     a.nb1 = int((a.d1 + ty - 1) / ty);
@@ -1020,10 +1084,13 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     const int64_t nblk = (nlog + 7) / 8 * 8;
     if (fused) *fused = false;
     if constexpr (!IsCx<S>::value) {
-        if (dots && ncols == 1 && (variant == 2 || variant == 3)) {
+        if (dots && ncols == 1 && (variant == 2 || variant == 3 || variant == 4)) {
             KFC_HIP(op->ctx, op->dotpart.ensure(sizeof(double) * 2 * size_t(nlog)));
             a.dots = op->dotpart.as<double>();
-            if (variant == 2) {
+            if (variant == 4) {
+                if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_pp<S, true, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+                else hipLaunchKernelGGL((k_stencil_pp<S, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+            } else if (variant == 2) {
                 if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
                 else hipLaunchKernelGGL((k_stencil_p<S, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
             } else {
@@ -1043,6 +1110,9 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     } else if (variant == 3) {
         if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_a<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_stencil_a<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else if (variant == 4) {
+        if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_pp<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_pp<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (variant == 0) {
         hipLaunchKernelGGL(k_stencil<S>, dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (cw) {
@@ -1198,7 +1268,8 @@ static int tune_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64
         return KFC_OK;
     }
     const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}, {2, 16}, {2, 32},
-                                         {2, 64}, {2, 128}, {3, 16}, {3, 32}, {3, 64}, {3, 128}};
+                                         {2, 64}, {2, 128}, {3, 16}, {3, 32}, {3, 64}, {3, 128},
+                                         {4, 16}, {4, 32}, {4, 64}, {4, 128}};
     hipEvent_t e0, e1;
     KFC_HIP(op->ctx, hipEventCreate(&e0));
     KFC_HIP(op->ctx, hipEventCreate(&e1));

@@ -122,8 +122,24 @@ def cyclic_stencil(N, h, c):
     return BandedOp(N, offs, ws)
 
 
+_LOCALOP_MEMO = {}
+
+
 def localop(b, port):
-    """``localop(basis, port)`` (src/bases.jl:162-281); port = (kind, k)."""
+    """``localop(basis, port)`` (src/bases.jl:162-281); port = (kind, k).  The
+    operators are read-only and a function of (basis, port) alone: a sweep's many
+    nodes on the same few grids share them (up to 1024 remembered)."""
+    key = (type(b).__name__, tuple(sorted(vars(b).items())), port[0], port[1])
+    op = _LOCALOP_MEMO.get(key)
+    if op is None:
+        op = _localop(b, port)
+        if len(_LOCALOP_MEMO) >= 1024:
+            _LOCALOP_MEMO.clear()
+        _LOCALOP_MEMO[key] = op
+    return op
+
+
+def _localop(b, port):
     kind, k = port
     if isinstance(b, ChargeBasis):
         n = np.arange(-b.ncut, b.ncut + 1, dtype=float)

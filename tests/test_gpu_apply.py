@@ -71,7 +71,8 @@ def test_fused_path_random(dims, cplx):
     assert _rel(_run(H, X, "f32"), O.apply_chain(Ho, X)) < 50 * TOL32
 
 
-STENCIL_SHAPES = [(0, 64), (1, 16), (1, 64), (2, 16), (2, 64), (2, 128), (3, 16), (3, 64), (3, 128)]
+STENCIL_SHAPES = [(0, 64), (1, 16), (1, 64), (2, 16), (2, 64), (2, 128), (3, 16), (3, 64), (3, 128), (4, 16), (4, 64),
+                  (4, 128)]
 
 
 @pytest.mark.parametrize("variant,march", STENCIL_SHAPES)
@@ -119,7 +120,7 @@ def test_cheb_step_dots(dims, variant, march):
             Bt, At = (torch.tensor(v, dtype=dt, device="cuda:0") for v in (B, A))
             Ct = torch.empty_like(Bt)
             d, fused = H.cheb_step_dots(Ct, Bt, At if use_a else None, a, b, g, prec)
-            assert fused == (variant in (2, 3))
+            assert fused == (variant in (2, 3, 4))
             Cd, Bd = Ct.double().cpu().numpy(), Bt.double().cpu().numpy()
             assert _rel(Cd, Co) < tol
             ref = np.array([Bd @ Bd, Bd @ Cd])
