@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <random>
 #include <thread>
 
@@ -53,19 +54,18 @@ struct BatchDev {
 };
 
 // Y = a (H X) + b X + g Z per problem (a, b, g from coef[3 p .. 3 p + 2]); Z may be null.
+// One block applies one column of one problem (any block size).
 template <class S>
-__global__ __launch_bounds__(256) void k_bapply(const BatchDev<S> d, const S* __restrict__ X, const S* __restrict__ Z,
-                                                S* __restrict__ Y, const double* __restrict__ coef) {
+__device__ inline void bapply_col(const BatchDev<S>& d, const S* __restrict__ X, const S* __restrict__ Z,
+                                  S* __restrict__ Y, const double* __restrict__ coef, int p, int col) {
     using R = typename RealOf<S>::type;
-    const int p = d.act[blockIdx.y];
-    const int col = blockIdx.x;
     const int64_t N = d.N[p];
     const S* Xc = X + d.xo[p] + col * N;
     const S* Zc = Z ? Z + d.xo[p] + col * N : nullptr;
     S* Yc = Y + d.xo[p] + col * N;
     const S* Wp = d.W + d.wo[p];
     const R a = R(coef[3 * p]), b = R(coef[3 * p + 1]), g = R(coef[3 * p + 2]);
-    for (int64_t i = threadIdx.x; i < N; i += 256) {
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
         S acc = s_zero(Xc[0]);
         for (int k = 0; k < d.B; ++k) {
             int64_t j = i + k - d.h;
@@ -76,6 +76,26 @@ __global__ __launch_bounds__(256) void k_bapply(const BatchDev<S> d, const S* __
         S y = s_add(s_scale(a, acc), s_scale(b, Xc[i]));
         if (Zc) y = s_add(y, s_scale(g, Zc[i]));
         Yc[i] = y;
+    }
+}
+
+// Launch shape 0 and 2: a grid of (columns, active problems), 256 or 128 threads.
+template <class S>
+__global__ __launch_bounds__(256) void k_bapply(const BatchDev<S> d, const S* __restrict__ X, const S* __restrict__ Z,
+                                                S* __restrict__ Y, const double* __restrict__ coef) {
+    bapply_col<S>(d, X, Z, Y, coef, d.act[blockIdx.y], blockIdx.x);
+}
+
+// Launch shape 1: a persistent grid walking the (problem, column) work list in order
+// of decreasing problem size (ord: active slots by decreasing N), so that the long
+// columns of uneven batches start first and the short ones fill in behind them.
+template <class S>
+__global__ __launch_bounds__(256) void k_bapply_list(const BatchDev<S> d, const int* __restrict__ ord, int nitems,
+                                                     const S* __restrict__ X, const S* __restrict__ Z,
+                                                     S* __restrict__ Y, const double* __restrict__ coef) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int q = ord[it / d.nb], col = it - (it / d.nb) * d.nb;
+        bapply_col<S>(d, X, Z, Y, coef, d.act[q], col);
     }
 }
 
@@ -287,6 +307,11 @@ struct Batch {
     std::vector<int> fact;
     int fdeg = 0;
     const void* fcoefs = nullptr;
+    // the filter step's launch shape (the tuner's pick for this active set's sizes) and
+    // the active slots by decreasing size for the persistent shape
+    int shape = 0, grid = 0;
+    std::vector<int64_t> shape_key;
+    DevBuf dord;
     Batch() = default;
     Batch(const Batch&) = delete;
     Batch& operator=(const Batch&) = delete;
@@ -364,9 +389,97 @@ int batch_init(Batch<S>& b, kfc_ctx* ctx, const std::vector<Band1D>& bands, int 
 template <class S>
 int set_active(Batch<S>& b, const std::vector<int>& act) {
     b.act = act;
-    if (!act.empty())
+    // active slots in order of decreasing problem size (stable), for the persistent shape
+    std::vector<int> ord(act.size());
+    for (size_t q = 0; q < act.size(); ++q) ord[q] = int(q);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b.N[act[x]] > b.N[act[y]]; });
+    KFC_HIP(b.ctx, b.dord.ensure(sizeof(int) * std::max<size_t>(1, ord.size())));
+    if (!act.empty()) {
         KFC_HIP(b.ctx, hipMemcpyAsync(b.dact.p, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, b.ctx->stream));
+        KFC_HIP(b.ctx, hipMemcpyAsync(b.dord.p, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, b.ctx->stream));
+    }
     KFC_HIP(b.ctx, hipStreamSynchronize(b.ctx->stream));
+    return KFC_OK;
+}
+
+// One filter-step launch over the active set in the batch's launch shape: 0 = a
+// block of 256 threads per (column, problem), 2 = the same with 128 threads, 1 = a
+// persistent grid of b.grid blocks over the size-ordered work list.
+template <class S>
+void launch_bapply(Batch<S>& b, const S* X, const S* Z, S* Y, const double* coef) {
+    const unsigned na = unsigned(b.act.size());
+    hipStream_t st = b.ctx->stream;
+    if (b.shape == 1) {
+        const int nitems = int(na) * b.nb;
+        hipLaunchKernelGGL(k_bapply_list<S>, dim3(unsigned(std::max(1, std::min(b.grid, nitems)))), dim3(256), 0, st,
+                           b.dev, b.dord.template as<int>(), nitems, X, Z, Y, coef);
+    } else {
+        hipLaunchKernelGGL(k_bapply<S>, dim3(b.nb, na), dim3(b.shape == 2 ? 128 : 256), 0, st, b.dev, X, Z, Y, coef);
+    }
+}
+
+// The tuner of the filter step's launch shape: on the first step of each active set
+// of a new size signature (sorted sizes, block width, bands, scalar size) every shape
+// is timed with HIP events over a few launches of that step and the fastest is kept
+// for every later batch with the same signature.  Every shape computes each output
+// entry with the same operations in the same order: the choice changes the speed only.
+std::mutex g_bshape_mu;
+std::map<std::vector<int64_t>, std::pair<int, int>> g_bshape;
+
+template <class S>
+int tune_bapply(Batch<S>& b, const S* X, S* Y, const double* coef) {
+    if (b.act.empty()) return KFC_OK;
+    std::vector<int64_t> key;
+    for (int p : b.act) key.push_back(b.N[p]);
+    std::sort(key.begin(), key.end());
+    key.push_back(b.nb);
+    key.push_back(b.B);
+    key.push_back(int64_t(sizeof(S)));
+    if (key == b.shape_key) return KFC_OK;
+    b.shape_key = key;
+    if (const char* env = std::getenv("KFC_BATCH_SHAPE")) {   // pinned for tests and measurements
+        b.shape = std::atoi(env);
+        b.grid = 8 * std::max(1, b.ctx->num_cus);
+        return KFC_OK;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_bshape_mu);
+        auto it = g_bshape.find(key);
+        if (it != g_bshape.end()) {
+            b.shape = it->second.first;
+            b.grid = it->second.second;
+            return KFC_OK;
+        }
+    }
+    const int cus = std::max(1, b.ctx->num_cus);
+    const std::pair<int, int> cands[] = {{0, 0}, {2, 0}, {1, 4 * cus}, {1, 8 * cus}, {1, 16 * cus}};
+    hipEvent_t e0, e1;
+    KFC_HIP(b.ctx, hipEventCreate(&e0));
+    KFC_HIP(b.ctx, hipEventCreate(&e1));
+    float best = 1e30f;
+    std::pair<int, int> pick = cands[0];
+    int rc = KFC_OK;
+    for (const auto& c : cands) {
+        b.shape = c.first;
+        b.grid = c.second;
+        launch_bapply<S>(b, X, nullptr, Y, coef);   // warm
+        if (hipEventRecord(e0, b.ctx->stream) != hipSuccess) { rc = KFC_EDEVICE; break; }
+        for (int r = 0; r < 3; ++r) launch_bapply<S>(b, X, nullptr, Y, coef);
+        if (hipEventRecord(e1, b.ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = KFC_EDEVICE; break; }
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = KFC_EDEVICE; break; }
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != KFC_OK || hipGetLastError() != hipSuccess) return kfc_fail(b.ctx, KFC_EDEVICE, "batch filter launch tuning failed");
+    b.shape = pick.first;
+    b.grid = pick.second;
+    std::lock_guard<std::mutex> g(g_bshape_mu);
+    g_bshape[key] = pick;
     return KFC_OK;
 }
 
@@ -383,8 +496,7 @@ int bapply(Batch<S>& b, const S* X, const S* Z, S* Y, const std::vector<double>&
     if (b.act.empty()) return KFC_OK;
     KFC_HIP(b.ctx, hipMemcpyAsync(b.dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice,
                                   b.ctx->stream));
-    hipLaunchKernelGGL(k_bapply<S>, dim3(b.nb, unsigned(b.act.size())), dim3(256), 0, b.ctx->stream, b.dev, X, Z, Y,
-                       b.dcoef.template as<double>());
+    launch_bapply<S>(b, X, Z, Y, b.dcoef.template as<double>());
     KFC_HIP(b.ctx, hipGetLastError());
     return KFC_OK;
 }
@@ -397,13 +509,12 @@ int filter_launches(Batch<S>& b, S* X, S* Y, S* Z, int degree) {
     const double* tab = b.dcoefs.template as<double>();
     const size_t stride = 3 * size_t(b.n);
     const dim3 grid(b.nb, unsigned(b.act.size()));
-    hipLaunchKernelGGL(k_bapply<S>, grid, dim3(256), 0, b.ctx->stream, b.dev, X, nullptr, Y, tab);
+    launch_bapply<S>(b, X, nullptr, Y, tab);
     S* x0 = X;
     S* y0 = Y;
     S* z0 = Z;
     for (int k = 2; k <= degree; ++k) {   // z = a H y + b y + g x
-        hipLaunchKernelGGL(k_bapply<S>, grid, dim3(256), 0, b.ctx->stream, b.dev, y0, x0, z0,
-                           tab + size_t(k - 1) * stride);
+        launch_bapply<S>(b, y0, x0, z0, tab + size_t(k - 1) * stride);
         S* t = x0;   // (x, y, z) <- (y, z, x)
         x0 = y0;
         y0 = z0;
@@ -420,9 +531,13 @@ int filter_launches(Batch<S>& b, S* X, S* Y, S* Z, int degree) {
 template <class S>
 int run_filter(Batch<S>& b, S* X, S* Y, S* Z, int degree) {
     if (b.act.empty()) return KFC_OK;
+    const int shape0 = b.shape, grid0 = b.grid;
+    const int trc = tune_bapply<S>(b, X, Y, b.dcoefs.template as<double>());   // outside any capture
+    if (trc != KFC_OK) return trc;
     if (!graphs_enabled()) return filter_launches<S>(b, X, Y, Z, degree);
     hipStream_t st = b.ctx->stream;
-    if (!b.fexec || b.fact != b.act || b.fdeg != degree || b.fcoefs != b.dcoefs.p) {
+    if (!b.fexec || b.fact != b.act || b.fdeg != degree || b.fcoefs != b.dcoefs.p || b.shape != shape0 ||
+        b.grid != grid0) {
         if (b.fexec) {
             KFC_HIP(b.ctx, hipGraphExecDestroy(b.fexec));
             b.fexec = nullptr;

@@ -87,3 +87,25 @@ def test_lockstep_subspace_graph_equals_launches(monkeypatch):
     for e, g in zip(eager, graph):
         assert np.array_equal(e[0], g[0]) and np.array_equal(e[1], g[1])
         assert e[2]["iterations"] == g[2]["iterations"]
+
+
+def test_filter_launch_shapes_equal(monkeypatch):
+    """The batched filter step's launch shapes (a block per (column, problem) of 256
+    or 128 threads, or a persistent grid over the size-ordered work list), pinned one
+    at a time and left to the tuner: the same leaves bit for bit, on an uneven batch."""
+    from kfc import transmon_sweep as T
+    insts = T.draw(6, seed=41)
+    for i, (n1, n2) in enumerate(((64, 256), (320, 64), (96, 160), (256, 128), (64, 64), (512, 96))):
+        insts[i]["N1"], insts[i]["N2"] = n1, n2
+    runs = []
+    for shape in ("0", "1", "2", None):
+        if shape is None:
+            monkeypatch.delenv("KFC_BATCH_SHAPE", raising=False)
+        else:
+            monkeypatch.setenv("KFC_BATCH_SHAPE", shape)
+        runs.append(_run(monkeypatch, True, insts, "mixed"))
+    for other in runs[1:]:
+        for ta, tb in zip(runs[0], other):
+            for la, lb in zip(ta.children, tb.children):
+                assert np.array_equal(la.energies, lb.energies) and np.array_equal(la.vectors, lb.vectors)
+            assert np.array_equal(ta.energies, tb.energies)
