@@ -170,6 +170,12 @@ int kfc_op_fused(const kfc_op* op);
  * times the candidates on the first application of each column count. */
 int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march);
 int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march);
+/* Fills every scratch block the operator holds (the chain path's two intermediate
+ * blocks at both precisions, the fused dots' partials, the time-evolution scratch)
+ * with `value`: the reference's device tests fill H.buf1 / H.buf2 with NaN and
+ * 1e300 before applying (test/cuda/test_device.jl:163-207); here the scratch is
+ * internal, so the tests poison it through this call.  No effect on results. */
+int kfc_op_fill_scratch(kfc_op* op, double value);
 /* Algorithmic work of one column of kfc_apply on the path the operator takes
  * (no counterpart in the reference; the roofline denominators): *bytes = HBM
  * bytes that must move (read x, write y, plus the stored diagonal when the path

@@ -237,6 +237,42 @@ int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march) {
 
 }  // extern "C"
 
+namespace {
+template <class T>
+__global__ void k_fill_scratch(T* p, size_t n, T v) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+template <class T>
+hipError_t fill_scratch(DevBuf& b, T v, hipStream_t st) {
+    if (!b.p) return hipSuccess;
+    const size_t n = b.bytes / sizeof(T);
+    hipLaunchKernelGGL(k_fill_scratch<T>, dim3(unsigned(std::min<size_t>(4096, (n + 255) / 256 + 1))), dim3(256), 0, st,
+                       b.as<T>(), n, v);
+    return hipGetLastError();
+}
+}  // namespace
+
+extern "C" {
+
+int kfc_op_fill_scratch(kfc_op* op, double value) {
+    if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_op_fill_scratch: null operator");
+    hipStream_t st = op->ctx->stream;
+    // FP64 tables' scratch holds doubles (complex: pairs of them), FP32 tables' floats
+    KFC_HIP(op->ctx, fill_scratch<double>(op->dd[KFC_PREC_F64].buf1, value, st));
+    KFC_HIP(op->ctx, fill_scratch<double>(op->dd[KFC_PREC_F64].buf2, value, st));
+    KFC_HIP(op->ctx, fill_scratch<float>(op->dd[KFC_PREC_F32].buf1, float(value), st));
+    KFC_HIP(op->ctx, fill_scratch<float>(op->dd[KFC_PREC_F32].buf2, float(value), st));
+    KFC_HIP(op->ctx, fill_scratch<double>(op->ddz.buf1, value, st));
+    KFC_HIP(op->ctx, fill_scratch<double>(op->ddz.buf2, value, st));
+    KFC_HIP(op->ctx, fill_scratch<double>(op->dotpart, value, st));
+    for (DevBuf& b : op->evo) KFC_HIP(op->ctx, fill_scratch<double>(b, value, st));
+    KFC_HIP(op->ctx, hipStreamSynchronize(st));
+    return KFC_OK;
+}
+
+}  // extern "C"
+
 int check_apply_args(kfc_op* op, const void* X, const void* Y, int64_t ncols, int64_t ld, int32_t& precision,
                      bool& real_on_complex, const char* who) {
     if (precision != KFC_PREC_F64 && precision != KFC_PREC_F32 && precision != KFC_PREC_Z64)

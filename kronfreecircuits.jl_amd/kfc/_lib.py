@@ -107,6 +107,7 @@ SIGNATURES = {
     "kfc_op_fused": (C.c_int, [_P]),
     "kfc_op_stencil_get": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "kfc_op_stencil_set": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "kfc_op_fill_scratch": (C.c_int, [_P, C.c_double]),
     "kfc_op_work": (C.c_int, [_P, _D, _D]),
     "kfc_apply": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_int32]),
     "kfc_apply_noalloc": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int64, C.c_int32]),

@@ -144,6 +144,12 @@ class KronFreeOperator:
         L.check(L.lib().kfc_op_work(self.handle, C.byref(b), C.byref(f)), self.ctx.handle)
         return b.value, f.value
 
+    def fill_scratch(self, value):
+        """Fill the library's internal scratch of this operator with ``value`` (the
+        reference's device tests fill ``H.buf1``/``H.buf2`` with NaN or 1e300 to show
+        that no result reads stale scratch, test/cuda/test_device.jl:163-207)."""
+        L.check(L.lib().kfc_op_fill_scratch(self.handle, float(value)), self.ctx.handle)
+
     def set_stencil(self, variant, march=0):
         """Pin the fused stencil's launch shape; ``variant=-1`` hands it back to the autotuner."""
         L.check(L.lib().kfc_op_stencil_set(self.handle, int(variant), int(march)), self.ctx.handle)

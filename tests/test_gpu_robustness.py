@@ -188,3 +188,48 @@ def test_o2_shrinking_growing_widths_and_views(path):
     Z = torch.zeros((4, N), dtype=dt, device="cuda")
     with pytest.raises(kfc.ArgumentError):
         H.mul(Z[1:3], Z[0:2])                             # overlapping views
+
+
+@pytest.mark.parametrize("path", ["chain", "dense_pair", "fused"])
+def test_o2_stale_scratch(path):
+    """test/cuda/test_device.jl:163-207: the scratch filled with NaN, then with 1e300,
+    between applies of changing widths at both precisions; every result equals the
+    oracle's, and a solve that follows a NaN fill gives valid pairs."""
+    import kfc
+    from kfc import configs
+    rng = np.random.default_rng(46)
+    if path == "chain":
+        dims = [4, 5, 3]
+        terms = random_ops(rng, dims, 4, True)
+        H = to_kfc(dims, terms, constant=0.3)
+        Ho = to_oracle(dims, terms, constant=0.3)
+    elif path == "dense_pair":
+        dims = [9, 7, 6]
+        mk = lambda n: (lambda A: (A + A.T) / 2)(rng.standard_normal((n, n)))
+        H = kfc.KronFreeOperator(dims, [kfc.AxisTerm(1.0, [(1, kfc.DiagonalOp(rng.uniform(0, 2, 7)))]),
+                                        kfc.AxisTerm(0.7, [(0, kfc.DenseOp(mk(9))), (1, kfc.DenseOp(mk(7)))]),
+                                        kfc.AxisTerm(0.4, [(1, kfc.DenseOp(mk(7))), (2, kfc.DenseOp(mk(6)))])],
+                                 block_capacity=8)
+        Ho = oracle_from_kfc_terms(H)
+    else:
+        H = configs.three_fluxonia_operator(configs.three_fluxonia_theta(9), N=20)
+        Ho = oracle_from_kfc_terms(H)
+    cplx = H.dtype == np.complex128
+    for w, fill in ((4, None), (3, float("nan")), (5, 1e300), (2, float("nan"))):
+        if fill is not None:
+            H.fill_scratch(fill)
+        X = rng.standard_normal((w, H.N)) + (1j * rng.standard_normal((w, H.N)) if cplx else 0)
+        ref = O.apply_chain(Ho, X.T).T
+        for prec, tol in (("f64", 1e-12), ("f32", 2e-5)):
+            dt = {("f64", False): torch.float64, ("f32", False): torch.float32,
+                  ("f64", True): torch.complex128, ("f32", True): torch.complex64}[(prec, cplx)]
+            Xt = torch.tensor(X, device="cuda").to(dt)
+            Yt = torch.empty_like(Xt)
+            H.mul(Yt, Xt, prec)
+            got = Yt.to(torch.complex128 if cplx else torch.float64).cpu().numpy()
+            assert np.all(np.isfinite(got)), (path, w, fill, prec)
+            assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref))), (path, w, fill, prec)
+    if path == "dense_pair":   # Hermitian by construction
+        H.fill_scratch(float("nan"))
+        E, V = kfc.solve(kfc.LOBPCG(tol=1e-9), H, 2)
+        _check_pairs(H, E, V, 1e-8)
