@@ -20,6 +20,10 @@ for p in (ROOT, os.path.join(ROOT, "kronfreecircuits.jl_amd")):
 HBM_PEAK_GBS = 8000.0
 
 
+# the fused stencil's kernels by launch-shape variant (kfc_op_stencil_get)
+STENCIL_KERNELS = {0: "k_stencil", 1: "k_stencil_q", 2: "k_stencil_p", 3: "k_stencil_a", 4: "k_stencil_pp"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -669,9 +673,12 @@ def main():
     m = 10
     log(f"[bench] three fluxonia N={args.N}^3 theta={ {k: [round(float(x), 4) for x in v] for k, v in theta.items()} }")
 
+    shapes = []   # the fused stencil's launch shape each solve used (the tuner's pick)
+
     def step(mode):
         H = configs.three_fluxonia_operator(theta, N=args.N)
         E, V, st = kfc.solve_stats(kfc.Lanczos(tol=args.tol, mode=mode, verbosity=args.verbosity), H, m)
+        shapes.append(H.stencil)
         return H, E, st
 
     # apply bandwidth (the dominant kernel)
@@ -699,7 +706,7 @@ def main():
     wall = time.perf_counter() - t0
     value = counted / wall
     st = stats[-1]
-    # roofline of the solve's apply kernel (k_stencil_p): HIP-event time of every
+    # roofline of the solve's apply kernel (the fused stencil): HIP-event time of every
     # apply on the library's stream over the timed solve, per column.  Nearly all
     # are Chebyshev steps y = a Hx + b x + g z: algorithmic bytes 3 N s (read x and
     # z, write y); the explicit check's block apply and the acceptance residuals
@@ -712,7 +719,8 @@ def main():
     traffic, traffic_src = measured_traffic(f"stencil_traffic_{args.N}.json") if args.mode == "f64" else (None, None)
     roof_apply = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                  "kernel": "k_stencil_p",
+                  "kernel": STENCIL_KERNELS.get(shapes[-1][0], f"variant {shapes[-1][0]}"),
+                  "launch_shape": list(shapes[-1]),
                   "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_col * 1e6, 1),
                   "seconds": round(st["seconds_apply_gpu"], 3)}
     # roofline of the block passes (k_tsmm: Gram-only first Gram-Schmidt pass,
