@@ -233,3 +233,36 @@ def test_o2_stale_scratch(path):
         H.fill_scratch(float("nan"))
         E, V = kfc.solve(kfc.LOBPCG(tol=1e-9), H, 2)
         _check_pairs(H, E, V, 1e-8)
+
+
+def test_no_device_allocation_after_warm_up():
+    """test/cuda/test_device.jl:18-36: the operator's applies on a block and on a
+    column view of it equal the host result, and after warm-up an apply on the view
+    allocates no device memory (the scratch blocks are kept at their largest width)."""
+    import kfc
+    rng = np.random.default_rng(61)
+    cd = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    dims = [40, 30, 3]
+    H = kfc.KronFreeOperator(dims, [kfc.AxisTerm(1.0, [(0, kfc.DiagonalOp(np.arange(1, 41).astype(complex)))]),
+                                    kfc.AxisTerm(0.2, [(0, kfc.DenseOp(cd(40, 40))), (1, kfc.DenseOp(cd(30, 30)))]),
+                                    kfc.AxisTerm(0.1, [(1, kfc.ShiftOp(30, 2, 0.5)),
+                                                       (2, kfc.BandedOp(3, [-1, 1], [np.ones(3), np.ones(3)]))])],
+                             block_capacity=6)
+    Ho = oracle_from_kfc_terms(H)
+    Xh = cd(6, 3600)
+    Xd = torch.tensor(Xh, device="cuda")
+    Yd = torch.empty_like(Xd)
+    H.mul(Yd, Xd)
+    assert np.max(np.abs(Yd.cpu().numpy() - O.apply_chain(Ho, Xh.T).T)) <= 1e-12 * np.max(np.abs(Yd.cpu().numpy()))
+    Xv, Yv = Xd[1:5], Yd[1:5]
+    H.mul(Yv, Xv)
+    H.mul(Yv, Xv)
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    for _ in range(10):
+        H.mul(Yv, Xv)
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    assert free1 == free0
+    assert np.max(np.abs(Yv.cpu().numpy() - O.apply_chain(Ho, Xh[1:5].T).T)) <= 1e-12 * np.max(np.abs(Yv.cpu().numpy()))
+    assert np.allclose(H.diagonal(), np.real(np.diag(O.to_matrix(Ho))), atol=1e-12)
