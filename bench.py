@@ -21,7 +21,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 # the fused stencil's kernels by launch-shape variant (kfc_op_stencil_get)
-STENCIL_KERNELS = {0: "k_stencil", 1: "k_stencil_q", 2: "k_stencil_p", 3: "k_stencil_a", 4: "k_stencil_pp"}
+STENCIL_KERNELS = {0: "k_stencil", 1: "k_stencil_q", 2: "k_stencil_p", 3: "k_stencil_a", 4: "k_stencil_pp",
+                   5: "k_stencil_pm"}
 
 
 def log(*a):

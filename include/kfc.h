@@ -164,9 +164,10 @@ int kfc_op_fused(const kfc_op* op);
 /* Fused-stencil launch shape (no counterpart in the reference: the choice of
  * the project's autotuner).  variant 0 = LDS tile, 1 = register queue,
  * 2 = plane sweep, 3 = plane sweep fed by LDS-DMA loads, 4 = plane sweep
- * loading two points a lane (2-4: constant weights only, others fall back to
- * 1; 4 on real, aligned blocks only, else 2); march = planes per block along the
- * last axis.  Setting variant -1 returns the choice to the autotuner, which
+ * loading two points a lane, 5 = as 4 with the stencil positions fixed when the
+ * kernel is compiled (19 or all 27) and buffer addressing (2-5: constant weights
+ * only, others fall back to 1; 4 and 5 on real, aligned blocks only, else 2; 5 on
+ * columns below 2^31 bytes, else 4); march = planes per block along the last axis.  Setting variant -1 returns the choice to the autotuner, which
  * times the candidates on the first application of each column count. */
 int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march);
 int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march);

@@ -667,6 +667,43 @@ __device__ inline S uniform_load(const S* p, int i) {
     }
 }
 
+// Raw buffer access for variant 5: a descriptor per column (num_records = the column's
+// bytes, below 2^31), a 32-bit byte offset per lane and one per plane.  A lane offset
+// with bit 31 set lies past every column, so its load returns zero and its store is
+// dropped.
+constexpr uint32_t ST_OOR = 0x80000000u;
+typedef unsigned st_u2 __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t col_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
+}
+// (the loaded vector is cast whole to a vector of S: a bit cast of one element of an
+// integer vector picks element 0 whatever the index, with this compiler)
+typedef float st_f2 __attribute__((ext_vector_type(2)));
+typedef double st_d2 __attribute__((ext_vector_type(2)));
+__device__ inline void pair_load(float* o, __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    const st_f2 v = __builtin_bit_cast(st_f2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    o[0] = v[0];
+    o[1] = v[1];
+}
+__device__ inline void pair_load(double* o, __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    const st_d2 v = __builtin_bit_cast(st_d2, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+    o[0] = v[0];
+    o[1] = v[1];
+}
+__device__ inline float point_load(float, __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+__device__ inline double point_load(double, __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+// streaming (non-temporal) stores: cache policy bit 1
+__device__ inline void point_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, soff, 2);
+}
+__device__ inline void point_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(st_u2, v), r, off, soff, 2);
+}
+
 // Zeros that the LDS-DMA variant loads for points outside the domain.
 __device__ uint32_t g_lds_zero[64] = {0};
 
@@ -684,10 +721,18 @@ constexpr int SA_D = 2;
 // instructions a thread instead of five.  The tile then starts one point further left
 // (columns x0 - 2 .. x0 + 65) so that every pair is aligned; the launcher checks the
 // alignment of the block, its leading dimension and its strides.
-template <class S, bool DP, bool ASYNC, bool DOT, bool PAIR = false>
+// MASK != 0 (variant 5, with PAIR): the stencil positions used are a subset of MASK,
+// known when the kernel is compiled (a.c is zero at the others), so the position tests
+// and the weights' bookkeeping leave the plane loop; the three partial sums then live
+// in three slots indexed by output plane mod 3 (the loop is unrolled three planes deep),
+// so no register moves rotate them from plane to plane.
+template <class S, bool DP, bool ASYNC, bool DOT, bool PAIR = false, uint32_t MASK = 0>
 __device__ inline void stencil_sweep(const StArgs<S>& a) {
     static_assert(!DOT || !IsCx<S>::value, "the fused dots are for real blocks");
     static_assert(!PAIR || (!IsCx<S>::value && !ASYNC), "paired loads are for the real plane sweep");
+    static_assert(MASK == 0 || PAIR, "compile-time masks are for the paired plane sweep");
+    static_assert(MASK == 0 || (MASK & 0x1FFu) != 0, "a compile-time mask reaches the plane above");
+    constexpr bool ROT = MASK != 0;
     constexpr int TW = PAIR ? SP_TX + 4 : SP_W;       // tile row width (points)
     constexpr int TT = TW * SP_H;                      // tile points
     constexpr int XOFF = PAIR ? 1 : 0;                 // tile column of x0 - 1
@@ -777,10 +822,31 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
         }
     };
     S pre[PAIR ? 2 * NLP : SP_NL];
+    // ROT: column descriptors and byte offsets (a pair lies wholly inside or wholly
+    // outside the domain, since the paired sweep runs on an even d0)
+    const uint32_t colb = uint32_t(int64_t(a.d0) * a.d1 * a.d2 * int64_t(sizeof(S)));
+    __amdgpu_buffer_rsrc_t xrs, yrs, ars;
+    uint32_t pb[ROT ? NLP : 1];
+    if constexpr (ROT) {
+        xrs = col_rsrc(Xc, colb);
+        yrs = col_rsrc(Yc, colb);
+        ars = col_rsrc(Ac ? Ac : Xc, colb);
+#pragma unroll
+        for (int j = 0; j < NLP; ++j) pb[j] = (pok[j] & 3) == 3 ? uint32_t(poff[j]) * uint32_t(sizeof(S)) : ST_OOR;
+    }
     auto fetch = [&](int z) {
         const bool zin = z >= 0 && z < a.d2;
         const S* __restrict__ Xz = Xc + (int64_t)z * a.s2;
-        if constexpr (PAIR) {
+        if constexpr (ROT) {
+            if (zin) {
+                const uint32_t so = uint32_t(z) * uint32_t(a.s2) * uint32_t(sizeof(S));
+#pragma unroll
+                for (int j = 0; j < NLP; ++j) pair_load(&pre[2 * j], xrs, pb[j], so);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 2 * NLP; ++j) pre[j] = zero;
+            }
+        } else if constexpr (PAIR) {
             typedef S v2 __attribute__((ext_vector_type(2)));
 #pragma unroll
             for (int j = 0; j < NLP; ++j) {
@@ -836,10 +902,151 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
             dp01[k][r] = x;
         }
     }
+    double sxx = 0, sxy = 0;
+    // output plane zp - 1 is complete: y = alpha acc + beta x (+ gamma A) at its rows
+    auto emit = [&](int zp, const S* accv, const S* ctrv) {
+        if (zp - 1 >= z0 && xin) {
+            const int64_t zo = (int64_t)(zp - 1) * a.s2;
+#pragma unroll
+            for (int r = 0; r < SP_R; ++r) {
+                const int y = y0 + r0 + r;
+                if (y < a.d1) {
+                    const int64_t idx = (int64_t)i0 + (int64_t)y * a.s1 + zo;
+                    S out = s_fma(s_mul(a.alpha, accv[r]), a.beta, ctrv[r]);
+                    if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
+                    store_stream(Yc + idx, out);
+                    if constexpr (DOT) {
+                        const double xv = double(ctrv[r]);
+                        sxx = fma(xv, xv, sxx);
+                        sxy = fma(xv, double(out), sxy);
+                    }
+                }
+            }
+        }
+    };
+    // ROT: the rows' byte offsets in a plane (bit 31 outside the domain)
+    uint32_t yb[ROT ? SP_R : 1];
+    if constexpr (ROT) {
+#pragma unroll
+        for (int r = 0; r < SP_R; ++r) {
+            const int y = y0 + r0 + r;
+            yb[r] = (xin && y < a.d1) ? uint32_t(i0 + y * int(a.s1)) * uint32_t(sizeof(S)) : ST_OOR;
+        }
+    }
+    // A at the rows of output plane zp - 1, read before the plane's work (buffer
+    // accesses through different descriptors keep their program order, so a read
+    // placed between the stores would wait for each)
+    auto a_rows = [&](int zp, S* av) {
+        if constexpr (ROT) {
+            if (Ac && zp - 1 >= z0) {
+                const uint32_t so = uint32_t(zp - 1) * uint32_t(a.s2) * uint32_t(sizeof(S));
+#pragma unroll
+                for (int r = 0; r < SP_R; ++r) av[r] = point_load(zero, ars, yb[r], so);
+            }
+        }
+    };
+    auto emit_b = [&](int zp, const S* accv, const S* ctrv, const S* av) {
+        if constexpr (ROT) {
+        if (zp - 1 >= z0) {
+            const uint32_t so = uint32_t(zp - 1) * uint32_t(a.s2) * uint32_t(sizeof(S));
+#pragma unroll
+            for (int r = 0; r < SP_R; ++r) {
+                S out = s_fma(s_mul(a.alpha, accv[r]), a.beta, ctrv[r]);
+                if (Ac) out = s_fma(out, a.gamma, av[r]);
+                point_store(out, yrs, yb[r], so);
+                if constexpr (DOT) {
+                    if (yb[r] != ST_OOR) {
+                        const double xv = double(ctrv[r]);
+                        sxx = fma(xv, xv, sxx);
+                        sxy = fma(xv, double(out), sxy);
+                    }
+                }
+            }
+        }
+        }
+    };
+    if constexpr (ROT) {
+        // slot k holds the partial sums and the centre value of the output planes
+        // o with o - (z0 - 1) = k mod 3
+        S acc[3][SP_R], ctr[3][SP_R];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int r = 0; r < SP_R; ++r) acc[k][r] = ctr[k][r] = zero;
+        // plane zp (in slot U) feeds outputs zp + 1, zp, zp - 1 (slots U + 1, U, U + 2 mod 3)
+        auto plane3 = [&](const S* __restrict__ t, int zp, auto KU) {
+            constexpr int U = decltype(KU)::value, kP = (U + 1) % 3, k0 = U, kM = (U + 2) % 3;
+            // plane zp is the first to reach output zp + 1: its first term starts that
+            // slot afresh (what the slot held belonged to output zp - 2, emitted)
+            constexpr int QF = __builtin_ctz(MASK & 0x1FFu);
+            S av[SP_R];
+            a_rows(zp, av);
+            S v[SP_R + 2][3];
+#pragma unroll
+            for (int rr = 0; rr < SP_R + 2; ++rr)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) v[rr][dx] = t[(r0 + rr) * TW + lane + dx + XOFF];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const int dy = q / 3, dx = q % 3;
+                if ((MASK >> q) & 1u) {
+                    const S w = a.c[q];
+#pragma unroll
+                    for (int r = 0; r < SP_R; ++r)
+                        acc[kP][r] = q == QF ? s_mul(w, v[r + dy][dx]) : s_fma(acc[kP][r], w, v[r + dy][dx]);
+                }
+                if ((MASK >> (9 + q)) & 1u) {
+                    const S w = a.c[9 + q];
+#pragma unroll
+                    for (int r = 0; r < SP_R; ++r) acc[k0][r] = s_fma(acc[k0][r], w, v[r + dy][dx]);
+                }
+                if ((MASK >> (18 + q)) & 1u) {
+                    const S w = a.c[18 + q];
+#pragma unroll
+                    for (int r = 0; r < SP_R; ++r) acc[kM][r] = s_fma(acc[kM][r], w, v[r + dy][dx]);
+                }
+            }
+            if (zp >= z0 && zp < z1) {
+                const S d2 = a.D2 ? uniform_load(a.D2, zp) : zero;
+                S dp2[NDP];
+#pragma unroll
+                for (int k = 0; k < NDP; ++k)
+                    dp2[k] = (DP && k < a.ndp) ? (a.dp2[k] ? uniform_load(a.dp2[k], zp) : one) : zero;
+#pragma unroll
+                for (int r = 0; r < SP_R; ++r) {
+                    S dv = s_add(d01[r], d2);
+#pragma unroll
+                    for (int k = 0; k < NDP; ++k)
+                        if (DP && k < a.ndp) dv = s_fma(dv, dp01[k][r], dp2[k]);
+                    const int y = y0 + r0 + r;
+                    if (a.Dfull && xin && y < a.d1)
+                        dv = s_add(dv, a.Dfull[(int64_t)i0 + (int64_t)y * a.s1 + (int64_t)zp * a.s2]);
+                    ctr[k0][r] = v[r + 1][1];
+                    acc[k0][r] = s_fma(acc[k0][r], dv, ctr[k0][r]);
+                }
+            }
+            emit_b(zp, acc[kM], ctr[kM], av);
+        };
+        int buf = 0;
+        auto step = [&](int zp, auto KU) {
+            if (zp + 1 <= z1) fetch(zp + 1);
+            plane3(tile[buf], zp, KU);
+            if (zp + 1 <= z1) stash(tile[buf ^ 1]);
+            __syncthreads();
+            buf ^= 1;
+        };
+        fetch(z0 - 1);
+        stash(tile[0]);
+        __syncthreads();
+        for (int zp = z0 - 1; zp <= z1; zp += 3) {
+            step(zp, std::integral_constant<int, 0>());
+            if (zp + 1 <= z1) step(zp + 1, std::integral_constant<int, 1>());
+            if (zp + 2 <= z1) step(zp + 2, std::integral_constant<int, 2>());
+        }
+    } else {
     S accM[SP_R], acc0[SP_R], accP[SP_R], ctrM[SP_R], ctr0[SP_R];
 #pragma unroll
     for (int r = 0; r < SP_R; ++r) accM[r] = acc0[r] = accP[r] = ctrM[r] = ctr0[r] = zero;
-    double sxx = 0, sxy = 0;
     auto plane = [&](const S* __restrict__ t, int zp) {
         // rows r0 .. r0 + SP_R + 1 of the patch, columns lane .. lane + 2
         S v[SP_R + 2][3];
@@ -890,25 +1097,7 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
                 acc0[r] = s_fma(acc0[r], dv, ctr0[r]);
             }
         }
-        // output zp - 1 is complete
-        if (zp - 1 >= z0 && xin) {
-            const int64_t zo = (int64_t)(zp - 1) * a.s2;
-#pragma unroll
-            for (int r = 0; r < SP_R; ++r) {
-                const int y = y0 + r0 + r;
-                if (y < a.d1) {
-                    const int64_t idx = (int64_t)i0 + (int64_t)y * a.s1 + zo;
-                    S out = s_fma(s_mul(a.alpha, accM[r]), a.beta, ctrM[r]);
-                    if (Ac) out = s_fma(out, a.gamma, Ac[idx]);
-                    store_stream(Yc + idx, out);
-                    if constexpr (DOT) {
-                        const double xv = double(ctrM[r]);
-                        sxx = fma(xv, xv, sxx);
-                        sxy = fma(xv, double(out), sxy);
-                    }
-                }
-            }
-        }
+        emit(zp, accM, ctrM);   // output zp - 1 is complete
 #pragma unroll
         for (int r = 0; r < SP_R; ++r) {
             accM[r] = acc0[r];
@@ -952,6 +1141,7 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
             __syncthreads();
         }
     }
+    }
     if constexpr (DOT) {
         __shared__ double red[2][4];
 #pragma unroll
@@ -979,6 +1169,35 @@ __global__ __launch_bounds__(256) void k_stencil_pp(const StArgs<S> a) {
         stencil_sweep<S, DP, false, DOT>(a);
     } else {
         stencil_sweep<S, DP, false, DOT, true>(a);
+    }
+}
+
+// Position masks of variant 5 (bit 9 g + 3 dy + dx; g = 0, 1, 2 for the plane above,
+// at and below the output): every position but the eight corners of the cube (a
+// 7-point Laplacian with couplings in each coordinate plane), or all 27.
+constexpr uint32_t ST_MASK19 = 0x7FFFFFFu & ~((1u << 0) | (1u << 2) | (1u << 6) | (1u << 8) | (1u << 18) | (1u << 20) |
+                                              (1u << 24) | (1u << 26));
+constexpr uint32_t ST_MASK27 = 0x7FFFFFFu;
+
+template <class S, bool DP, bool DOT, uint32_t MASK>
+__global__ __launch_bounds__(256) void k_stencil_pm(const StArgs<S> a) {
+    if constexpr (IsCx<S>::value) {
+        stencil_sweep<S, DP, false, DOT>(a);   // never launched: variant 5 takes real blocks
+    } else {
+        stencil_sweep<S, DP, false, DOT, true, MASK>(a);
+    }
+}
+
+// Variant 5 on the smallest compile-time mask that holds every position used.
+template <class S, bool DOT>
+static void launch_pm(const StArgs<S>& a, int64_t nblk, hipStream_t st) {
+    const bool m19 = (a.used & ~ST_MASK19) == 0;
+    if (a.ndp > 0) {
+        if (m19) hipLaunchKernelGGL((k_stencil_pm<S, true, DOT, ST_MASK19>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_pm<S, true, DOT, ST_MASK27>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else {
+        if (m19) hipLaunchKernelGGL((k_stencil_pm<S, false, DOT, ST_MASK19>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_stencil_pm<S, false, DOT, ST_MASK27>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
     }
 }
 
@@ -1065,7 +1284,9 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     for (int p = 0; p < 27; ++p) cw &= !(a.u[p] || a.v[p] || a.w[p]);
     int variant = (op->stencil_variant >= 2 && !cw) ? 1 : op->stencil_variant;
     // paired loads need every pair aligned: real blocks, an aligned base, even strides
-    if (variant == 4 && (IsCx<S>::value || reinterpret_cast<uintptr_t>(X) % (2 * sizeof(S)) != 0 || ld % 2 != 0 ||
+    // variant 5 addresses a column with 32-bit byte offsets
+    if (variant == 5 && int64_t(a.d0) * a.d1 * a.d2 * int64_t(sizeof(S)) >= (int64_t(1) << 31)) variant = 4;
+    if ((variant == 4 || variant == 5) && (IsCx<S>::value || reinterpret_cast<uintptr_t>(X) % (2 * sizeof(S)) != 0 || ld % 2 != 0 ||
                          a.s1 % 2 != 0 || a.s2 % 2 != 0))
         variant = 2;
     const int ty = variant >= 2 ? SP_TY : ST_TY;
@@ -1084,10 +1305,12 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     const int64_t nblk = (nlog + 7) / 8 * 8;
     if (fused) *fused = false;
     if constexpr (!IsCx<S>::value) {
-        if (dots && ncols == 1 && (variant == 2 || variant == 3 || variant == 4)) {
+        if (dots && ncols == 1 && (variant >= 2 && variant <= 5)) {
             KFC_HIP(op->ctx, op->dotpart.ensure(sizeof(double) * 2 * size_t(nlog)));
             a.dots = op->dotpart.as<double>();
-            if (variant == 4) {
+            if (variant == 5) {
+                launch_pm<S, true>(a, nblk, st);
+            } else if (variant == 4) {
                 if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_pp<S, true, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
                 else hipLaunchKernelGGL((k_stencil_pp<S, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
             } else if (variant == 2) {
@@ -1110,6 +1333,8 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
     } else if (variant == 3) {
         if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_a<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_stencil_a<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+    } else if (variant == 5) {
+        if constexpr (!IsCx<S>::value) launch_pm<S, false>(a, nblk, st);
     } else if (variant == 4) {
         if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_pp<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_stencil_pp<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
@@ -1269,7 +1494,8 @@ static int tune_stencil(kfc_op* op, const void* X, const void* A, void* Y, int64
     }
     const std::pair<int, int> cands[] = {{0, 64}, {1, 16}, {1, 32}, {1, 64}, {1, 128}, {2, 16}, {2, 32},
                                          {2, 64}, {2, 128}, {3, 16}, {3, 32}, {3, 64}, {3, 128},
-                                         {4, 16}, {4, 32}, {4, 64}, {4, 128}};
+                                         {4, 16}, {4, 32}, {4, 64}, {4, 128}, {5, 16}, {5, 32}, {5, 64},
+                                         {5, 128}};
     hipEvent_t e0, e1;
     KFC_HIP(op->ctx, hipEventCreate(&e0));
     KFC_HIP(op->ctx, hipEventCreate(&e1));

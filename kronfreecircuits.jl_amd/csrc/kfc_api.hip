@@ -221,7 +221,7 @@ int kfc_op_stencil_get(const kfc_op* op, int32_t* variant, int32_t* march) {
 
 int kfc_op_stencil_set(kfc_op* op, int32_t variant, int32_t march) {
     if (!op) return kfc_fail(nullptr, KFC_EARG, "kfc_op_stencil_set: null operator");
-    if (variant < -1 || variant > 4) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: variant must lie in -1:4");
+    if (variant < -1 || variant > 5) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: variant must lie in -1:5");
     if (march < 0 || march > 4096) return kfc_fail(op->ctx, KFC_EARG, "kfc_op_stencil_set: march must lie in 0:4096");
     if (variant < 0) {
         op->tuned_cols[0] = op->tuned_cols[1] = op->tuned_cols[2] = -1;

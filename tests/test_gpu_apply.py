@@ -72,7 +72,7 @@ def test_fused_path_random(dims, cplx):
 
 
 STENCIL_SHAPES = [(0, 64), (1, 16), (1, 64), (2, 16), (2, 64), (2, 128), (3, 16), (3, 64), (3, 128), (4, 16), (4, 64),
-                  (4, 128)]
+                  (4, 128), (5, 16), (5, 64), (5, 128)]
 
 
 @pytest.mark.parametrize("variant,march", STENCIL_SHAPES)
@@ -90,6 +90,27 @@ def test_stencil_variants(dims, cplx, variant, march):
     H.set_stencil(variant, march)
     Ho = to_oracle(dims, terms, constant=0.25)
     X = rng.standard_normal((H.N, 3)) + (1j * rng.standard_normal((H.N, 3)) if cplx else 0)
+    Yo = O.apply_chain(Ho, X)
+    assert _rel(_run(H, X), Yo) < TOL64
+    assert _rel(_run(H, X, "f32"), Yo) < 50 * TOL32
+    assert H.stencil == (variant, march)
+
+
+@pytest.mark.parametrize("variant,march", [(2, 64), (4, 64), (5, 16), (5, 64), (5, 128)])
+@pytest.mark.parametrize("dims", [[70, 20, 11], [64, 33, 40], [66, 17, 130]])
+def test_stencil_cube_corners(dims, variant, march):
+    """A kept term with offsets on all three axes reaches the eight corners of the
+    stencil cube, outside the 19-position set (the plane sweeps then run on all 27
+    positions); with a per-row band along axis 0 beside it, every weight constant."""
+    rng = np.random.default_rng(hash((tuple(dims), variant, march, 5)) % 2**32)
+    terms = _fusable_terms(rng, dims, False)[:-1]
+    terms.append((0.31, [(0, "banded", ([-1, 1], [np.full(dims[0], 0.8), np.full(dims[0], -1.3)])),
+                         (1, "shift", (1, 0.6)), (2, "banded", ([-1, 1], [np.full(dims[2], 1.1), np.full(dims[2], 0.4)]))]))
+    H = to_kfc(dims, terms, constant=-0.125)
+    assert H.fused
+    H.set_stencil(variant, march)
+    Ho = to_oracle(dims, terms, constant=-0.125)
+    X = rng.standard_normal((H.N, 3))
     Yo = O.apply_chain(Ho, X)
     assert _rel(_run(H, X), Yo) < TOL64
     assert _rel(_run(H, X, "f32"), Yo) < 50 * TOL32
@@ -120,7 +141,7 @@ def test_cheb_step_dots(dims, variant, march):
             Bt, At = (torch.tensor(v, dtype=dt, device="cuda:0") for v in (B, A))
             Ct = torch.empty_like(Bt)
             d, fused = H.cheb_step_dots(Ct, Bt, At if use_a else None, a, b, g, prec)
-            assert fused == (variant in (2, 3, 4))
+            assert fused == (variant in (2, 3, 4, 5))
             Cd, Bd = Ct.double().cpu().numpy(), Bt.double().cpu().numpy()
             assert _rel(Cd, Co) < tol
             ref = np.array([Bd @ Bd, Bd @ Cd])
