@@ -1570,7 +1570,7 @@ static int apply_fixed(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_
                        hipStream_t st) {
     if (op->plan.ok) return launch_stencil<S>(op, X, nullptr, Y, ncols, ld, cd(1), cd(0), cd(0), st);
     if constexpr (!IsCx<S>::value)
-        if (op->pairs.ok) return launch_pairs<S>(op, X, Y, ncols, ld, st);
+        if (op->pairs.ok) return launch_pairs<S>(op, X, Y, ncols, ld, st, false);
     return launch_chain<S>(op, X, Y, ncols, ld, st, buf1, buf2);
 }
 

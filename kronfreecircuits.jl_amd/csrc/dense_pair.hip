@@ -55,6 +55,7 @@ struct PairArgs {
     T c[MAXT];
     const T* A[MAXT];     // dp x dp column-major
     const T* B[MAXT];     // dq x dq column-major
+    T* Z;                 // N x ncols scratch of the split form (null: the group runs whole)
 };
 
 // P: padded extent (multiple of 16) bounding dp and dq.  Slabs s0, s0 + step, ...
@@ -914,6 +915,94 @@ void k_dense_pair_staged_batch(const PairArgs<T>* __restrict__ tab) {
     dense_pair_chunks_staged<T, PI, PJ, NT, staged_ws<T>()>(a, xcd_block(int(blockIdx.x), int(gridDim.x)), gridDim.x);
 }
 
+// The split form of a one-term group whose p axis is strided (L >= 16 entries of the
+// axes before it), as two mode products that each read and write whole runs of
+// contiguous entries:
+//   STAGE 1   Z[.., j, r] = sum_j' (c B)[j, j'] X[.., j', r]     the q axis, on runs of
+//                                                              stride(q) = L dp M entries
+//   STAGE 2   Y[l, i, ..] (+)= sum_i' A[i, i'] Z[l, i', ..]      the p axis, on runs of
+//                                                              L entries
+// Each is a product of a d x d matrix G with the d columns of an (rows x d) block (column
+// stride rows, blocks rows * d apart, block columns ld apart):  out^T = G in^T on
+// 16 x 16 x 4 MFMA tiles, a wave per 16 consecutive rows, G's fragments in registers
+// for the whole launch.  The loads and stores of a tile move 16 consecutive entries of
+// each column (a 128-byte line in FP64) instead of the slab kernels' gathers at stride
+// L.  Traffic: X read and Z written, then Z, Y (or X and the diagonal) read and Y
+// written; the slab kernels read X and Y and write Y once, but through 8-byte pieces.
+// MODE 0: out = G in (stage 1); 1: out += G in; 2: out = D .* X + G in (the group that
+// writes Y first).
+template <class T, int KS, int STAGE, int MODE>
+__device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep) {
+    using v4 = typename V4<T>::type;
+    constexpr int NTJ = KS / 4;   // 16-wide output tiles; KS contraction steps of 4 bound d
+    // tile counts and extents fit 32 bits (split_ok checks them)
+    const int d = STAGE == 1 ? a.dq : a.dp;
+    const uint32_t rows = uint32_t(STAGE == 1 ? a.sq : a.sp);
+    const uint32_t nout = uint32_t(STAGE == 1 ? a.R : a.M * a.dq * a.R);
+    const uint32_t ncol = uint32_t(a.nslab / (a.L * a.M * a.R));
+    const T* __restrict__ in = STAGE == 1 ? a.X : a.Z;
+    T* __restrict__ out = STAGE == 1 ? a.Z : a.Y;
+    const T* __restrict__ G = STAGE == 1 ? a.B[0] : a.A[0];
+    const T gc = STAGE == 1 ? a.c[0] : T(1);
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    // A operand of step ks for output tile tj: (c) G[16 tj + r16, 4 ks + k4]
+    T g[NTJ][KS];
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int j = 16 * tj + r16, jp = 4 * ks + k4;
+            g[tj][ks] = (j < d && jp < d) ? gc * G[j + (int64_t)jp * d] : T(0);
+        }
+    const uint32_t ntr = (rows + 15) / 16, total = ntr * nout * ncol;
+    const uint32_t wave = uint32_t(w0) + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    for (uint32_t t = wave; t < total; t += uint32_t(wstep)) {
+        const uint32_t rt = t % ntr, rest = t / ntr, o = rest % nout, col = rest / nout;
+        const int64_t cb = int64_t(o) * rows * d, base = int64_t(col) * a.ld + cb;
+        const uint32_t row = 16 * rt + r16;
+        const bool rok = row < rows;
+        T b[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int jp = 4 * ks + k4;
+            b[ks] = (rok && jp < d) ? in[base + row + (int64_t)jp * rows] : T(0);
+        }
+        v4 acc[NTJ];
+#pragma unroll
+        for (int tj = 0; tj < NTJ; ++tj) acc[tj] = v4{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+            if (4 * ks < d)   // uniform: steps past d are padding
+#pragma unroll
+                for (int tj = 0; tj < NTJ; ++tj) acc[tj] = mfma(g[tj][ks], b[ks], acc[tj]);
+        // lane (r16, k4) holds out[row, 16 tj + out_row(k4, v)]
+#pragma unroll
+        for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * tj + out_row(T(0), k4, v);
+                if (rok && j < d) {
+                    const int64_t e = row + (int64_t)j * rows;
+                    T r = acc[tj][v];
+                    if constexpr (MODE == 1) r += out[base + e];
+                    if constexpr (MODE == 2) r += a.D[cb + e] * a.X[base + e];
+                    out[base + e] = r;
+                }
+            }
+    }
+}
+
+template <class T, int KS, int STAGE, int MODE>
+__global__ __launch_bounds__(256) void k_pair_mode(const PairArgs<T> a) {
+    pair_mode<T, KS, STAGE, MODE>(a, int64_t(blockIdx.x) * 4, int64_t(gridDim.x) * 4);
+}
+
+template <class T, int KS, int STAGE, int MODE>
+__global__ __launch_bounds__(256) void k_pair_mode_batch(const PairArgs<T>* __restrict__ tab) {
+    const PairArgs<T> a = tab[blockIdx.y];
+    pair_mode<T, KS, STAGE, MODE>(a, int64_t(blockIdx.x) * 4, int64_t(gridDim.x) * 4);
+}
+
 // Blocks of kernel K that fit on the device at once (occupancy x compute units): a
 // chain-kernel grid is capped there, so that every block works through many slabs and
 // its setup (A_t into LDS, the B_t fragments) is paid once per block, not per slab.
@@ -1033,6 +1122,16 @@ int op_build_pairs(kfc_op* op) {
     }
     for (const PairGroup& g : P.groups)
         if (g.A.size() > size_t(MAXT)) return KFC_OK;
+    // the first group writes Y = D .* X + ...: a group the split form may take goes after
+    // the others, whose slab kernels form D .* X from the X they read anyway (the split
+    // form's second stage would read X once more for it)
+    {
+        std::vector<int64_t> st(op->ndims + 1, 1);
+        for (int a = 1; a <= op->ndims; ++a) st[a] = st[a - 1] * op->dims[a - 1];
+        std::stable_partition(P.groups.begin(), P.groups.end(), [&](const PairGroup& g) {
+            return !(g.A.size() == 1 && st[g.p] >= 16 && g.dp <= 64 && g.dq <= 64);
+        });
+    }
     P.ok = true;
     return KFC_OK;
 }
@@ -1070,8 +1169,31 @@ template int pairs_upload<double>(kfc_op*, DeviceData&);
 template int pairs_upload<float>(kfc_op*, DeviceData&);
 
 // The launch arguments of group gi of op on ncols columns of X -> Y.
+// Whether some group of op can run in the split form (then its applies want N x ncols
+// scalars of scratch).
+static bool op_wants_pairz(const kfc_op* op) {
+    int64_t stride = 1;
+    std::vector<int64_t> st(op->ndims + 1, 1);
+    for (int a = 0; a < op->ndims; ++a) {
+        st[a] = stride;
+        stride *= op->dims[a];
+    }
+    for (const PairGroup& g : op->pairs.groups)
+        if (g.A.size() == 1 && st[g.p] >= 16 && g.dp <= 64 && g.dq <= 64) return true;
+    return false;
+}
+
+// The context's scratch for the split form, grown to hold `bytes`; null when no group of
+// op splits or when the allocation fails (the groups then run whole).
+static void* pairz_for(kfc_op* op, size_t bytes) {
+    if (!op_wants_pairz(op)) return nullptr;
+    return op->ctx->pairz.ensure(bytes) == hipSuccess ? op->ctx->pairz.p : nullptr;
+}
+
+// z: N x ncols scalars of scratch for the split form (null: every group runs whole).
 template <class T>
-static PairArgs<T> pair_args(const kfc_op* op, size_t gi, const void* X, void* Y, int64_t ncols, int64_t ld) {
+static PairArgs<T> pair_args(const kfc_op* op, size_t gi, const void* X, void* Y, int64_t ncols, int64_t ld,
+                             void* z = nullptr) {
     const int prec = std::is_same<T, double>::value ? 0 : 1;
     const DeviceData& dd = op->dd[prec];
     std::vector<int64_t> stride(op->ndims + 1, 1);
@@ -1100,6 +1222,7 @@ static PairArgs<T> pair_args(const kfc_op* op, size_t gi, const void* X, void* Y
         a.A[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t]);
         a.B[t] = static_cast<const T*>(dd.pair_ptr[gi][2 * t + 1]);
     }
+    a.Z = static_cast<T*>(z);
     return a;
 }
 
@@ -1123,11 +1246,13 @@ bool pairs_same_structure(const kfc_op* a, const kfc_op* b) {
 // work in flight (a single operator's columns or a whole lockstep sweep), so the
 // launcher times them per group on the first application of each structure and keeps
 // the winner.
+// Family 4 runs a one-term group with a strided p axis in the split form (pair_mode),
+// when the launch record carries its scratch; other groups take family 2 instead.
 struct PairShape {
     int family, grid;
 };
 constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2},
-                                     {2, 1}, {2, 2}, {3, 1}, {3, 2}};
+                                     {2, 1}, {2, 2}, {3, 1}, {3, 2}, {4, 1}, {4, 2}};
 constexpr int kNumPairShapes = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
 
 template <class T, int P, int NT>
@@ -1203,6 +1328,51 @@ int launch_wave_group(const PairShape& sh, const PairArgs<T>& a, const PairArgs<
     return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
 }
 
+// A group that can run in the split form: one term, at least 16 entries before its p
+// axis, extents up to 64.
+template <class T>
+bool split_ok(const PairArgs<T>& a, int ntmax) {
+    const int64_t ncol = a.nslab / (a.L * a.M * a.R);
+    const int64_t tiles = std::max((a.sq + 15) / 16 * a.R, (a.sp + 15) / 16 * a.M * a.dq * a.R) * ncol;
+    return ntmax == 1 && a.L >= 16 && a.dp <= 64 && a.dq <= 64 && tiles < (int64_t(1) << 31);
+}
+
+template <class T, int KS>
+int launch_split_sized(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus,
+                       hipStream_t st, int stage) {
+    const int mult = std::max(1, sh.grid);
+    const int64_t ncol = a.nslab / (a.L * a.M * a.R);
+    const int64_t rows = stage == 1 ? a.sq : a.sp, nout = stage == 1 ? a.R : a.M * a.dq * a.R;
+    const int64_t tiles = (rows + 15) / 16 * nout * ncol;
+    auto grid_for = [&](auto kernel) -> unsigned {
+        const int64_t cap = resident_blocks(kernel, 256, cus) * mult;
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, (cap + n - 1) / n)));
+    };
+    auto go = [&](auto kb, auto k1) {
+        if (tab) hipLaunchKernelGGL(kb, dim3(grid_for(kb), unsigned(n)), dim3(256), 0, st, tab);
+        else hipLaunchKernelGGL(k1, dim3(grid_for(k1)), dim3(256), 0, st, a);
+    };
+    if (stage == 1) go(k_pair_mode_batch<T, KS, 1, 0>, k_pair_mode<T, KS, 1, 0>);
+    else if (a.D) go(k_pair_mode_batch<T, KS, 2, 2>, k_pair_mode<T, KS, 2, 2>);
+    else go(k_pair_mode_batch<T, KS, 2, 1>, k_pair_mode<T, KS, 2, 1>);
+    return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
+}
+
+// The split form: stage 1 (the q axis into Z), then stage 2 (the p axis from Z into Y).
+template <class T>
+int launch_split(const PairShape& sh, const PairArgs<T>& a, const PairArgs<T>* tab, int n, int cus, hipStream_t st) {
+    for (int stage = 1; stage <= 2; ++stage) {
+        const int d = stage == 1 ? a.dq : a.dp;
+        int rc;
+        if (d <= 16) rc = launch_split_sized<T, 4>(sh, a, tab, n, cus, st, stage);
+        else if (d <= 32) rc = launch_split_sized<T, 8>(sh, a, tab, n, cus, st, stage);
+        else if (d <= 48) rc = launch_split_sized<T, 12>(sh, a, tab, n, cus, st, stage);
+        else rc = launch_split_sized<T, 16>(sh, a, tab, n, cus, st, stage);
+        if (rc != KFC_OK) return rc;
+    }
+    return KFC_OK;
+}
+
 // the smallest of 16, 32, 40, 48 bounding d (0: none)
 inline int wave_extent(int d) { return d <= 16 ? 16 : d <= 32 ? 32 : d <= 40 ? 40 : d <= 48 ? 48 : 0; }
 
@@ -1230,8 +1400,14 @@ int launch_wave_sized(const PairShape& sh, const PairArgs<T>& a, const PairArgs<
 // One group on n problems (tab: their device records, or null with n = 1 and a passed
 // by value).  Groups of at most two terms with extents up to 48 take shape sh.
 template <class T>
-int launch_group(const PairShape& sh, const PairArgs<T>& a, int ntmax, const PairArgs<T>* tab, int n, int cus,
+int launch_group(const PairShape& sh0, const PairArgs<T>& a, int ntmax, const PairArgs<T>* tab, int n, int cus,
                  hipStream_t st) {
+    PairShape sh = sh0;
+    if (sh.family == 4) {
+        // every problem of a batch carries scratch or none (pairs_fill_table)
+        if (a.Z && split_ok(a, ntmax)) return launch_split<T>(sh, a, tab, n, cus, st);
+        sh.family = 2;
+    }
     const int Pm = std::max(a.Mp, a.Nq);
     // the wave-per-slab kernels address a slab with byte offsets below 2^31, padded
     // tiles included
@@ -1272,9 +1448,10 @@ struct PairTuneKey {
 std::mutex g_pair_tune_mu;
 std::map<PairTuneKey, std::vector<int>> g_pair_tune;
 
-PairTuneKey pair_tune_key(const kfc_op* op, size_t scalar, int64_t slabs_total) {
+PairTuneKey pair_tune_key(const kfc_op* op, size_t scalar, int64_t slabs_total, bool scratch) {
     PairTuneKey k;
     k.sig = op->dims;
+    k.sig.push_back(scratch ? 1 : 0);   // the split form is a candidate only with scratch
     for (const PairGroup& g : op->pairs.groups) {
         k.sig.push_back(g.p);
         k.sig.push_back(g.q);
@@ -1354,19 +1531,35 @@ int run_pair_groups(kfc_ctx* ctx, const PairTuneKey& key, int ng, hipStream_t st
     return apply(pick);   // Y from the chosen shapes
 }
 
+// Problem q's scratch for the split form is the q-th N x ncols piece of the context's
+// buffer, when the problems share one context and the buffer already holds all n;
+// this fill allocates nothing (a graph replays its launches after each refill).
 template <class T>
 void pairs_fill_table(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
                       void* host_tab) {
     const size_t ng = ops[0]->pairs.groups.size();
     PairArgs<T>* h = static_cast<PairArgs<T>*>(host_tab);
+    const size_t per = size_t(ops[0]->N) * size_t(ncols);
+    const DevBuf& zb = ops[0]->ctx->pairz;
+    bool zok = op_wants_pairz(ops[0]) && zb.p && zb.bytes >= per * size_t(n) * sizeof(T);
+    for (int q = 1; q < n && zok; ++q) zok = ops[q]->ctx == ops[0]->ctx;
     for (size_t gi = 0; gi < ng; ++gi)
-        for (int q = 0; q < n; ++q) h[gi * n + q] = pair_args<T>(ops[q], gi, X[q], Y[q], ncols, ld);
+        for (int q = 0; q < n; ++q)
+            h[gi * n + q] = pair_args<T>(ops[q], gi, X[q], Y[q], ncols, ld, zok ? zb.as<T>() + per * q : nullptr);
+}
+
+int pairs_reserve(kfc_op* op, int n, int64_t ncols, int prec) {
+    const size_t bytes = size_t(op->N) * size_t(ncols) * size_t(n) * (prec == 0 ? sizeof(double) : sizeof(float));
+    if (op_wants_pairz(op)) KFC_HIP(op->ctx, op->ctx->pairz.ensure(bytes));
+    return KFC_OK;
 }
 
 template <class T>
 int launch_pairs_batch(kfc_op* const* ops, int n, const void* const* X, void* const* Y, int64_t ncols, int64_t ld,
                        hipStream_t st, void* host_tab, void* dev_tab) {
     if (n == 0) return KFC_OK;
+    // the scratch of the split form, if the context can hold it (else every group runs whole)
+    (void)pairz_for(ops[0], size_t(ops[0]->N) * size_t(ncols) * size_t(n) * sizeof(T));
     pairs_fill_table<T>(ops, n, X, Y, ncols, ld, host_tab);
     return launch_pairs_filled<T>(ops, n, st, host_tab, dev_tab);
 }
@@ -1387,7 +1580,9 @@ int launch_pairs_filled(kfc_op* const* ops, int n, hipStream_t st, void* host_ta
     }
     const int cus = op0->ctx->num_cus;
     auto one = [&](const PairShape& sh, int gi) { return launch_group<T>(sh, h[gi * n], ntmax[gi], d + gi * n, n, cus, st); };
-    return run_pair_groups(op0->ctx, pair_tune_key(op0, sizeof(T), slabs), int(ng), st, one);
+    bool scratch = false;
+    for (size_t gi = 0; gi < ng; ++gi) scratch |= h[gi * n].Z != nullptr;
+    return run_pair_groups(op0->ctx, pair_tune_key(op0, sizeof(T), slabs, scratch), int(ng), st, one);
 }
 
 template void pairs_fill_table<double>(kfc_op* const*, int, const void* const*, void* const*, int64_t, int64_t, void*);
@@ -1402,19 +1597,20 @@ template int launch_pairs_batch<float>(kfc_op* const*, int, const void* const*, 
 size_t pair_args_bytes(int prec) { return prec == 0 ? sizeof(PairArgs<double>) : sizeof(PairArgs<float>); }
 
 template <class T>
-int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st) {
+int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st, bool scratch) {
     if (op->pairs.groups.empty()) return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "dense pair path without groups");
     const size_t ng = op->pairs.groups.size();
+    void* z = scratch ? pairz_for(op, size_t(op->N) * size_t(ncols) * sizeof(T)) : nullptr;
     std::vector<PairArgs<T>> args(ng);
     int64_t slabs = 0;
     for (size_t gi = 0; gi < ng; ++gi) {
-        args[gi] = pair_args<T>(op, gi, X, Y, ncols, ld);
+        args[gi] = pair_args<T>(op, gi, X, Y, ncols, ld, z);
         slabs += args[gi].nslab;
     }
     const int cus = op->ctx->num_cus;
     auto one = [&](const PairShape& sh, int gi) { return launch_group<T>(sh, args[gi], args[gi].nterm, nullptr, 1, cus, st); };
-    return run_pair_groups(op->ctx, pair_tune_key(op, sizeof(T), slabs), int(ng), st, one);
+    return run_pair_groups(op->ctx, pair_tune_key(op, sizeof(T), slabs, z != nullptr), int(ng), st, one);
 }
 
-template int launch_pairs<double>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t);
-template int launch_pairs<float>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t);
+template int launch_pairs<double>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t, bool);
+template int launch_pairs<float>(kfc_op*, const void*, void*, int64_t, int64_t, hipStream_t, bool);

@@ -76,27 +76,6 @@ using cd = std::complex<double>;
 // ------------------------------------------------------------------------------------------
 // Context
 
-struct kfc_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    int num_cus = 256;
-    kfc_callback cb = nullptr;      // per-iteration solver callback (kfc_ctx_set_callback)
-    void* cb_user = nullptr;
-};
-
-int kfc_fail(kfc_ctx* ctx, int code, const std::string& msg);
-// Hands info to the context's solver callback, if any: KFC_EABORT (with the message set)
-// when the callback asks to stop, else KFC_OK.
-int ctx_callback(kfc_ctx* ctx, const kfc_iter_info& info, const char* who);
-
-#define KFC_HIP(ctx, call)                                                                     \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return kfc_fail((ctx), KFC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
 // A device allocation owned by the library.
 struct DevBuf {
     void* p = nullptr;
@@ -120,6 +99,29 @@ struct DevBuf {
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
+
+struct kfc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 256;
+    kfc_callback cb = nullptr;      // per-iteration solver callback (kfc_ctx_set_callback)
+    void* cb_user = nullptr;
+    DevBuf pairz;                   // the split dense-pair products' intermediate (grow-only)
+};
+
+int kfc_fail(kfc_ctx* ctx, int code, const std::string& msg);
+// Hands info to the context's solver callback, if any: KFC_EABORT (with the message set)
+// when the callback asks to stop, else KFC_OK.
+int ctx_callback(kfc_ctx* ctx, const kfc_iter_info& info, const char* who);
+
+#define KFC_HIP(ctx, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return kfc_fail((ctx), KFC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
 
 // ------------------------------------------------------------------------------------------
 // Operator (host copy of the descriptor plus device data per precision)
@@ -230,7 +232,14 @@ int op_spectral_interval(kfc_op* op, double* lo, double* hi);
 int op_build_plan(kfc_op* op);
 int op_build_pairs(kfc_op* op);
 template <class T> int pairs_upload(kfc_op* op, DeviceData& dd);
-template <class T> int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st);
+// scratch: the split form of a group may use the context's scratch (grown here); false
+// for callers that must not allocate (every group then runs whole).
+template <class T>
+int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st, bool scratch = true);
+// Grows the context's scratch for n problems of ncols columns at precision prec (0: FP64)
+// when a group of op can run in the split form; a graph that captures pair launches
+// reserves it first, since filling a table allocates nothing.
+int pairs_reserve(kfc_op* op, int n, int64_t ncols, int prec);
 // Many operators of one pair structure in one launch per group (dense_pair.hip):
 // problem q maps X[q] -> Y[q]; host_tab (pinned) and dev_tab hold n x groups
 // launch records of pair_args_bytes(prec) bytes each.

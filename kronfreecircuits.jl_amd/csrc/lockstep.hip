@@ -1318,6 +1318,12 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
         }
         pair_batch &= !ops[i]->plan.ok && ops[i]->pairs.ok && pairs_same_structure(op0, ops[i]);
     }
+    // the split pair products' scratch for every point's W columns, before any step is
+    // captured (a table fill allocates nothing)
+    if (pair_batch) {
+        rc = pairs_reserve(op0, n, W, 0);
+        if (rc != KFC_OK) return rc;
+    }
     const int64_t rows_per = std::max<int64_t>(LS_SR, ((N + 15) / 16 + LS_SR - 1) / LS_SR * LS_SR);
     const int nchunk = int((N + rows_per - 1) / rows_per);
     DevBuf dpart, dred, dslot, dtheta, dpth, dM, dtab;

@@ -7,7 +7,8 @@ pinned here with KFC_PAIR_SHAPE and checked on its own: the single-operator appl
 against the oracle's per-term chain (FP64 and FP32), and the batched apply of the
 lockstep sweep against the one-point solver.  The operators cover unit and
 strided p axes (the strided ones also through the LDS-staged kernel), extents that are not multiples of 4 or 16, one and two terms per
-group (shared factors merged on the host), and every tile size up to 48."""
+group (shared factors merged on the host), every tile size up to 48, and the split form
+of one-term groups with a strided p axis (two mode products through scratch)."""
 import numpy as np
 import pytest
 
@@ -17,7 +18,7 @@ from tests.helpers import to_kfc, to_oracle
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-N_SHAPES = 11          # kPairShapes in dense_pair.hip
+N_SHAPES = 13          # kPairShapes in dense_pair.hip
 TOL64 = 1e-13
 TOL32 = 4e-5
 
@@ -55,7 +56,10 @@ CASES = [([40, 40, 30], [(0, 1), (1, 2)]),      # the zero-pi parent's structure
          ([3, 37, 29], [(1, 2)]),               # strided p axis, ragged extents
          ([45, 6, 33], [(0, 2), (1, 2)]),        # 48-wide tiles
          ([13, 10, 7, 2], [(0, 1), (2, 3)]),     # 16-wide tiles, four axes
-         ([26, 31], [(0, 1)])]                  # 32-wide tiles, one slab per column
+         ([26, 31], [(0, 1)]),                  # 32-wide tiles, one slab per column
+         ([18, 11, 26], [(1, 2)]),              # split form (shapes 11, 12) as the group writing D .* X
+         ([16, 5, 3, 7], [(1, 3)]),             # split form with an axis between p and q
+         ([20, 50, 9], [(0, 1), (1, 2)])]       # split form, p extent above 48 (64-wide tiles)
 
 
 @pytest.mark.parametrize("shape", range(N_SHAPES))
@@ -73,7 +77,7 @@ def test_pair_shape_apply(monkeypatch, shape, case):
         assert _rel(_run(H, X, "f32"), Yo) < TOL32, (shape, dims, ncols)
 
 
-@pytest.mark.parametrize("shape", [0, 7, 8, 9, 10])
+@pytest.mark.parametrize("shape", [0, 7, 8, 9, 10, 11, 12])
 def test_pair_shape_lockstep(monkeypatch, shape):
     """The batched (one launch for every point) form of the shape in a lockstep sweep:
     each point's pairs match the one-point solver and meet the tolerance."""
