@@ -931,7 +931,11 @@ void k_dense_pair_staged_batch(const PairArgs<T>* __restrict__ tab) {
 // written; the slab kernels read X and Y and write Y once, but through 8-byte pieces.
 // MODE 0: out = G in (stage 1); 1: out += G in; 2: out = D .* X + G in (the group that
 // writes Y first).
-template <class T, int KS, int STAGE, int MODE>
+// FLAT (family 5): a tile is 16 consecutive rows of the (blocks x rows) sequence of one
+// block column, crossing block boundaries (no lanes idle when rows is not a multiple of
+// 16), and G's fragments are read from LDS at each step instead of being held in
+// registers (about half the registers, twice the waves a SIMD).
+template <class T, int KS, int STAGE, int MODE, bool FLAT = false>
 __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep) {
     using v4 = typename V4<T>::type;
     constexpr int NTJ = KS / 4;   // 16-wide output tiles; KS contraction steps of 4 bound d
@@ -945,6 +949,53 @@ __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep
     const T* __restrict__ G = STAGE == 1 ? a.B[0] : a.A[0];
     const T gc = STAGE == 1 ? a.c[0] : T(1);
     const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    if constexpr (FLAT) {
+        constexpr int LDG = 16 * NTJ;   // G staged column-major, zero-padded to LDG x 4 KS
+        __shared__ T Gs[LDG * 4 * KS];
+        for (int e = threadIdx.x; e < LDG * 4 * KS; e += blockDim.x) {
+            const int j = e % LDG, jp = e / LDG;
+            Gs[e] = (j < d && jp < d) ? gc * G[j + (int64_t)jp * d] : T(0);
+        }
+        __syncthreads();
+        const uint32_t nrow = rows * nout, ntile = (nrow + 15) / 16, total = ntile * ncol;
+        const uint32_t wave = uint32_t(w0) + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+        for (uint32_t t = wave; t < total; t += uint32_t(wstep)) {
+            const uint32_t tr = t % ntile, col = t / ntile;
+            const uint32_t gr = 16 * tr + r16;   // this lane's row of the block sequence
+            const bool rok = gr < nrow;
+            const uint32_t o = gr / rows, row = gr - o * rows;
+            const int64_t cb = int64_t(o) * rows * d + row, base = int64_t(col) * a.ld + cb;
+            T b[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int jp = 4 * ks + k4;
+                b[ks] = (rok && jp < d) ? in[base + (int64_t)jp * rows] : T(0);
+            }
+            v4 acc[NTJ];
+#pragma unroll
+            for (int tj = 0; tj < NTJ; ++tj) acc[tj] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+                if (4 * ks < d)
+#pragma unroll
+                    for (int tj = 0; tj < NTJ; ++tj)
+                        acc[tj] = mfma(Gs[(16 * tj + r16) + (4 * ks + k4) * LDG], b[ks], acc[tj]);
+#pragma unroll
+            for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int j = 16 * tj + out_row(T(0), k4, v);
+                    if (rok && j < d) {
+                        const int64_t e = (int64_t)j * rows;
+                        T r = acc[tj][v];
+                        if constexpr (MODE == 1) r += out[base + e];
+                        if constexpr (MODE == 2) r += a.D[cb + e] * a.X[base + e];
+                        out[base + e] = r;
+                    }
+                }
+        }
+        return;
+    }
     // A operand of step ks for output tile tj: (c) G[16 tj + r16, 4 ks + k4]
     T g[NTJ][KS];
 #pragma unroll
@@ -992,15 +1043,15 @@ __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep
     }
 }
 
-template <class T, int KS, int STAGE, int MODE>
+template <class T, int KS, int STAGE, int MODE, bool FLAT>
 __global__ __launch_bounds__(256) void k_pair_mode(const PairArgs<T> a) {
-    pair_mode<T, KS, STAGE, MODE>(a, int64_t(blockIdx.x) * 4, int64_t(gridDim.x) * 4);
+    pair_mode<T, KS, STAGE, MODE, FLAT>(a, int64_t(blockIdx.x) * 4, int64_t(gridDim.x) * 4);
 }
 
-template <class T, int KS, int STAGE, int MODE>
+template <class T, int KS, int STAGE, int MODE, bool FLAT>
 __global__ __launch_bounds__(256) void k_pair_mode_batch(const PairArgs<T>* __restrict__ tab) {
     const PairArgs<T> a = tab[blockIdx.y];
-    pair_mode<T, KS, STAGE, MODE>(a, int64_t(blockIdx.x) * 4, int64_t(gridDim.x) * 4);
+    pair_mode<T, KS, STAGE, MODE, FLAT>(a, int64_t(blockIdx.x) * 4, int64_t(gridDim.x) * 4);
 }
 
 // Blocks of kernel K that fit on the device at once (occupancy x compute units): a
@@ -1248,11 +1299,12 @@ bool pairs_same_structure(const kfc_op* a, const kfc_op* b) {
 // the winner.
 // Family 4 runs a one-term group with a strided p axis in the split form (pair_mode),
 // when the launch record carries its scratch; other groups take family 2 instead.
+// Family 5 is family 4 with rows tiled across blocks and G in LDS (pair_mode FLAT).
 struct PairShape {
     int family, grid;
 };
 constexpr PairShape kPairShapes[] = {{0, 0}, {0, 1}, {0, 2}, {0, 4}, {1, 0}, {1, 1}, {1, 2},
-                                     {2, 1}, {2, 2}, {3, 1}, {3, 2}, {4, 1}, {4, 2}};
+                                     {2, 1}, {2, 2}, {3, 1}, {3, 2}, {4, 1}, {4, 2}, {5, 1}, {5, 2}};
 constexpr int kNumPairShapes = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
 
 template <class T, int P, int NT>
@@ -1343,7 +1395,8 @@ int launch_split_sized(const PairShape& sh, const PairArgs<T>& a, const PairArgs
     const int mult = std::max(1, sh.grid);
     const int64_t ncol = a.nslab / (a.L * a.M * a.R);
     const int64_t rows = stage == 1 ? a.sq : a.sp, nout = stage == 1 ? a.R : a.M * a.dq * a.R;
-    const int64_t tiles = (rows + 15) / 16 * nout * ncol;
+    const bool flat = sh.family == 5;
+    const int64_t tiles = (flat ? (rows * nout + 15) / 16 : (rows + 15) / 16 * nout) * ncol;
     auto grid_for = [&](auto kernel) -> unsigned {
         const int64_t cap = resident_blocks(kernel, 256, cus) * mult;
         return unsigned(std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, (cap + n - 1) / n)));
@@ -1352,9 +1405,15 @@ int launch_split_sized(const PairShape& sh, const PairArgs<T>& a, const PairArgs
         if (tab) hipLaunchKernelGGL(kb, dim3(grid_for(kb), unsigned(n)), dim3(256), 0, st, tab);
         else hipLaunchKernelGGL(k1, dim3(grid_for(k1)), dim3(256), 0, st, a);
     };
-    if (stage == 1) go(k_pair_mode_batch<T, KS, 1, 0>, k_pair_mode<T, KS, 1, 0>);
-    else if (a.D) go(k_pair_mode_batch<T, KS, 2, 2>, k_pair_mode<T, KS, 2, 2>);
-    else go(k_pair_mode_batch<T, KS, 2, 1>, k_pair_mode<T, KS, 2, 1>);
+    if (flat) {
+        if (stage == 1) go(k_pair_mode_batch<T, KS, 1, 0, true>, k_pair_mode<T, KS, 1, 0, true>);
+        else if (a.D) go(k_pair_mode_batch<T, KS, 2, 2, true>, k_pair_mode<T, KS, 2, 2, true>);
+        else go(k_pair_mode_batch<T, KS, 2, 1, true>, k_pair_mode<T, KS, 2, 1, true>);
+    } else {
+        if (stage == 1) go(k_pair_mode_batch<T, KS, 1, 0, false>, k_pair_mode<T, KS, 1, 0, false>);
+        else if (a.D) go(k_pair_mode_batch<T, KS, 2, 2, false>, k_pair_mode<T, KS, 2, 2, false>);
+        else go(k_pair_mode_batch<T, KS, 2, 1, false>, k_pair_mode<T, KS, 2, 1, false>);
+    }
     return hipGetLastError() == hipSuccess ? KFC_OK : KFC_EDEVICE;
 }
 
@@ -1403,7 +1462,7 @@ template <class T>
 int launch_group(const PairShape& sh0, const PairArgs<T>& a, int ntmax, const PairArgs<T>* tab, int n, int cus,
                  hipStream_t st) {
     PairShape sh = sh0;
-    if (sh.family == 4) {
+    if (sh.family == 4 || sh.family == 5) {
         // every problem of a batch carries scratch or none (pairs_fill_table)
         if (a.Z && split_ok(a, ntmax)) return launch_split<T>(sh, a, tab, n, cus, st);
         sh.family = 2;

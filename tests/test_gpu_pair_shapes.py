@@ -18,7 +18,7 @@ from tests.helpers import to_kfc, to_oracle
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-N_SHAPES = 13          # kPairShapes in dense_pair.hip
+N_SHAPES = 15          # kPairShapes in dense_pair.hip
 TOL64 = 1e-13
 TOL32 = 4e-5
 
@@ -57,7 +57,7 @@ CASES = [([40, 40, 30], [(0, 1), (1, 2)]),      # the zero-pi parent's structure
          ([45, 6, 33], [(0, 2), (1, 2)]),        # 48-wide tiles
          ([13, 10, 7, 2], [(0, 1), (2, 3)]),     # 16-wide tiles, four axes
          ([26, 31], [(0, 1)]),                  # 32-wide tiles, one slab per column
-         ([18, 11, 26], [(1, 2)]),              # split form (shapes 11, 12) as the group writing D .* X
+         ([18, 11, 26], [(1, 2)]),              # split form (shapes 11-14) as the group writing D .* X
          ([16, 5, 3, 7], [(1, 3)]),             # split form with an axis between p and q
          ([20, 50, 9], [(0, 1), (1, 2)])]       # split form, p extent above 48 (64-wide tiles)
 
@@ -77,7 +77,7 @@ def test_pair_shape_apply(monkeypatch, shape, case):
         assert _rel(_run(H, X, "f32"), Yo) < TOL32, (shape, dims, ncols)
 
 
-@pytest.mark.parametrize("shape", [0, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("shape", [0, 7, 8, 9, 10, 11, 12, 13, 14])
 def test_pair_shape_lockstep(monkeypatch, shape):
     """The batched (one launch for every point) form of the shape in a lockstep sweep:
     each point's pairs match the one-point solver and meet the tolerance."""
