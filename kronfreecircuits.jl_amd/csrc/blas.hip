@@ -286,13 +286,13 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
     // the next slab is fetched into registers while the current one is contracted
     // (NPF >= ppad / 4 columns per wave)
     double pf[NPF];
-    auto fetch = [&](int64_t rb) {
+    auto fetch = [&](double* dst, int64_t rb) {
         const int64_t row = rb + lane;
         const bool rok = row < r1;
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int c = wv + 4 * j;
-            pf[j] = (rok && c < p) ? double(X[row + c * ldx]) : 0.0;
+            dst[j] = (rok && c < p) ? double(X[row + c * ldx]) : 0.0;
         }
     };
     // M's operands for this lane held in registers (k step i: row 4 i + hi), the
@@ -313,16 +313,16 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
             for (int i = 0; i < NPF; ++i) m2r[i] = 4 * i < ppad ? sM2[(4 * i + hi) * 16 + lo] : 0.0;
         }
     }
-    if (r0 < r1) fetch(r0);
-    for (int64_t rb = r0; rb < r1; rb += 64) {
+    // one 64-row slab from the registers cur (refilled with the next slab)
+    auto slab = [&](int64_t rb, double* cur) {
         __syncthreads();   // the previous slab's readers are done (and sM is in place)
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int c = wv + 4 * j;
-            if (c < ppad) sX[c * TS_LDR + lane] = pf[j];
+            if (c < ppad) sX[c * TS_LDR + lane] = cur[j];
         }
         __syncthreads();
-        if (rb + 64 < r1) fetch(rb + 64);
+        if (rb + 64 < r1) fetch(cur, rb + 64);
         gd4 acc[TQ];
 #pragma unroll
         for (int j = 0; j < TQ; ++j) acc[j] = gd4{0, 0, 0, 0};
@@ -390,7 +390,9 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
                 }
             }
         }
-    }
+    };
+    if (r0 < r1) fetch(pf, r0);
+    for (int64_t rb = r0; rb < r1; rb += 64) slab(rb, pf);
     if (GR == 0) return;
     // sum the waves' Gram tiles in LDS
     double* red = sm;
@@ -408,7 +410,8 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
         }
         __syncthreads();
     }
-    double* out = part + (int64_t)blockIdx.x * gp * q;
+    // a block's partial Gram is gp x qy (qy = q2 for the chained product)
+    double* out = part + (int64_t)blockIdx.x * gp * qy;
     for (int e = tid; e < TPZ * 256; e += 256) {
         const int z = e >> 8, rc = e & 255;
         const int gi = 16 * z + (rc >> 4), gj = rc & 15;
@@ -416,31 +419,36 @@ __global__ __launch_bounds__(256) void k_tsmm(const S* __restrict__ X, int64_t l
     }
 }
 
+// resident (non-null): report the kernel's resident blocks per compute unit instead of
+// launching it.
 template <class S, int TQ, int GR, int TPZ>
 void launch_tsmm(int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p, int ppad, const double* M,
                  int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz, double* part, const double* M2,
-                 int q2) {
+                 int q2, int* resident) {
+    auto go = [&](auto kernel) {
+        if (resident) {
+            int b = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, 256, shmem) != hipSuccess || b < 1) b = 1;
+            *resident = b;
+            return;
+        }
+        hipLaunchKernelGGL(kernel, dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N, rows_per, pz,
+                           part, M2, q2);
+    };
     // prefetch registers sized to the block width: fewer registers, more waves
-    if (ppad <= 16)
-        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 4>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part, M2, q2);
-    else if (ppad <= 32)
-        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 8>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part, M2, q2);
-    else if (ppad <= 48)
-        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 12>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part, M2, q2);
-    else
-        hipLaunchKernelGGL((k_tsmm<S, TQ, GR, TPZ, 20>), dim3(nchunk), dim3(256), shmem, st, X, ldx, p, ppad, M, q, Y,
-                           ldy, N, rows_per, pz, part, M2, q2);
+    if (ppad <= 16) go(k_tsmm<S, TQ, GR, TPZ, 4>);
+    else if (ppad <= 32) go(k_tsmm<S, TQ, GR, TPZ, 8>);
+    else if (ppad <= 48) go(k_tsmm<S, TQ, GR, TPZ, 12>);
+    else go(k_tsmm<S, TQ, GR, TPZ, 20>);
 }
 
 template <class S>
 void tsmm_dispatch(int gr, int tq, int tpz, int nchunk, size_t shmem, hipStream_t st, const S* X, int64_t ldx, int p,
                    int ppad, const double* M, int q, S* Y, int64_t ldy, int64_t N, int64_t rows_per, int pz,
-                   double* part, const double* M2 = nullptr, int q2 = 0) {
+                   double* part, const double* M2 = nullptr, int q2 = 0, int* resident = nullptr) {
 #define KFC_TS(TQv, GRv, TPZv) \
-    launch_tsmm<S, TQv, GRv, TPZv>(nchunk, shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N, rows_per, pz, part, M2, q2)
+    launch_tsmm<S, TQv, GRv, TPZv>(nchunk, shmem, st, X, ldx, p, ppad, M, q, Y, ldy, N, rows_per, pz, part, M2, q2, \
+                                   resident)
     if (gr == 0) {
         switch (tq) {
             case 1: KFC_TS(1, 0, 1); break;
@@ -475,6 +483,21 @@ void tsmm_dispatch(int gr, int tq, int tpz, int nchunk, size_t shmem, hipStream_
         }
     }
 #undef KFC_TS
+}
+
+// Row chunks of a k_tsmm launch: one block per resident slot (the kernel's occupancy x
+// the compute units), each taking whole 64-row slabs, so that every block runs from the
+// start and none waits for a second round (2048 blocks at 3 a unit left a third of the
+// device idle for the last third of a pass).
+template <class S>
+int tsmm_chunks(const Work& w, int gr, int tq, int tpz, int ppad, size_t shmem, int64_t N, int64_t& rows_per) {
+    int b = 1;
+    tsmm_dispatch<S>(gr, tq, tpz, 0, shmem, nullptr, nullptr, 0, 0, ppad, nullptr, 0, nullptr, 0, 0, 0, 0, nullptr,
+                     nullptr, 0, &b);
+    const int64_t slabs = std::max<int64_t>(1, (N + 63) / 64);
+    const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(int64_t(b) * w.ctx->num_cus, slabs));
+    rows_per = (slabs + nch - 1) / nch * 64;
+    return int((N + rows_per - 1) / rows_per);
 }
 
 template <class S>
@@ -600,8 +623,10 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
     }
     if constexpr (!IsCx<S>::value) {
         const int tq = (q + 15) / 16, ppad = (p + 3) / 4 * 4, tpz = (gz + 15) / 16;
-        const int nchunk = nchunks_for(N, 4096, 2048);
-        const int64_t rows_per = (N + nchunk - 1) / nchunk;
+        size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 * tq + (gram ? 16 * TS_LDR : 0);
+        words = std::max<size_t>(words, size_t(gz > 0 ? tpz : 1) * 256);
+        int64_t rows_per = 0;
+        const int nchunk = tsmm_chunks<S>(w, gram, tq, tpz, ppad, words * sizeof(double), N, rows_per);
         std::vector<double> h(size_t(p) * q);
         for (size_t i = 0; i < h.size(); ++i) h[i] = M[i].real();
         KFC_HIP(w.ctx, w.small2.ensure(sizeof(double) * h.size()));
@@ -611,8 +636,6 @@ int blas_tsmm(Work& w, const S* X, int64_t ldx, int p, const std::vector<cd>& M,
             KFC_HIP(w.ctx, w.part.ensure(sizeof(double) * size_t(nchunk) * gp * q));
             KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(gp) * q));
         }
-        size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 * tq + (gram ? 16 * TS_LDR : 0);
-        words = std::max<size_t>(words, size_t(gz > 0 ? tpz : 1) * 256);
         KFC_HIP(w.ctx, w.tsmm_begin());
         tsmm_dispatch<S>(gram, tq, tpz, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
                          w.small2.as<double>(), q, Y, ldy, N, rows_per, pz, gram ? w.part.as<double>() : nullptr);
@@ -651,8 +674,10 @@ int blas_tsmm_chain(Work& w, const S* X, int64_t ldx, int p, const std::vector<c
     }
     if constexpr (!IsCx<S>::value) {
         const int ppad = (p + 3) / 4 * 4;
-        const int nchunk = nchunks_for(N, 4096, 2048);
-        const int64_t rows_per = (N + nchunk - 1) / nchunk;
+        size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 + 16 * TS_LDR + size_t(ppad) * 16;
+        words = std::max<size_t>(words, 256);
+        int64_t rows_per = 0;
+        const int nchunk = tsmm_chunks<S>(w, 6, 1, 1, ppad, words * sizeof(double), N, rows_per);
         std::vector<double> h(size_t(p) * (q1 + q2));
         for (size_t i = 0; i < size_t(p) * q1; ++i) h[i] = M1[i].real();
         for (size_t i = 0; i < size_t(p) * q2; ++i) h[size_t(p) * q1 + i] = M2[i].real();
@@ -661,8 +686,6 @@ int blas_tsmm_chain(Work& w, const S* X, int64_t ldx, int p, const std::vector<c
                                       w.ctx->stream));
         KFC_HIP(w.ctx, w.part.ensure(sizeof(double) * size_t(nchunk) * q2 * q2));
         KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(q2) * q2));
-        size_t words = size_t(ppad) * TS_LDR + size_t(ppad) * 16 + 16 * TS_LDR + size_t(ppad) * 16;
-        words = std::max<size_t>(words, 256);
         KFC_HIP(w.ctx, w.tsmm_begin());
         tsmm_dispatch<S>(6, 1, 1, nchunk, words * sizeof(double), w.ctx->stream, X, ldx, p, ppad,
                          w.small2.as<double>(), q1, Y, ldy, N, rows_per, pz, w.part.as<double>(),
