@@ -114,10 +114,13 @@ class MaterializedNode:
     solver: object = None
     stats: dict = field(default_factory=dict)
     seconds: float = 0.0
+    ports: dict = field(default_factory=dict)   # port matrices of basis children, by key
 
 
 def _axis_apply(op, V, dims, a):
     """op applied along axis a of the column-major tensors V[:, j] (host)."""
+    if len(dims) == 1:
+        return dense(op) @ V
     T = V.reshape(list(dims) + [V.shape[1]], order="F")
     T = np.moveaxis(np.tensordot(dense(op), np.moveaxis(T, a, 0), axes=(1, 0)), 0, a)
     return T.reshape(V.shape, order="F")
@@ -154,20 +157,47 @@ def sandwich(V, dims, ops, ctx=None):
     return G.reshape((k, k), order="F")
 
 
+def _basis_key(b):
+    """A basis's type and scalar parameters (None when it holds anything else)."""
+    items = []
+    for k, v in sorted(vars(b).items()):
+        if not isinstance(v, (bool, int, float, str)):
+            return None
+        items.append((k, v))
+    return (type(b).__name__,) + tuple(items)
+
+
 def port_matrix(mn, mode, port, device=None):
     """``_port_matrix`` (src/materialize.jl:443-459): V' O V with O the mode's port
     on its axis (a basis child's localop or a node child's port).  ``device``:
     form it with ``sandwich`` (default: when V has at least DEVICE_PORT_ENTRIES
-    entries)."""
+    entries).  Ports on basis children are kept on the materialized node (its
+    vectors never change), keyed by mode, port and the basis's parameters, so a node
+    shared by many parents (a sweep's fixed leaf) forms each port once."""
     a = mn.node.child_of(mode)
     ch = mn.node.children[a]
+    key = None
+    if isinstance(ch, tuple):
+        bk = _basis_key(ch[1])
+        if bk is not None:
+            key = (mode, port, bk, device)
+            try:
+                hash(key)
+            except TypeError:
+                key = None
+    if key is not None and key in mn.ports:
+        return mn.ports[key].copy()
     op = localop(ch[1], port) if isinstance(ch, tuple) else DenseOp(port_matrix(mn.children[a], mode, port))
     V = mn.vectors
     if device is None:
         device = V.size >= DEVICE_PORT_ENTRIES
     if device:
-        return sandwich(V, mn.dims, [(a, op)])
-    return V.conj().T @ _axis_apply(op, V, mn.dims, a)
+        P = sandwich(V, mn.dims, [(a, op)])
+    else:
+        P = V.conj().T @ _axis_apply(op, V, mn.dims, a)
+    if key is not None:
+        mn.ports[key] = P.copy()
+    return P
 
 
 def joint_port(mn, factors, device=None):
