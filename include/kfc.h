@@ -199,7 +199,9 @@ int kfc_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int
  * allocating routine.  The chain path's two intermediate blocks come from a static
  * device array of 64 MiB that every GPU the library runs on holds until the process
  * ends: 2 x prod(dims) x ncols x element size (plus 256 bytes) must fit, else KFC_EARG;
- * the fused-stencil and dense-pair paths need no scratch.  The operator's tables for the
+ * the fused-stencil path needs no scratch, and the dense-pair path runs every group as
+ * one kernel (the split form kfc_apply may pick, which takes context scratch, is not
+ * used here).  The operator's tables for the
  * precision must be on the device (kfc_op_create places the FP64 ones, the first FP32
  * kfc_apply the FP32 ones), the stencil keeps the launch shape the tuner last chose, and
  * complex blocks on a real operator (KFC_PREC_Z64) are refused (KFC_EUNSUPPORTED).  All
