@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import kfc_oracle as O
-from tests.helpers import to_kfc, to_oracle
+from tests.helpers import oracle_from_kfc_terms, to_kfc, to_oracle
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -95,3 +95,37 @@ def test_pair_shape_lockstep(monkeypatch, shape):
     for th, (El, Vl, _, _) in zip(pts, lock):
         H, _ = sw.parent(th)
         assert np.max(kfc.residuals(H, El, Vl)) <= 1e-8
+        assert np.max(O.true_residuals(oracle_from_kfc_terms(H), El, Vl)) <= 1e-8
+
+
+@pytest.mark.parametrize("shape", [11, 12, 13, 14])
+def test_split_form_lockstep(monkeypatch, shape):
+    """The split form in the batched launches of a lockstep sweep (one launch per
+    stage for every active point, scratch pieces per point from the context): a
+    zero-pi parent [16, 16, 10], whose (phi, zeta) group has 16 entries before its p
+    axis, so the pinned shape takes the split form there.  Each point's pairs match
+    the one-point solver's and have fresh residuals within the tolerance under the
+    oracle's chain apply; the step graphs replay bit for bit what the separate
+    launches compute."""
+    from kfc import configs
+    from kfc.sweeps import ZeroPiSweep
+    import kfc
+    monkeypatch.setenv("KFC_PAIR_SHAPE", str(shape))
+    sw = ZeroPiSweep(ncut=12, Nphi=48, Nzeta=10, keep=16, m=6, contexts=1)
+    pts = [configs.zero_pi_theta(9400 + i) for i in range(5)]
+    H0, _ = sw.parent(pts[0])
+    assert list(H0.dims) == [16, 16, 10]
+    seq, _ = sw.run_plain(pts[:2])
+    monkeypatch.setenv("KFC_GRAPHS", "0")
+    eager, _ = sw.run_batched(pts, mode="f64", lockstep=True)
+    monkeypatch.setenv("KFC_GRAPHS", "1")
+    lock, _ = sw.run_batched(pts, mode="f64", lockstep=True)
+    for (Ee, Ve, _, _), (El, Vl, _, _) in zip(eager, lock):
+        assert np.array_equal(Ee, El) and np.array_equal(Ve, Vl)
+    for (Es, _, _, _), (El, _, sl, _) in zip(seq, lock):
+        assert sl["max_residual"] <= 1e-8
+        assert np.max(np.abs(Es - El) / np.abs(Es)) < 1e-9
+    for th, (El, Vl, _, _) in zip(pts, lock):
+        H, _ = sw.parent(th)
+        assert np.max(kfc.residuals(H, El, Vl)) <= 1e-8
+        assert np.max(O.true_residuals(oracle_from_kfc_terms(H), El, Vl)) <= 1e-8
