@@ -1620,7 +1620,7 @@ int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int 
 }
 
 int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
-                      hipStream_t st) {
+                      hipStream_t st, const void* A, cd gamma) {
     if (op->cplx || !op->plan.ok)
         return kfc_fail(op->ctx, KFC_EUNSUPPORTED, "promoted apply: a real operator on the fused stencil path only");
     if (ncols == 0 || op->N == 0) return KFC_OK;
@@ -1634,7 +1634,7 @@ int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t
         op->stencil_variant = op->variant_z;
         op->stencil_march = op->march_z;
     }
-    const int rc = apply_t<Cx<double>>(op, X, nullptr, Y, ncols, ld, alpha, beta, cd(0), st);
+    const int rc = apply_t<Cx<double>>(op, X, A, Y, ncols, ld, alpha, beta, gamma, st);
     if (!op->stencil_pinned) {
         op->variant_z = op->stencil_variant;
         op->march_z = op->stencil_march;

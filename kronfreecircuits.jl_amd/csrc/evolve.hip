@@ -116,6 +116,13 @@ int affine_apply(kfc_op* op, const double2* V, double2* W, int64_t m, int64_t ld
         // its epilogue, straight into W when nothing else is added
         const cd a = alpha * scale, b = -alpha * scale * shift;
         if (!use_beta && !d) return op_apply_promoted(op, V, W, m, ld, a, b, st);
+        // beta W rides in the epilogue too (y = a H x + b x + g A with A = W): each entry
+        // of W is read once, by the thread that then writes it.  The tuner replays its
+        // candidates on the same buffers, so the first application of a column count
+        // goes through scratch below and tunes; later ones run in this one pass.
+        const int slot = 2;   // the complex-on-real tables' tuned width
+        if (!d && (op->stencil_pinned || op->tuned_cols[slot] == m))
+            return op_apply_promoted(op, V, W, m, ld, a, b, st, W, beta);
         KFC_HIP(op->ctx, op->evo[0].ensure(sizeof(double2) * size_t(ld) * m));
         double2* Y = op->evo[0].as<double2>();
         int rc = op_apply_promoted(op, V, Y, m, ld, a, b, st);

@@ -292,10 +292,12 @@ int op_precond_diag_dev(kfc_op* op, bool jacobi, DevBuf& dD);
 int op_shift_inverse_dev(kfc_op* op, const double* D, double sigma, double* out);
 int op_ensure_device(kfc_op* op, int prec);
 int op_apply(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, int prec, hipStream_t st);
-// Y = alpha H X + beta X for complex128 blocks and a real operator on the fused stencil
-// path (op->plan.ok), the stencil running on complex tables (op->ddz).
+// Y = alpha H X + beta X (+ gamma A) for complex128 blocks and a real operator on the
+// fused stencil path (op->plan.ok), the stencil running on complex tables (op->ddz).  A
+// may be Y itself: every stencil variant reads A only at the entry it then writes.
 int op_apply_promoted(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, cd alpha, cd beta,
-                      hipStream_t st);
+                      hipStream_t st, const void* A = nullptr,
+                      cd gamma = cd(0));
 // The argument checks of kfc_apply (precision, sizes, aliasing); a KFC_PREC_Z64 precision
 // on a complex operator becomes KFC_PREC_F64, on a real one sets real_on_complex.
 int check_apply_args(kfc_op* op, const void* X, const void* Y, int64_t ncols, int64_t ld, int32_t& precision,
