@@ -726,8 +726,16 @@ constexpr int SA_D = 2;
 // and the weights' bookkeeping leave the plane loop; the three partial sums then live
 // in three slots indexed by output plane mod 3 (the loop is unrolled three planes deep),
 // so no register moves rotate them from plane to plane.
-template <class S, bool DP, bool ASYNC, bool DOT, bool PAIR = false, uint32_t MASK = 0>
+// RW (complex blocks on a real operator's promoted tables, whose weights and diagonal
+// have zero imaginary parts): each position is a real weight times a complex value, two
+// FMAs instead of a complex product's four.
+template <class S, bool DP, bool ASYNC, bool DOT, bool PAIR = false, uint32_t MASK = 0, bool RW = false>
 __device__ inline void stencil_sweep(const StArgs<S>& a) {
+    static_assert(!RW || (IsCx<S>::value && MASK == 0), "real weights are for complex blocks on the plane sweep");
+    auto wfma = [](S acc, S w, S v) -> S {
+        if constexpr (RW) return S{s_fma(acc.re, w.re, v.re), s_fma(acc.im, w.re, v.im)};
+        else return s_fma(acc, w, v);
+    };
     static_assert(!DOT || !IsCx<S>::value, "the fused dots are for real blocks");
     static_assert(!PAIR || (!IsCx<S>::value && !ASYNC), "paired loads are for the real plane sweep");
     static_assert(MASK == 0 || PAIR, "compile-time masks are for the paired plane sweep");
@@ -1061,17 +1069,17 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
             if ((a.used >> q) & 1u) {
                 const S w = a.c[q];
 #pragma unroll
-                for (int r = 0; r < SP_R; ++r) accP[r] = s_fma(accP[r], w, v[r + dy][dx]);
+                for (int r = 0; r < SP_R; ++r) accP[r] = wfma(accP[r], w, v[r + dy][dx]);
             }
             if ((a.used >> (9 + q)) & 1u) {
                 const S w = a.c[9 + q];
 #pragma unroll
-                for (int r = 0; r < SP_R; ++r) acc0[r] = s_fma(acc0[r], w, v[r + dy][dx]);
+                for (int r = 0; r < SP_R; ++r) acc0[r] = wfma(acc0[r], w, v[r + dy][dx]);
             }
             if ((a.used >> (18 + q)) & 1u) {
                 const S w = a.c[18 + q];
 #pragma unroll
-                for (int r = 0; r < SP_R; ++r) accM[r] = s_fma(accM[r], w, v[r + dy][dx]);
+                for (int r = 0; r < SP_R; ++r) accM[r] = wfma(accM[r], w, v[r + dy][dx]);
             }
         }
         // the diagonal of output zp
@@ -1094,7 +1102,7 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
                 const int y = y0 + r0 + r;
                 if (a.Dfull && xin && y < a.d1) dv = s_add(dv, a.Dfull[(int64_t)i0 + (int64_t)y * a.s1 + (int64_t)zp * a.s2]);
                 ctr0[r] = v[r + 1][1];
-                acc0[r] = s_fma(acc0[r], dv, ctr0[r]);
+                acc0[r] = wfma(acc0[r], dv, ctr0[r]);
             }
         }
         emit(zp, accM, ctrM);   // output zp - 1 is complete
@@ -1158,9 +1166,9 @@ __device__ inline void stencil_sweep(const StArgs<S>& a) {
     }
 }
 
-template <class S, bool DP, bool DOT>
+template <class S, bool DP, bool DOT, bool RW = false>
 __global__ __launch_bounds__(256) void k_stencil_p(const StArgs<S> a) {
-    stencil_sweep<S, DP, false, DOT>(a);
+    stencil_sweep<S, DP, false, DOT, false, 0, RW>(a);
 }
 
 template <class S, bool DP, bool DOT>
@@ -1327,7 +1335,15 @@ static int launch_stencil(kfc_op* op, const void* X, const void* A, void* Y, int
             return KFC_OK;
         }
     }
-    if (variant == 2) {
+    // complex blocks on a real operator: the promoted tables' weights are real
+    bool real_weights = false;
+    if constexpr (IsCx<S>::value) real_weights = !op->cplx;
+    if (variant == 2 && real_weights) {
+        if constexpr (IsCx<S>::value) {
+            if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((k_stencil_p<S, false, false, true>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
+        }
+    } else if (variant == 2) {
         if (a.ndp > 0) hipLaunchKernelGGL((k_stencil_p<S, true, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_stencil_p<S, false, false>), dim3(unsigned(nblk)), dim3(256), 0, st, a);
     } else if (variant == 3) {
