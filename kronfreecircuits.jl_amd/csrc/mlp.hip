@@ -1,5 +1,5 @@
 // The warm-start network: a GELU MLP with BF16 GEMM operands on MFMA
-// (v_mfma_f32_16x16x16bf16_1k) and FP32 accumulation, FP32 master weights and
+// (gfx950's v_mfma_f32_16x16x32_bf16) and FP32 accumulation, FP32 master weights and
 // Adam on the device.  It maps circuit parameters to the coefficients of an
 // initial LOBPCG block (no reference counterpart; it feeds the X0 argument of
 // the LOBPCG solver, src/lobpcg.jl:217).
@@ -12,7 +12,8 @@
 namespace {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
-typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 __device__ inline unsigned short f2bf(float x) {
     // round to nearest even
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A, 
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f4v{0, 0, 0, 0};
-    const int r16 = lane & 15, kq = (lane >> 4) * 4;
+    const int r16 = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
     for (int k0 = 0; k0 < K; k0 += 32) {
         // stage 64 x 32 of A and B as BF16
         for (int e = tid; e < 64 * 32; e += 256) {
@@ -67,26 +68,28 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A, 
             sB[mm][kk] = f2bf(bv);
         }
         __syncthreads();
+        // the whole 32-wide K tile in one 16 x 16 x 32 step: lane group g = lane / 16
+        // supplies k = 8 g .. 8 g + 7 of both operands (the same k in the same slot of
+        // A and B, so the products pair up whatever order the unit sums them in)
+        s8v a[2], b[2];
 #pragma unroll
-        for (int ks = 0; ks < 32; ks += 16) {
-            s4v a[2], b[2];
+        for (int i = 0; i < 2; ++i) {
+            const int row = wm + i * 16 + r16;
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int row = wm + i * 16 + r16;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) a[i][j] = short(sA[row][ks + kq + j]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int coln = wn + i * 16 + r16;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) b[i][j] = short(sB[coln][ks + kq + j]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 8; ++j) a[i][j] = short(sA[row][k8 + j]);
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int coln = wn + i * 16 + r16;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[i][j] = short(sB[coln][k8 + j]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a[i]),
+                                                                    __builtin_bit_cast(bf8v, b[j]), acc[i][j], 0, 0, 0);
         __syncthreads();
     }
 #pragma unroll
