@@ -1238,7 +1238,9 @@ static bool op_wants_pairz(const kfc_op* op) {
 // op splits or when the allocation fails (the groups then run whole).
 static void* pairz_for(kfc_op* op, size_t bytes) {
     if (!op_wants_pairz(op)) return nullptr;
-    return op->ctx->pairz.ensure(bytes) == hipSuccess ? op->ctx->pairz.p : nullptr;
+    if (op->ctx->pairz.ensure(bytes) == hipSuccess) return op->ctx->pairz.p;
+    (void)hipGetLastError();   // a failed allocation must not fail the launches after it
+    return nullptr;
 }
 
 // z: N x ncols scalars of scratch for the split form (null: every group runs whole).
@@ -1608,8 +1610,8 @@ void pairs_fill_table(kfc_op* const* ops, int n, const void* const* X, void* con
 }
 
 int pairs_reserve(kfc_op* op, int n, int64_t ncols, int prec) {
-    const size_t bytes = size_t(op->N) * size_t(ncols) * size_t(n) * (prec == 0 ? sizeof(double) : sizeof(float));
-    if (op_wants_pairz(op)) KFC_HIP(op->ctx, op->ctx->pairz.ensure(bytes));
+    // when the device cannot hold it, the tables get no scratch and every group runs whole
+    (void)pairz_for(op, size_t(op->N) * size_t(ncols) * size_t(n) * (prec == 0 ? sizeof(double) : sizeof(float)));
     return KFC_OK;
 }
 

@@ -238,7 +238,8 @@ template <class T>
 int launch_pairs(kfc_op* op, const void* X, void* Y, int64_t ncols, int64_t ld, hipStream_t st, bool scratch = true);
 // Grows the context's scratch for n problems of ncols columns at precision prec (0: FP64)
 // when a group of op can run in the split form; a graph that captures pair launches
-// reserves it first, since filling a table allocates nothing.
+// reserves it first, since filling a table allocates nothing.  If the device cannot hold
+// it, the groups run whole (KFC_OK either way).
 int pairs_reserve(kfc_op* op, int n, int64_t ncols, int prec);
 // Many operators of one pair structure in one launch per group (dense_pair.hip):
 // problem q maps X[q] -> Y[q]; host_tab (pinned) and dev_tab hold n x groups
