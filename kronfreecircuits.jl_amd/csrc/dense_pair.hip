@@ -971,6 +971,17 @@ __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep
                 const int jp = 4 * ks + k4;
                 b[ks] = (rok && jp < d) ? in[base + (int64_t)jp * rows] : T(0);
             }
+            // MODE 1: the output's old value, loaded with the operands so that its
+            // latency overlaps the products
+            T pre[NTJ][4];
+#pragma unroll
+            for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int j = 16 * tj + out_row(T(0), k4, v);
+                    const int64_t e = (int64_t)j * rows;
+                    pre[tj][v] = (MODE == 1 && rok && j < d) ? out[base + e] : T(0);
+                }
             v4 acc[NTJ];
 #pragma unroll
             for (int tj = 0; tj < NTJ; ++tj) acc[tj] = v4{0, 0, 0, 0};
@@ -988,7 +999,7 @@ __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep
                     if (rok && j < d) {
                         const int64_t e = (int64_t)j * rows;
                         T r = acc[tj][v];
-                        if constexpr (MODE == 1) r += out[base + e];
+                        if constexpr (MODE == 1) r += pre[tj][v];
                         if constexpr (MODE == 2) r += a.D[cb + e] * a.X[base + e];
                         out[base + e] = r;
                     }
@@ -1018,6 +1029,15 @@ __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep
             const int jp = 4 * ks + k4;
             b[ks] = (rok && jp < d) ? in[base + row + (int64_t)jp * rows] : T(0);
         }
+        T pre[NTJ][4];
+#pragma unroll
+        for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * tj + out_row(T(0), k4, v);
+                const int64_t e = row + (int64_t)j * rows;
+                pre[tj][v] = (MODE == 1 && rok && j < d) ? out[base + e] : T(0);
+            }
         v4 acc[NTJ];
 #pragma unroll
         for (int tj = 0; tj < NTJ; ++tj) acc[tj] = v4{0, 0, 0, 0};
@@ -1035,7 +1055,7 @@ __device__ inline void pair_mode(const PairArgs<T>& a, int64_t w0, int64_t wstep
                 if (rok && j < d) {
                     const int64_t e = row + (int64_t)j * rows;
                     T r = acc[tj][v];
-                    if constexpr (MODE == 1) r += out[base + e];
+                    if constexpr (MODE == 1) r += pre[tj][v];
                     if constexpr (MODE == 2) r += a.D[cb + e] * a.X[base + e];
                     out[base + e] = r;
                 }
