@@ -395,11 +395,50 @@ class Team {
         cv_.notify_all();
         for (auto& t : th_) t.join();
     }
+    int size() const { return int(th_.size()); }
     void run(int n, const std::function<void(int)>& fn) {
         if (th_.empty() || n < 2) {
             for (int i = 0; i < n; ++i) fn(i);
             return;
         }
+        start(n, fn);
+        finish();
+    }
+    // fn(0 .. n-1) as run() does (tasks are taken in index order); as each consecutive
+    // group of `chunk` tasks finishes, the calling thread runs group(lo, hi) for it, in
+    // group order, while the team works on later tasks.  The first nonzero return of
+    // group() is returned once every task has finished (later groups are not handed on).
+    int run_groups(int n, int chunk, const std::function<void(int)>& fn, const std::function<int(int, int)>& group) {
+        chunk = std::max(1, chunk);
+        const int ng = (n + chunk - 1) / chunk;
+        if (th_.empty() || n < 2 || ng < 2) {
+            run(n, fn);
+            return n > 0 ? group(0, n) : 0;
+        }
+        std::vector<std::atomic<int>> left(ng);
+        for (int g = 0; g < ng; ++g) left[g].store(std::min(chunk, n - g * chunk));
+        const std::function<void(int)> task = [&](int i) {
+            fn(i);
+            if (left[i / chunk].fetch_sub(1) == 1) {
+                std::lock_guard<std::mutex> g(mu_);
+                group_.notify_one();
+            }
+        };
+        start(n, task);
+        int rc = 0;
+        for (int g = 0; g < ng; ++g) {
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                group_.wait(l, [&] { return left[g].load() == 0; });
+            }
+            if (rc == 0) rc = group(g * chunk, std::min(n, (g + 1) * chunk));
+        }
+        finish();
+        return rc;
+    }
+
+  private:
+    void start(int n, const std::function<void(int)>& fn) {
         {
             std::lock_guard<std::mutex> g(mu_);
             fn_ = &fn;
@@ -409,12 +448,12 @@ class Team {
             ++gen_;
         }
         cv_.notify_all();
+    }
+    void finish() {
         std::unique_lock<std::mutex> g(mu_);
         done_.wait(g, [this] { return busy_ == 0; });
         fn_ = nullptr;
     }
-
-  private:
     void loop() {
         uint64_t seen = 0;
         for (;;) {
@@ -437,7 +476,7 @@ class Team {
     }
     std::vector<std::thread> th_;
     std::mutex mu_;
-    std::condition_variable cv_, done_;
+    std::condition_variable cv_, done_, group_;
     const std::function<void(int)>* fn_ = nullptr;
     int n_ = 0, busy_ = 0;
     uint64_t gen_ = 0;
@@ -896,7 +935,10 @@ struct Phase {
     }
 
     // Rayleigh-Ritz of each listed point on its first c columns, then the rotation.
-    // Points whose basis collapses are marked failed.
+    // Points whose basis collapses are marked failed.  The points go to the host team
+    // in groups of its size; each group's rotation is launched as soon as its
+    // Rayleigh-Ritz problems are solved, so the device rotates one group while the
+    // host solves the next (each point's rows are rotated as in one launch).
     int rr_rotate(const std::vector<int>& list, const std::vector<int>& cs) {
         const int n = int(list.size());
         std::vector<char> ok(n, 1);
@@ -906,54 +948,62 @@ struct Phase {
         const auto t0 = clk::now();
         n_pointiter += n;
         ++n_iter;
-        team.run(n, [&](int q) {
-            LsPoint& pt = pts[list[q]];
-            ok[q] = ls_rayleigh_ritz(cs[q], cfg.W, pt.nlock, std::is_same<T, float>::value, rec + size_t(q) * LS_E,
-                                     pt.theta, Mh + size_t(q) * LS_CMAX * LS_MO, npn[q]);
-        });
-        t_rr += secs(t0);
-        // rotate the points that succeeded
-        std::vector<int> rl;
-        std::vector<LsSlot> sl;
-        std::vector<int> src;   // index into Mh of each rotated point
-        for (int q = 0; q < n; ++q) {
-            LsPoint& pt = pts[list[q]];
-            if (!ok[q]) {
-                // an FP32 basis that collapses ends the point's FP32 phase (the FP64
-                // phase restarts from its X); in FP64 the point goes to the one-point path
-                if (switch_tol > 0) pt.done = true;
-                else pt.failed = true;
-                ++n_collapsed;
-                continue;
+        constexpr size_t MW = size_t(LS_CMAX) * LS_MO;   // doubles of one rotation matrix
+        int rdone = 0;   // rotated points so far: their matrices and slots are packed in front
+        auto rotate_group = [&](int lo, int hi) -> int {
+            std::vector<int> rl, src;
+            std::vector<LsSlot> sl;
+            for (int q = lo; q < hi; ++q) {
+                LsPoint& pt = pts[list[q]];
+                if (!ok[q]) {
+                    // an FP32 basis that collapses ends the point's FP32 phase (the FP64
+                    // phase restarts from its X); in FP64 the point goes to the one-point path
+                    if (switch_tol > 0) pt.done = true;
+                    else pt.failed = true;
+                    ++n_collapsed;
+                    continue;
+                }
+                rl.push_back(list[q]);
+                sl.push_back(LsSlot{list[q], pt.par, cs[q], cfg.W + npn[q]});
+                src.push_back(q);
             }
-            rl.push_back(list[q]);
-            sl.push_back(LsSlot{list[q], pt.par, cs[q], cfg.W + npn[q]});
-            src.push_back(q);
-        }
-        if (rl.empty()) return KFC_OK;
-        // compact the rotation matrices of the rotated points
-        for (size_t r = 0; r < src.size(); ++r)
-            if (int(r) != src[r])
-                std::copy(Mh + size_t(src[r]) * LS_CMAX * LS_MO, Mh + size_t(src[r] + 1) * LS_CMAX * LS_MO,
-                          Mh + r * LS_CMAX * LS_MO);
-        const int nr = int(rl.size());
-        KFC_HIP(ctx, hipMemcpyAsync(dM.p, hM.p, sizeof(double) * LS_CMAX * LS_MO * nr, hipMemcpyHostToDevice, st));
-        int rc = upload_slots(sl, 1);
-        if (rc != KFC_OK) return rc;
-        const LsArgs<T> a = args(nr);
-        // rows of 2048 per block with every point active, finer (>= 64 rows) for fewer points
-        const int f = std::max(1, std::min(16, int(pts.size()) / std::max(1, nr)));
-        const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>((N + 63) / 64, ((N + 2047) / 2048) * f)));
-        LsArgs<T> ar = a;
-        ar.slot = dslot.as<LsSlot>() + pts.size();
-        ar.rows_per = ((N + gx - 1) / gx + 63) / 64 * 64;
-        hipLaunchKernelGGL(k_ls_rot<T>, dim3(gx, nr, 2), dim3(256), 0, st, ar, dM.as<double>());
-        KFC_HIP(ctx, hipGetLastError());
-        for (size_t r = 0; r < rl.size(); ++r) {
-            pts[rl[r]].par ^= 1;
-            pts[rl[r]].np = npn[src[r]];
-        }
-        return KFC_OK;
+            if (rl.empty()) return KFC_OK;
+            // pack this group's matrices after the earlier groups' (rdone <= lo <= src[r],
+            // and the team writes only matrices at indices >= hi)
+            const int r0 = rdone, nr = int(rl.size());
+            for (int r = 0; r < nr; ++r)
+                if (r0 + r != src[r]) std::copy(Mh + size_t(src[r]) * MW, Mh + size_t(src[r] + 1) * MW, Mh + size_t(r0 + r) * MW);
+            rdone += nr;
+            KFC_HIP(ctx, hipMemcpyAsync(dM.as<double>() + size_t(r0) * MW, Mh + size_t(r0) * MW, sizeof(double) * MW * nr,
+                                        hipMemcpyHostToDevice, st));
+            LsSlot* hs = hslot.as<LsSlot>() + pts.size() + r0;
+            LsSlot* ds = dslot.as<LsSlot>() + pts.size() + r0;
+            std::copy(sl.begin(), sl.end(), hs);
+            KFC_HIP(ctx, hipMemcpyAsync(ds, hs, sizeof(LsSlot) * nr, hipMemcpyHostToDevice, st));
+            LsArgs<T> ar = args(nr);
+            // rows of 2048 per block with every point in the launch, finer (>= 64 rows) for fewer
+            const int f = std::max(1, std::min(16, int(pts.size()) / std::max(1, nr)));
+            const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>((N + 63) / 64, ((N + 2047) / 2048) * f)));
+            ar.slot = ds;
+            ar.rows_per = ((N + gx - 1) / gx + 63) / 64 * 64;
+            hipLaunchKernelGGL(k_ls_rot<T>, dim3(gx, nr, 2), dim3(256), 0, st, ar, dM.as<double>() + size_t(r0) * MW);
+            KFC_HIP(ctx, hipGetLastError());
+            for (int r = 0; r < nr; ++r) {
+                pts[rl[r]].par ^= 1;
+                pts[rl[r]].np = npn[src[r]];
+            }
+            return KFC_OK;
+        };
+        const int rc = team.run_groups(
+            n, std::max(1, team.size()),
+            [&](int q) {
+                LsPoint& pt = pts[list[q]];
+                ok[q] = ls_rayleigh_ritz(cs[q], cfg.W, pt.nlock, std::is_same<T, float>::value, rec + size_t(q) * LS_E,
+                                         pt.theta, Mh + size_t(q) * MW, npn[q]);
+            },
+            rotate_group);
+        t_rr += secs(t0);
+        return rc;
     }
 
     int run(int64_t* iters_out) {
