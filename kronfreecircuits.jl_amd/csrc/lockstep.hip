@@ -531,14 +531,17 @@ bool ls_rr_solve(const std::vector<int>& U, int c, int W, bool single, const std
     int k = 0;
     std::vector<double> B;
     std::vector<double> L = Gs;
+    // B upper triangular (the Cholesky path): the products below skip its zero half
+    bool upper = false;
     if (chol_lower(n, L, chol_floor)) {
         k = n;
-        // B = L^{-T}: back substitution on L^T B = I
+        upper = true;
+        // B = L^{-T}: back substitution on L^T B = I (B[i, j] = 0 for i > j)
         B.assign(size_t(n) * n, 0.0);
         for (int j = 0; j < n; ++j)
-            for (int i = n - 1; i >= 0; --i) {
+            for (int i = j; i >= 0; --i) {
                 double s = i == j ? 1.0 : 0.0;
-                for (int l = i + 1; l < n; ++l) s -= L[l + size_t(i) * n] * B[l + size_t(j) * n];
+                for (int l = i + 1; l <= j; ++l) s -= L[l + size_t(i) * n] * B[l + size_t(j) * n];
                 B[i + size_t(j) * n] = s / L[i + size_t(i) * n];
             }
     } else {
@@ -567,7 +570,8 @@ bool ls_rr_solve(const std::vector<int>& U, int c, int W, bool single, const std
     for (int j = 0; j < k; ++j)
         for (int i = 0; i <= j; ++i) {
             double s = 0;
-            for (int l = 0; l < n; ++l) s += B[l + size_t(i) * n] * KB[l + size_t(j) * n];
+            const int le = upper ? i + 1 : n;   // B[l, i] = 0 past the diagonal
+            for (int l = 0; l < le; ++l) s += B[l + size_t(i) * n] * KB[l + size_t(j) * n];
             Kp[i + size_t(j) * k] = Kp[j + size_t(i) * k] = s;
         }
     std::vector<double> lam, Y;
@@ -578,7 +582,7 @@ bool ls_rr_solve(const std::vector<int>& U, int c, int W, bool single, const std
     for (int j = 0; j < W; ++j)
         for (int i = 0; i < n; ++i) {
             double s = 0;
-            for (int l = 0; l < k; ++l) s += B[i + size_t(l) * n] * Y[l + size_t(j) * k];
+            for (int l = upper ? i : 0; l < k; ++l) s += B[i + size_t(l) * n] * Y[l + size_t(j) * k];   // B[i, l] = 0 for l < i
             C[U[i] + size_t(j) * c] = d[i] * s;
         }
     return true;
