@@ -128,10 +128,20 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const A* __restrict__ part,
 // FP32 blocks too.
 typedef double gd4 __attribute__((ext_vector_type(4)));
 
+// ntq > 0 (TP = TQ = 4): a Gram wider than 64 in 64 x 64 tiles, tile blockIdx.y = (ti,
+// tj) with ti = y / ntq, tj = y % ntq; block x of tile y writes its partial tile
+// (column-major, leading dimension the tile's rows) at part + (x * gridDim.y + y) * 4096.
 template <class S, int TP, int TQ>
 __global__ __launch_bounds__(256) void k_gram_mfma(const S* __restrict__ Am, int64_t lda, int p,
                                                    const S* __restrict__ Bm, int64_t ldb, int q, int64_t N,
-                                                   int64_t rows_per, double* __restrict__ part) {
+                                                   int64_t rows_per, double* __restrict__ part, int ntq) {
+    if (ntq > 0) {
+        const int ti = int(blockIdx.y) / ntq, tj = int(blockIdx.y) % ntq;
+        Am += int64_t(64) * ti * lda;
+        Bm += int64_t(64) * tj * ldb;
+        p = min(64, p - 64 * ti);
+        q = min(64, q - 64 * tj);
+    }
     constexpr int LDR = 65;   // odd row stride (in doubles) against bank conflicts
     __shared__ double smem[(TP + TQ) * 16 * LDR];
     double* sA = smem;
@@ -200,7 +210,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const S* __restrict__ Am, int
         }
         __syncthreads();
     }
-    double* out = part + (int64_t)blockIdx.x * p * q;
+    double* out = ntq > 0 ? part + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) * 4096
+                          : part + (int64_t)blockIdx.x * p * q;
     for (int e = tid; e < TP * TQ * 256; e += 256) {
         const int tile = e >> 8, rc = e & 255, i = tile / TQ, j = tile % TQ;
         const int gi = 16 * i + (rc >> 4), gj = 16 * j + (rc & 15);
@@ -212,7 +223,7 @@ template <class S, int TP, int TQ>
 void launch_gram_mfma(int nchunk, hipStream_t st, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q,
                       int64_t N, int64_t rows_per, double* part) {
     hipLaunchKernelGGL((k_gram_mfma<S, TP, TQ>), dim3(nchunk), dim3(256), 0, st, A, lda, p, B, ldb, q, N, rows_per,
-                       part);
+                       part, 0);
 }
 
 template <class S, int TP>
@@ -500,11 +511,46 @@ int tsmm_chunks(const Work& w, int gr, int tq, int tpz, int ppad, size_t shmem, 
     return int((N + rows_per - 1) / rows_per);
 }
 
+// A real Gram wider than 64 columns on either side: all its 64 x 64 tiles in one
+// k_gram_mfma launch, the chunks' partial tiles summed by one reduction.
+template <class S>
+int gram_tiled(Work& w, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q, int64_t N,
+               std::vector<cd>& G) {
+    if (N <= 0) return KFC_OK;   // G is zero
+    const int ntp = (p + 63) / 64, ntq = (q + 63) / 64, nt = ntp * ntq;
+    // about a thousand blocks over all tiles, whole 64-row slabs each
+    const int64_t slabs = std::max<int64_t>(1, (N + 63) / 64);
+    const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(slabs, (1024 + nt - 1) / nt));
+    const int64_t rows_per = (slabs + nch - 1) / nch * 64;
+    const int nchunk = int((N + rows_per - 1) / rows_per);
+    KFC_HIP(w.ctx, w.part.ensure(sizeof(double) * size_t(nchunk) * nt * 4096));
+    KFC_HIP(w.ctx, w.small.ensure(sizeof(double) * size_t(nt) * 4096));
+    hipLaunchKernelGGL((k_gram_mfma<S, 4, 4>), dim3(nchunk, nt), dim3(256), 0, w.ctx->stream, A, lda, p, B, ldb, q, N,
+                       rows_per, w.part.as<double>(), ntq);
+    KFC_HIP(w.ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_gram_reduce<double>, dim3(unsigned(nt * 4096)), dim3(256), 0, w.ctx->stream,
+                       w.part.as<double>(), nchunk, int64_t(nt) * 4096, w.small.as<double>());
+    KFC_HIP(w.ctx, hipGetLastError());
+    std::vector<double> h(size_t(nt) * 4096);
+    KFC_HIP(w.ctx, hipMemcpyAsync(h.data(), w.small.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, w.ctx->stream));
+    KFC_HIP(w.ctx, hipStreamSynchronize(w.ctx->stream));
+    for (int t = 0; t < nt; ++t) {
+        const int ti = t / ntq, tj = t % ntq, pt = std::min(64, p - 64 * ti), qt = std::min(64, q - 64 * tj);
+        for (int j = 0; j < qt; ++j)
+            for (int i = 0; i < pt; ++i)
+                G[(64 * ti + i) + size_t(64 * tj + j) * p] = cd(h[size_t(t) * 4096 + i + size_t(j) * pt], 0.0);
+    }
+    return KFC_OK;
+}
+
 template <class S>
 int blas_gram(Work& w, const S* A, int64_t lda, int p, const S* B, int64_t ldb, int q, int64_t N, std::vector<cd>& G) {
     using Acc = typename AccOf<S>::type;
     G.assign(size_t(p) * q, cd(0));
     if (p == 0 || q == 0) return KFC_OK;
+    if constexpr (!IsCx<S>::value) {
+        if (p > 64 || q > 64) return gram_tiled<S>(w, A, lda, p, B, ldb, q, N, G);
+    }
     const int ntp = (p + 63) / 64, ntq = (q + 63) / 64;
     // enough row chunks to fill the chip several times over
     const int nchunk = nchunks_for(N, 4096, std::max(64, 2048 / (ntp * ntq)));

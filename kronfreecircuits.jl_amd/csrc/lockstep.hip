@@ -738,6 +738,9 @@ struct Phase {
     std::map<int, hipGraphExec_t> gexec = {};
     std::set<int> gseen = {};
     int64_t n_graph_steps = 0, n_captures = 0;
+    // host time capturing and instantiating step graphs, in the first Rayleigh-Ritz,
+    // and in the steps run as separate launches (their waits included)
+    double t_capture = 0, t_init = 0, t_eager = 0;
     bool apply_untimed = false;    // some applies ran inside graphs (no events there)
 
     ~Phase() {
@@ -911,6 +914,7 @@ struct Phase {
                 gseen.insert(nact);
                 return KFC_OK;
             }
+            const auto tc = clk::now();
             KFC_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const int rc = enqueue_step(act, pth, D, pmode);
             hipGraph_t g = nullptr;
@@ -927,6 +931,7 @@ struct Phase {
                 return kfc_fail(ctx, KFC_EDEVICE, std::string("lockstep step instantiate: ") + hipGetErrorString(ei));
             it = gexec.emplace(nact, x).first;
             ++n_captures;
+            t_capture += secs(tc);
         }
         KFC_HIP(ctx, hipGraphLaunch(it->second, st));
         const auto t0 = clk::now();
@@ -1022,6 +1027,7 @@ struct Phase {
         }
         if (act.empty()) return KFC_OK;
         // initial Rayleigh-Ritz of the start blocks: AX, G = X^T X, K = X^T A X
+        const auto ti = clk::now();
         shape_for(int(act.size()));
         int rc = apply(act, 0, W, 0);
         if (rc != KFC_OK) return rc;
@@ -1039,6 +1045,7 @@ struct Phase {
             rc = rr_rotate(act, std::vector<int>(act.size(), W));
             if (rc != KFC_OK) return rc;
         }
+        t_init += secs(ti);
         bool first_prep = true;
         for (int64_t it = 0;; ++it) {
             act.clear();
@@ -1069,6 +1076,7 @@ struct Phase {
                     for (int p : act) pts[p].matvec -= W;
                 }
             }
+            const auto te = clk::now();
             if (!graphed) {
             rc = upload_theta();
             if (rc == KFC_OK) rc = upload_slots(sl);
@@ -1097,6 +1105,7 @@ struct Phase {
             if (rc != KFC_OK) return rc;
             rc = read_apply_time();
             if (rc != KFC_OK) return rc;
+            t_eager += secs(te);
             }
             // convergence of every column on the residuals of the current X
             std::vector<int> go;
@@ -1442,9 +1451,9 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
         if (rc != KFC_OK) return rc;
         untimed |= ph.apply_untimed;
         if (trace)
-            std::fprintf(stderr, "[lockstep] fp32 phase: %lld steps (%lld from %lld graphs), %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, %.3f s\n",
+            std::fprintf(stderr, "[lockstep] fp32 phase: %lld steps (%lld from %lld graphs), %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, first rayleigh-ritz %.3f s, launched steps %.3f s, captures %.3f s, %.3f s\n",
                          (long long)ph.n_iter, (long long)ph.n_graph_steps, (long long)ph.n_captures,
-                         (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, secs(t0));
+                         (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, ph.t_init, ph.t_eager, ph.t_capture, secs(t0));
         // FP64 continuation from each point's FP32 X
         for (int i = 0; i < n; ++i) {
             it32[i] = pts[i].it;
@@ -1470,9 +1479,9 @@ int lockstep_run(int n, kfc_op* const* ops, int64_t m, int64_t guard, double tol
         if (rc != KFC_OK) return rc;
         untimed |= ph.apply_untimed;
         if (trace)
-            std::fprintf(stderr, "[lockstep] fp64 phase: %lld steps (%lld from %lld graphs), %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, %.3f s\n",
+            std::fprintf(stderr, "[lockstep] fp64 phase: %lld steps (%lld from %lld graphs), %lld point-steps, %lld collapsed, wait %.3f s, rayleigh-ritz %.3f s, first rayleigh-ritz %.3f s, launched steps %.3f s, captures %.3f s, %.3f s\n",
                          (long long)ph.n_iter, (long long)ph.n_graph_steps, (long long)ph.n_captures,
-                         (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, secs(t0));
+                         (long long)ph.n_pointiter, (long long)ph.n_collapsed, ph.t_wait, ph.t_rr, ph.t_init, ph.t_eager, ph.t_capture, secs(t0));
         // fresh FP64 residuals of the returned m pairs: A[par ^ 1] X slot = H X
         std::vector<int> fin;
         for (int i = 0; i < n; ++i)
