@@ -1264,13 +1264,40 @@ int subspace_starts(kfc_op* const* ops, int n, int W, const SubspaceSeed& sub, d
             return kfc_fail(ctx, KFC_EBREAKDOWN, "subspace start of point " + std::to_string(i) +
                                                      ": the subspace holds fewer than " + std::to_string(W) +
                                                      " independent directions");
-    for (int i = 0; i < n; ++i) {
-        int rc = subspace_block(w, sub, i, N, BC, map, X.as<double>());
-        std::vector<cd> M(Y[i].size());
-        for (size_t k = 0; k < M.size(); ++k) M[k] = Y[i][k];
-        if (rc == KFC_OK)
-            rc = blas_gemm<double>(w, X.as<double>(), N, ns, M, W, S0 + size_t(i) * 3 * W * N, N, N, 0.0);
-        if (rc != KFC_OK) return kfc_fail(ctx, rc, "subspace start of point " + std::to_string(i) + ": " + ops[i]->ctx->err);
+    if (!sub.C) {
+        // S_i Y_i = scatter_i(B Y_i), S_i's rows being B's rows placed by the point's row
+        // map (zero elsewhere): one product of B with every point's Y_i, then W columns
+        // scattered per point (the same sums as S_i Y_i, row by row)
+        std::vector<cd> M(size_t(ns) * W * n);
+        for (int i = 0; i < n; ++i)
+            for (size_t k = 0; k < size_t(ns) * W; ++k) M[size_t(i) * ns * W + k] = Y[i][k];
+        DevBuf Z;
+        KFC_HIP(ctx, Z.ensure(sizeof(double) * size_t(sub.nf) * W * n));
+        int rc = blas_gemm<double>(w, sub.B, sub.nf, ns, M, W * n, Z.as<double>(), sub.nf, sub.nf, 0.0);
+        if (rc != KFC_OK) return kfc_fail(ctx, rc, "subspace starts: " + ctx->err);
+        KFC_HIP(ctx, map.ensure(sizeof(int64_t) * size_t(sub.nf)));
+        const int64_t tot = sub.nf * W;
+        const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>((tot + 255) / 256, 8192)));
+        for (int i = 0; i < n; ++i) {
+            double* Si = S0 + size_t(i) * 3 * W * N;
+            KFC_HIP(ctx, hipMemcpyAsync(map.p, sub.rowmaps[i], sizeof(int64_t) * size_t(sub.nf), hipMemcpyHostToDevice,
+                                        ctx->stream));
+            KFC_HIP(ctx, hipMemsetAsync(Si, 0, sizeof(double) * size_t(N) * W, ctx->stream));
+            hipLaunchKernelGGL(k_frame_scatter, dim3(gx), dim3(256), 0, ctx->stream, Z.as<double>() + size_t(i) * W * sub.nf,
+                               sub.nf, sub.nf, W, map.as<int64_t>(), Si, N);
+            KFC_HIP(ctx, hipGetLastError());
+        }
+        KFC_HIP(ctx, hipStreamSynchronize(ctx->stream));   // Z is freed on return
+    } else {
+        for (int i = 0; i < n; ++i) {
+            int rc = subspace_block(w, sub, i, N, BC, map, X.as<double>());
+            std::vector<cd> M(Y[i].size());
+            for (size_t k = 0; k < M.size(); ++k) M[k] = Y[i][k];
+            if (rc == KFC_OK)
+                rc = blas_gemm<double>(w, X.as<double>(), N, ns, M, W, S0 + size_t(i) * 3 * W * N, N, N, 0.0);
+            if (rc != KFC_OK)
+                return kfc_fail(ctx, rc, "subspace start of point " + std::to_string(i) + ": " + ops[i]->ctx->err);
+        }
     }
     if (trace)
         std::fprintf(stderr, "[lockstep] subspace starts: projections %.3f s, Rayleigh-Ritz %.3f s, blocks %.3f s\n", t_dev,
