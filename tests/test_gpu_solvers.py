@@ -73,10 +73,12 @@ def test_lobpcg_zero_pi_tree_small(mode, precond):
     assert np.max(r) < 1e-8
 
 
-def test_lobpcg_wide_block_vs_dense():
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+def test_lobpcg_wide_block_vs_dense(mode):
     """m = 24 (block [X R P] of about 3 (m + guard) > 64 columns): its Grams run as
-    64 x 64 tiles on the matrix cores, the last tiles ragged.  Eigenvalues and true
-    residuals against the dense spectrum of the oracle's operator."""
+    64 x 64 tiles on the matrix cores, the last tiles ragged (FP32 blocks in the mixed
+    mode's first phase).  Eigenvalues and true residuals against the dense spectrum of
+    the oracle's operator."""
     import kfc
     from kfc import configs
     th = configs.three_fluxonia_theta(3)
@@ -85,7 +87,7 @@ def test_lobpcg_wide_block_vs_dense():
     w = np.linalg.eigvalsh(O.to_matrix(Ho))
     m = 24
     assert 3 * (m + kfc.solvers.guard_count(kfc.LOBPCG(), H.N, m)) > 64
-    E, V, st = kfc.solve_stats(kfc.LOBPCG(tol=1e-8), H, m)
+    E, V, st = kfc.solve_stats(kfc.LOBPCG(tol=1e-8, mode=mode), H, m)
     assert np.max(np.abs(E - w[:m])) < 1e-9 * max(1.0, np.max(np.abs(w[:m])))
     assert np.max(O.true_residuals(Ho, E, V)) < 1e-7
     assert np.allclose(V.T @ V, np.eye(m), atol=1e-9)
